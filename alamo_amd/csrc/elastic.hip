@@ -1,0 +1,882 @@
+// elastic.hip -- libalamo_elastic.so: C-ABI, operator storage and the
+// multigrid driver of the nodal elastic solve on one MI355X.
+//
+// The driver restates the amrex::MLMG control flow that
+// Solver::Nonlocal::Linear configures (src/Solver/Nonlocal/Linear.H:255-290):
+// F-cycle every iteration (setMaxFmgIter(1e8)), nu1 = nu2 = 2 Fsmooth calls,
+// BiCGStab bottom solve, inf-norm stopping test against
+// max(tol_abs, tol_rel * max(|b|, |r0|)).  All field work runs in HIP kernels
+// on the caller's stream; the host only reads one residual norm per cycle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/alamo_elastic.h"
+#include "bottom_kernels.h"
+#include "elastic_kernels.h"
+
+using ael::kBlock;
+using ael::Lvl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ALAMO_EL_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+// grid of the XCD-remapped stencil kernels: a multiple of 8 blocks
+inline unsigned nblk8(long n) { return (unsigned)((nblk(n) + ael::kXcd - 1) / ael::kXcd * ael::kXcd); }
+
+int ncomp_of(int sym) { return sym == 0 ? 2 : (sym == 1 ? 21 : (sym == 2 ? 9 : -1)); }
+
+template <typename T>
+struct Vecs {
+    T* C = nullptr;
+    T* diag = nullptr;
+    T* cor = nullptr;
+    T* cor_hold = nullptr;
+    T* res = nullptr;
+    T* rescor = nullptr;
+    T* tmp = nullptr;
+};
+
+struct Level {
+    Lvl L;
+    Vecs<double> d;
+    Vecs<float> f;
+};
+
+template <typename T> Vecs<T>& vecs(Level& l);
+template <> Vecs<double>& vecs<double>(Level& l) { return l.d; }
+template <> Vecs<float>& vecs<float>(Level& l) { return l.f; }
+
+}  // namespace
+
+struct alamo_el_s {
+    int sym = 0;
+    int nc = 0;
+    std::vector<Level> lv;
+    int* bct_dev = nullptr;
+    double* bcv_dev = nullptr;
+    int bct_host[78];
+    double bcv_host[78];
+    int uniform = 0;
+    double omega = 2.0 / 3.0;
+    bool model_set = false, prepared = false, have32 = false, prepared32 = false;
+    double* red_partial = nullptr;
+    double* red_out = nullptr;
+    double* red_host = nullptr;  // pinned
+    double* red_host_vec = nullptr;  // pinned, FGMRES coefficients
+    double* bottom_work = nullptr;
+    int* bottom_status = nullptr;
+    double* gu9 = nullptr;  // Newton scratch, allocated on demand
+    double* dw9 = nullptr;
+    double* csoa = nullptr;
+    // dense coarsest level (when it has at most kDenseMax unknowns)
+    int dense_M = 0;
+    double* dense_I = nullptr;     // identity, the unit vectors of the assembly
+    double* dense_A = nullptr;     // operator, column-major
+    double* dense_Ainv = nullptr;  // inverse, column-major (null when singular)
+    // FGMRES storage, allocated on demand
+    double* kh = nullptr;          // Gram-Schmidt coefficients on the device
+    double* kpart = nullptr;       // multi-dot partials
+    int krylov_m = 0;
+    double* kv = nullptr;  // (m+1) vectors
+    double* kz = nullptr;  // m vectors
+    double* kw = nullptr;
+};
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+constexpr int kDotBlocks = 512;
+constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M doubles
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch
+// ---------------------------------------------------------------------------
+template <typename T>
+void launch_fapply(alamo_el_t h, const Level& l, const T* u, const T* C, T* f, hipStream_t s) {
+    const Lvl L = l.L;
+    switch (h->sym) {
+        case 0: ael::k_fapply<T, 0><<<nblk8(L.N), kBlock, 0, s>>>(u, C, h->bct_dev, h->uniform, L, f); break;
+        case 1: ael::k_fapply<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(u, C, h->bct_dev, h->uniform, L, f); break;
+        default: ael::k_fapply<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(u, C, h->bct_dev, h->uniform, L, f); break;
+    }
+}
+
+template <typename T>
+void launch_residual(alamo_el_t h, const Level& l, const T* x, const T* b, const T* C, T* r, hipStream_t s) {
+    const Lvl L = l.L;
+    switch (h->sym) {
+        case 0: ael::k_residual<T, 0><<<nblk8(L.N), kBlock, 0, s>>>(x, b, C, h->bct_dev, h->uniform, L, r); break;
+        case 1: ael::k_residual<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(x, b, C, h->bct_dev, h->uniform, L, r); break;
+        default: ael::k_residual<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(x, b, C, h->bct_dev, h->uniform, L, r); break;
+    }
+}
+
+template <typename T>
+void launch_jacobi(alamo_el_t h, const Level& l, const T* x, const T* b, const T* diag, const T* C, T* xo,
+                   hipStream_t s) {
+    const Lvl L = l.L;
+    const T w = (T)h->omega;
+    switch (h->sym) {
+        case 0: ael::k_jacobi<T, 0><<<nblk8(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, h->uniform, L, w, xo); break;
+        case 1: ael::k_jacobi<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, h->uniform, L, w, xo); break;
+        default: ael::k_jacobi<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, h->uniform, L, w, xo); break;
+    }
+}
+
+template <typename T>
+void launch_diagonal(alamo_el_t h, const Level& l, const T* C, T* diag, hipStream_t s) {
+    const Lvl L = l.L;
+    switch (h->sym) {
+        case 0: ael::k_diagonal<T, 0><<<nblk8(L.N), kBlock, 0, s>>>(C, h->bct_dev, L, diag); break;
+        case 1: ael::k_diagonal<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(C, h->bct_dev, L, diag); break;
+        default: ael::k_diagonal<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(C, h->bct_dev, L, diag); break;
+    }
+}
+
+template <typename T>
+void launch_fill(T* p, T v, long n, hipStream_t s) {
+    if (n > 0) ael::k_fill<T><<<nblk(n), kBlock, 0, s>>>(p, v, n);
+}
+
+template <typename T>
+void launch_copy(T* dst, const T* src, long n, hipStream_t s) {
+    if (n > 0) ael::k_cast<T, T><<<nblk(n), kBlock, 0, s>>>(src, dst, n);
+}
+
+template <typename T>
+void launch_axpy(T* y, T a, const T* x, long n, hipStream_t s) {
+    if (n > 0) ael::k_axpy<T><<<nblk(n), kBlock, 0, s>>>(y, a, x, n);
+}
+
+// |v|_inf (op 0) or a.b (op 1) to the host.
+template <typename T>
+int reduce_host(alamo_el_t h, const T* a, const T* b, long n, int op, double* out, hipStream_t s) {
+    const int nb = (int)std::min<long>(kRedBlocks, std::max<long>(1, nblk(n)));
+    ael::k_reduce_partial<T><<<nb, kBlock, 0, s>>>(a, b, n, op, h->red_partial);
+    ael::k_reduce_final<<<1, kBlock, 0, s>>>(h->red_partial, nb, op, h->red_out);
+    HIPCHK(hipMemcpyAsync(h->red_host, h->red_out, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *out = h->red_host[0];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid cycles, templated on the working precision
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Cycle {
+    alamo_el_t h;
+    hipStream_t s;
+    int nu1, nu2, nuf, bottom_max_iter;
+    double bottom_tol;
+    int bottom_solver = ALAMO_EL_BOTTOM_BICGSTAB;
+
+    int nlev() const { return (int)h->lv.size(); }
+    Level& L(int l) { return h->lv[l]; }
+    Vecs<T>& V(int l) { return vecs<T>(h->lv[l]); }
+    long M(int l) { return 3 * h->lv[l].L.N; }
+
+    // Operator<Grid::Node>::Fsmooth x n: 2n Jacobi sweeps (Operator.cpp:107).
+    void smooth(int l, T* x, const T* b, int n) {
+        Vecs<T>& v = V(l);
+        for (int q = 0; q < n; ++q) {
+            launch_jacobi<T>(h, L(l), x, b, v.diag, v.C, v.tmp, s);
+            launch_jacobi<T>(h, L(l), v.tmp, b, v.diag, v.C, x, s);
+        }
+    }
+    void corr_res(int l, T* r, const T* x, const T* b) { launch_residual<T>(h, L(l), x, b, V(l).C, r, s); }
+    void restrict_res(int lf, const T* fine, T* crse) {
+        ael::k_restrict<T><<<nblk8(L(lf + 1).L.N), kBlock, 0, s>>>(fine, L(lf).L, crse, L(lf + 1).L, 3);
+    }
+    void interp(int lf, T* fine, const T* crse, int zero_first) {
+        ael::k_interp<T><<<nblk8(L(lf).L.N), kBlock, 0, s>>>(crse, L(lf + 1).L, fine, L(lf).L, 3, zero_first);
+    }
+    void bottom() {
+        const int l = nlev() - 1;
+        Level& lv = L(l);
+        const int Mb = (int)M(l);
+        if (bottom_solver == ALAMO_EL_BOTTOM_DIRECT && h->dense_Ainv) {
+            // exact FP64 solve with the inverse of the coarsest operator
+            ael::k_bottom_direct<T><<<1, kBlock, sizeof(double) * Mb, s>>>(h->dense_Ainv, V(l).res, V(l).cor, Mb);
+            return;
+        }
+        double* x = lv.d.cor;
+        const double* b = lv.d.res;
+        if (!std::is_same<T, double>::value) launch_copy_cast(lv);
+        if (h->dense_A) {
+            ael::k_bottom_bicgstab_dense<<<1, kBlock, sizeof(double) * 9 * Mb, s>>>(
+                x, b, h->dense_A, lv.d.diag, Mb, h->omega, bottom_max_iter, bottom_tol, nuf, h->bottom_status);
+            if (!std::is_same<T, double>::value)
+                ael::k_cast<double, float><<<nblk(Mb), kBlock, 0, s>>>(lv.d.cor, lv.f.cor, Mb);
+            return;
+        }
+        const Lvl LL = lv.L;
+        switch (h->sym) {
+            case 0:
+                ael::k_bottom_solve<0><<<1, ael::kBottomBlock, 0, s>>>(x, b, lv.d.C, lv.d.diag, h->bct_dev, h->uniform,
+                                                                       LL, h->omega, bottom_max_iter, bottom_tol, nuf,
+                                                                       h->bottom_work, h->bottom_status);
+                break;
+            case 1:
+                ael::k_bottom_solve<1><<<1, ael::kBottomBlock, 0, s>>>(x, b, lv.d.C, lv.d.diag, h->bct_dev, h->uniform,
+                                                                       LL, h->omega, bottom_max_iter, bottom_tol, nuf,
+                                                                       h->bottom_work, h->bottom_status);
+                break;
+            default:
+                ael::k_bottom_solve<2><<<1, ael::kBottomBlock, 0, s>>>(x, b, lv.d.C, lv.d.diag, h->bct_dev, h->uniform,
+                                                                       LL, h->omega, bottom_max_iter, bottom_tol, nuf,
+                                                                       h->bottom_work, h->bottom_status);
+                break;
+        }
+        if (!std::is_same<T, double>::value) {
+            ael::k_cast<double, float><<<nblk(M(l)), kBlock, 0, s>>>(lv.d.cor, lv.f.cor, M(l));
+        }
+    }
+    // FP32 hierarchy: the bottom residual goes to FP64 for the bottom solve.
+    void launch_copy_cast(Level& lv) {
+        const long m = 3 * lv.L.N;
+        ael::k_cast<float, double><<<nblk(m), kBlock, 0, s>>>(lv.f.res, lv.d.res, m);
+    }
+
+    // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.
+    void vcycle(int top) {
+        const int bot = nlev() - 1;
+        for (int l = top; l < bot; ++l) {
+            Vecs<T>& v = V(l);
+            launch_fill<T>(v.cor, T(0), M(l), s);
+            smooth(l, v.cor, v.res, nu1);
+            corr_res(l, v.rescor, v.cor, v.res);
+            restrict_res(l, v.rescor, V(l + 1).res);
+        }
+        bottom();
+        for (int l = bot - 1; l >= top; --l) {
+            Vecs<T>& v = V(l);
+            interp(l, v.cor, V(l + 1).cor, 0);
+            smooth(l, v.cor, v.res, nu2);
+        }
+    }
+
+    // amrex::MLMG::mgFcycle.  The residual goes down by nodal injection
+    // (MLLinOp::avgDownResMG -> amrex::average_down on nodal data).
+    void fcycle() {
+        const int bot = nlev() - 1;
+        for (int l = 1; l <= bot; ++l)
+            ael::k_inject<T><<<nblk(L(l).L.N), kBlock, 0, s>>>(V(l - 1).res, L(l - 1).L, V(l).res, L(l).L, 3);
+        bottom();
+        for (int l = bot - 1; l >= 0; --l) {
+            Vecs<T>& v = V(l);
+            interp(l, v.cor, V(l + 1).cor, 1);
+            corr_res(l, v.rescor, v.cor, v.res);
+            std::swap(v.res, v.rescor);
+            std::swap(v.cor, v.cor_hold);
+            vcycle(l);
+            launch_axpy<T>(v.cor, T(1), v.cor_hold, M(l), s);
+        }
+    }
+
+    void run(int cycle) {
+        if (cycle == ALAMO_EL_CYCLE_F) fcycle();
+        else vcycle(0);
+    }
+};
+
+int alloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(ALAMO_EL_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+#define ALLOC(ptr, count)                                                         \
+    do {                                                                          \
+        int rc_ = alloc((void**)&(ptr), sizeof(*(ptr)) * (size_t)(count));        \
+        if (rc_) return rc_;                                                      \
+    } while (0)
+
+int ensure32(alamo_el_t h) {
+    if (h->have32) return 0;
+    for (auto& l : h->lv) {
+        const long N = l.L.N;
+        ALLOC(l.f.C, (long)h->nc * N);
+        ALLOC(l.f.diag, 3 * N);
+        ALLOC(l.f.cor, 3 * N);
+        ALLOC(l.f.cor_hold, 3 * N);
+        ALLOC(l.f.res, 3 * N);
+        ALLOC(l.f.rescor, 3 * N);
+        ALLOC(l.f.tmp, 3 * N);
+    }
+    h->have32 = true;
+    return 0;
+}
+
+// Gauss-Jordan inverse with partial pivoting of a column-major M x M matrix;
+// false when a pivot vanishes.
+bool invert_dense(const std::vector<double>& A, int M, std::vector<double>& inv) {
+    std::vector<double> a(A);  // column-major: a[c*M + r]
+    inv.assign((size_t)M * M, 0.0);
+    for (int q = 0; q < M; ++q) inv[(size_t)q * M + q] = 1.0;
+    for (int c = 0; c < M; ++c) {
+        int piv = c;
+        double best = std::fabs(a[(size_t)c * M + c]);
+        for (int r = c + 1; r < M; ++r)
+            if (std::fabs(a[(size_t)c * M + r]) > best) { best = std::fabs(a[(size_t)c * M + r]); piv = r; }
+        if (!(best > 0.0)) return false;
+        if (piv != c)
+            for (int k = 0; k < M; ++k) {
+                std::swap(a[(size_t)k * M + c], a[(size_t)k * M + piv]);
+                std::swap(inv[(size_t)k * M + c], inv[(size_t)k * M + piv]);
+            }
+        const double d = a[(size_t)c * M + c];
+        for (int k = 0; k < M; ++k) { a[(size_t)k * M + c] /= d; inv[(size_t)k * M + c] /= d; }
+        for (int r = 0; r < M; ++r) {
+            if (r == c) continue;
+            const double f = a[(size_t)c * M + r];
+            if (f == 0.0) continue;
+            for (int k = 0; k < M; ++k) {
+                a[(size_t)k * M + r] -= f * a[(size_t)k * M + c];
+                inv[(size_t)k * M + r] -= f * inv[(size_t)k * M + c];
+            }
+        }
+    }
+    return true;
+}
+
+// Dense coarsest-level operator (A e_q for every unit vector) and its inverse.
+int assemble_dense(alamo_el_t h, hipStream_t s) {
+    Level& lb = h->lv.back();
+    const int M = (int)(3 * lb.L.N);
+    if (M > kDenseMax) return 0;
+    if (h->dense_M != M) {
+        for (double* p : {h->dense_I, h->dense_A, h->dense_Ainv})
+            if (p) (void)hipFree(p);
+        h->dense_I = h->dense_A = h->dense_Ainv = nullptr;
+        ALLOC(h->dense_I, (long)M * M);
+        ALLOC(h->dense_A, (long)M * M);
+        ALLOC(h->dense_Ainv, (long)M * M);
+        std::vector<double> I((size_t)M * M, 0.0);
+        for (int q = 0; q < M; ++q) I[(size_t)q * M + q] = 1.0;
+        HIPCHK(hipMemcpy(h->dense_I, I.data(), sizeof(double) * M * M, hipMemcpyHostToDevice));
+        h->dense_M = M;
+    }
+    dim3 grid(nblk(lb.L.N), M);
+    switch (h->sym) {
+        case 0: ael::k_assemble_dense<0><<<grid, kBlock, 0, s>>>(h->dense_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->dense_A); break;
+        case 1: ael::k_assemble_dense<1><<<grid, kBlock, 0, s>>>(h->dense_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->dense_A); break;
+        default: ael::k_assemble_dense<2><<<grid, kBlock, 0, s>>>(h->dense_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->dense_A); break;
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> A((size_t)M * M), inv;
+    HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, sizeof(double) * M * M, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (invert_dense(A, M, inv)) {
+        HIPCHK(hipMemcpy(h->dense_Ainv, inv.data(), sizeof(double) * M * M, hipMemcpyHostToDevice));
+    } else {
+        (void)hipFree(h->dense_Ainv);
+        h->dense_Ainv = nullptr;
+    }
+    return 0;
+}
+
+int prepare(alamo_el_t h, hipStream_t s) {
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    // averageDownCoeffsSameAmrLevel (src/Operator/Elastic.cpp:746-910)
+    for (size_t l = 1; l < h->lv.size(); ++l) {
+        Level& c = h->lv[l];
+        Level& f = h->lv[l - 1];
+        ael::k_restrict<double><<<nblk8(c.L.N), kBlock, 0, s>>>(f.d.C, f.L, c.d.C, c.L, h->nc);
+    }
+    // Diagonal(true) on every MG level (src/Operator/Operator.cpp:16-29)
+    for (auto& l : h->lv) launch_diagonal<double>(h, l, l.d.C, l.d.diag, s);
+    HIPCHK(hipGetLastError());
+    int rc = assemble_dense(h, s);
+    if (rc) return rc;
+    h->prepared = true;
+    h->prepared32 = false;
+    return 0;
+}
+
+int prepare32(alamo_el_t h, hipStream_t s) {
+    if (h->prepared32) return 0;
+    int rc = ensure32(h);
+    if (rc) return rc;
+    for (auto& l : h->lv) {
+        const long N = l.L.N;
+        ael::k_cast<double, float><<<nblk((long)h->nc * N), kBlock, 0, s>>>(l.d.C, l.f.C, (long)h->nc * N);
+        ael::k_cast<double, float><<<nblk(3 * N), kBlock, 0, s>>>(l.d.diag, l.f.diag, 3 * N);
+    }
+    HIPCHK(hipGetLastError());
+    h->prepared32 = true;
+    return 0;
+}
+
+bool bad_level(alamo_el_t h, int l) { return !h || l < 0 || l >= (int)h->lv.size(); }
+
+// Flexible GMRES(m) with one multigrid cycle as right preconditioner; the
+// stopping test is the inf-norm of the true FP64 residual.
+int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const alamo_el_opts& o, int& iters,
+                 double& resnorm, hipStream_t s);
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* alamo_el_last_error(void) { return g_err.c_str(); }
+
+void alamo_el_default_opts(alamo_el_opts* o) {
+    if (!o) return;
+    o->max_iter = 200;
+    o->nu1 = 2;
+    o->nu2 = 2;
+    o->cycle = ALAMO_EL_CYCLE_F;
+    o->bottom_max_iter = 200;
+    o->final_smooth = 8;
+    o->omega = 2.0 / 3.0;
+    o->bottom_tol_rel = 1e-4;
+    o->precision = ALAMO_EL_PREC_FP64;
+    o->outer = ALAMO_EL_OUTER_MLMG;
+    o->restart = 30;
+    o->verbose = 0;
+    o->bottom_solver = ALAMO_EL_BOTTOM_BICGSTAB;
+}
+
+int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_coarsening_level, alamo_el_t* out) {
+    if (!n_cell || !dx || !out) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (ncomp_of(sym) < 0) return fail(ALAMO_EL_EINVAL, "unknown stiffness symmetry");
+    for (int a = 0; a < 3; ++a)
+        if (n_cell[a] < 2 || !(dx[a] > 0)) return fail(ALAMO_EL_EINVAL, "n_cell must be >= 2 and dx > 0");
+    alamo_el_t h = new alamo_el_s();
+    *out = nullptr;
+    h->sym = sym;
+    h->nc = ncomp_of(sym);
+    int nc[3] = {n_cell[0], n_cell[1], n_cell[2]};
+    double d[3] = {dx[0], dx[1], dx[2]};
+    for (;;) {
+        Level l;
+        l.L.nx = nc[0] + 1;
+        l.L.ny = nc[1] + 1;
+        l.L.nz = nc[2] + 1;
+        l.L.N = (long)l.L.nx * l.L.ny * l.L.nz;
+        l.L.dx = d[0];
+        l.L.dy = d[1];
+        l.L.dz = d[2];
+        h->lv.push_back(l);
+        if (max_coarsening_level >= 0 && (int)h->lv.size() > max_coarsening_level) break;
+        bool ok = true;
+        for (int a = 0; a < 3; ++a)
+            if (nc[a] % 2 != 0 || nc[a] / 2 < 2) ok = false;
+        if (!ok) break;
+        for (int a = 0; a < 3; ++a) { nc[a] /= 2; d[a] *= 2.0; }
+    }
+    auto cleanup = [&](int rc) { alamo_el_destroy(h); return rc; };
+    for (auto& l : h->lv) {
+        const long N = l.L.N;
+        int rc;
+        if ((rc = alloc((void**)&l.d.C, sizeof(double) * h->nc * N))) return cleanup(rc);
+        for (double** p : {&l.d.diag, &l.d.cor, &l.d.cor_hold, &l.d.res, &l.d.rescor, &l.d.tmp})
+            if ((rc = alloc((void**)p, sizeof(double) * 3 * N))) return cleanup(rc);
+    }
+    const long Mb = 3 * h->lv.back().L.N;
+    int rc;
+    if ((rc = alloc((void**)&h->bct_dev, sizeof(int) * 78))) return cleanup(rc);
+    if ((rc = alloc((void**)&h->bcv_dev, sizeof(double) * 78))) return cleanup(rc);
+    if ((rc = alloc((void**)&h->red_partial, sizeof(double) * kRedBlocks))) return cleanup(rc);
+    if ((rc = alloc((void**)&h->red_out, sizeof(double) * 2))) return cleanup(rc);
+    if ((rc = alloc((void**)&h->bottom_work, sizeof(double) * 10 * Mb))) return cleanup(rc);
+    if ((rc = alloc((void**)&h->bottom_status, sizeof(int) * 4))) return cleanup(rc);
+    if (hipHostMalloc((void**)&h->red_host, sizeof(double) * 2) != hipSuccess)
+        return cleanup(fail(ALAMO_EL_EHIP, "hipHostMalloc failed"));
+    for (int q = 0; q < 78; ++q) { h->bct_host[q] = ALAMO_EL_BC_DISPLACEMENT; h->bcv_host[q] = 0.0; }
+    if (hipMemcpy(h->bct_dev, h->bct_host, sizeof(h->bct_host), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->bcv_dev, h->bcv_host, sizeof(h->bcv_host), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(ALAMO_EL_EHIP, "hipMemcpy of the boundary table failed"));
+    *out = h;
+    return 0;
+}
+
+int alamo_el_destroy(alamo_el_t h) {
+    if (!h) return 0;
+    for (auto& l : h->lv) {
+        for (double* p : {l.d.C, l.d.diag, l.d.cor, l.d.cor_hold, l.d.res, l.d.rescor, l.d.tmp})
+            if (p) (void)hipFree(p);
+        for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp})
+            if (p) (void)hipFree(p);
+    }
+    for (void* p : {(void*)h->bct_dev, (void*)h->bcv_dev, (void*)h->red_partial, (void*)h->red_out,
+                    (void*)h->bottom_work, (void*)h->bottom_status, (void*)h->gu9, (void*)h->dw9, (void*)h->csoa,
+                    (void*)h->kv, (void*)h->kz, (void*)h->kw, (void*)h->kh, (void*)h->kpart,
+                    (void*)h->dense_I, (void*)h->dense_A, (void*)h->dense_Ainv})
+        if (p) (void)hipFree(p);
+    if (h->red_host) (void)hipHostFree(h->red_host);
+    if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
+    delete h;
+    return 0;
+}
+
+int alamo_el_num_levels(alamo_el_t h, int* nlev) {
+    if (!h || !nlev) return fail(ALAMO_EL_EINVAL, "null argument");
+    *nlev = (int)h->lv.size();
+    return 0;
+}
+
+int alamo_el_level_nodes(alamo_el_t h, int mglev, int nodes[3]) {
+    if (bad_level(h, mglev) || !nodes) return fail(ALAMO_EL_EINVAL, "bad level");
+    nodes[0] = h->lv[mglev].L.nx;
+    nodes[1] = h->lv[mglev].L.ny;
+    nodes[2] = h->lv[mglev].L.nz;
+    return 0;
+}
+
+int alamo_el_set_bc(alamo_el_t h, const int type_host[78], const double val_host[78]) {
+    if (!h || !type_host || !val_host) return fail(ALAMO_EL_EINVAL, "null argument");
+    for (int q = 0; q < 78; ++q) {
+        if (type_host[q] < 0 || type_host[q] > 4) return fail(ALAMO_EL_EINVAL, "bad boundary type");
+        if (type_host[q] == ALAMO_EL_BC_PERIODIC)
+            return fail(ALAMO_EL_EINVAL, "periodic boundaries are not supported by this library");
+    }
+    std::memcpy(h->bct_host, type_host, sizeof(h->bct_host));
+    std::memcpy(h->bcv_host, val_host, sizeof(h->bcv_host));
+    HIPCHK(hipMemcpy(h->bct_dev, h->bct_host, sizeof(h->bct_host), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->bcv_dev, h->bcv_host, sizeof(h->bcv_host), hipMemcpyHostToDevice));
+    h->prepared = false;
+    return 0;
+}
+
+int alamo_el_set_uniform(alamo_el_t h, int uniform) {
+    if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
+    h->uniform = uniform ? 1 : 0;
+    return 0;
+}
+
+int alamo_el_set_omega(alamo_el_t h, double omega) {
+    if (!h || !(omega > 0.0)) return fail(ALAMO_EL_EINVAL, "bad omega");
+    h->omega = omega;
+    return 0;
+}
+
+int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t s) {
+    if (!h || !C) return fail(ALAMO_EL_EINVAL, "null argument");
+    Level& l = h->lv[0];
+    const long n = (long)h->nc * l.L.N;
+    if (layout == ALAMO_EL_SOA) {
+        HIPCHK(hipMemcpyAsync(l.d.C, C, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else if (layout == ALAMO_EL_AOS) {
+        ael::k_aos_to_soa<<<nblk(l.L.N), kBlock, 0, s>>>(C, l.d.C, l.L.N, h->nc);
+        HIPCHK(hipGetLastError());
+    } else {
+        return fail(ALAMO_EL_EINVAL, "bad layout");
+    }
+    h->model_set = true;
+    h->prepared = false;
+    h->prepared32 = false;
+    return 0;
+}
+
+int alamo_el_prepare(alamo_el_t h, hipStream_t s) {
+    if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
+    return prepare(h, s);
+}
+
+int alamo_el_fapply(alamo_el_t h, int mglev, const double* u, double* f, hipStream_t s) {
+    if (bad_level(h, mglev) || !u || !f) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    if (mglev > 0 && !h->prepared) {
+        int rc = prepare(h, s);
+        if (rc) return rc;
+    }
+    Level& l = h->lv[mglev];
+    launch_fapply<double>(h, l, u, l.d.C, f, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_diagonal(alamo_el_t h, int mglev, double* diag, hipStream_t s) {
+    if (bad_level(h, mglev) || !diag) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    if (mglev > 0 && !h->prepared) {
+        int rc = prepare(h, s);
+        if (rc) return rc;
+    }
+    Level& l = h->lv[mglev];
+    launch_diagonal<double>(h, l, l.d.C, diag, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_fsmooth(alamo_el_t h, int mglev, double* x, const double* b, int nsmooth, hipStream_t s) {
+    if (bad_level(h, mglev) || !x || !b || nsmooth < 0) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (!h->prepared) {
+        int rc = prepare(h, s);
+        if (rc) return rc;
+    }
+    Cycle<double> c{h, s, 2, 2, 8, 200, 1e-4, ALAMO_EL_BOTTOM_BICGSTAB};
+    c.smooth(mglev, x, b, nsmooth);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_residual(alamo_el_t h, int mglev, const double* x, const double* b, double* r, hipStream_t s) {
+    if (bad_level(h, mglev) || !x || !b || !r) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    if (mglev > 0 && !h->prepared) {
+        int rc = prepare(h, s);
+        if (rc) return rc;
+    }
+    Level& l = h->lv[mglev];
+    launch_residual<double>(h, l, x, b, l.d.C, r, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_restrict(alamo_el_t h, int cmglev, double* crse, const double* fine, hipStream_t s) {
+    if (bad_level(h, cmglev) || cmglev == 0 || !crse || !fine) return fail(ALAMO_EL_EINVAL, "bad argument");
+    ael::k_restrict<double><<<nblk8(h->lv[cmglev].L.N), kBlock, 0, s>>>(fine, h->lv[cmglev - 1].L, crse,
+                                                                        h->lv[cmglev].L, 3);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_interp_add(alamo_el_t h, int fmglev, double* fine, const double* crse, hipStream_t s) {
+    if (bad_level(h, fmglev) || fmglev + 1 >= (int)h->lv.size() || !crse || !fine)
+        return fail(ALAMO_EL_EINVAL, "bad argument");
+    ael::k_interp<double><<<nblk8(h->lv[fmglev].L.N), kBlock, 0, s>>>(crse, h->lv[fmglev + 1].L, fine,
+                                                                      h->lv[fmglev].L, 3, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_bc_init(alamo_el_t h, double* rhs, int homogeneous, hipStream_t s) {
+    if (!h || !rhs) return fail(ALAMO_EL_EINVAL, "bad argument");
+    const Lvl L = h->lv[0].L;
+    ael::k_bc_init<<<nblk(L.N), kBlock, 0, s>>>(rhs, h->bct_dev, h->bcv_dev, homogeneous, L);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double* F0, const double* u,
+                        const double* b, double* rhs, hipStream_t s) {
+    if (!h || !C || !u || !b || !rhs) return fail(ALAMO_EL_EINVAL, "bad argument");
+    const Lvl L = h->lv[0].L;
+    if (!h->gu9) ALLOC(h->gu9, 9 * L.N);
+    if (!h->dw9) ALLOC(h->dw9, 9 * L.N);
+    const double* Cs = C;
+    if (layout == ALAMO_EL_AOS) {
+        if (!h->csoa) ALLOC(h->csoa, (long)h->nc * L.N);
+        ael::k_aos_to_soa<<<nblk(L.N), kBlock, 0, s>>>(C, h->csoa, L.N, h->nc);
+        Cs = h->csoa;
+    } else if (layout != ALAMO_EL_SOA) {
+        return fail(ALAMO_EL_EINVAL, "bad layout");
+    }
+    switch (h->sym) {
+        case 0: ael::k_newton_dw<0><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, h->gu9, h->dw9); break;
+        case 1: ael::k_newton_dw<1><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, h->gu9, h->dw9); break;
+        default: ael::k_newton_dw<2><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, h->gu9, h->dw9); break;
+    }
+    ael::k_newton_rhs<<<nblk(L.N), kBlock, 0, s>>>(u, b, h->gu9, h->dw9, h->bct_dev, L, rhs);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_norm_inf(alamo_el_t h, int mglev, const double* v, double* out_host, hipStream_t s) {
+    if (bad_level(h, mglev) || !v || !out_host) return fail(ALAMO_EL_EINVAL, "bad argument");
+    return reduce_host<double>(h, v, v, 3 * h->lv[mglev].L.N, 0, out_host, s);
+}
+
+int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, double tol_abs,
+                   const alamo_el_opts* opts_in, alamo_el_info* info, hipStream_t s) {
+    if (!h || !x || !b) return fail(ALAMO_EL_EINVAL, "null argument");
+    alamo_el_opts o;
+    alamo_el_default_opts(&o);
+    if (opts_in) o = *opts_in;
+    if (o.max_iter < 0 || o.nu1 < 0 || o.nu2 < 0 || o.bottom_max_iter < 0 || !(o.omega > 0))
+        return fail(ALAMO_EL_EINVAL, "bad solver options");
+    const auto t0 = std::chrono::steady_clock::now();
+    h->omega = o.omega;
+    int rc;
+    if (!h->prepared && (rc = prepare(h, s))) return rc;
+    if (o.precision == ALAMO_EL_PREC_FP32_IR && (rc = prepare32(h, s))) return rc;
+    Level& l0 = h->lv[0];
+    const long M0 = 3 * l0.L.N;
+    // MLMG::solve: r0, norms and the stopping target
+    launch_residual<double>(h, l0, x, b, l0.d.C, l0.d.res, s);
+    double resnorm0, rhsnorm0;
+    if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 0, &resnorm0, s))) return rc;
+    if ((rc = reduce_host<double>(h, b, b, M0, 0, &rhsnorm0, s))) return rc;
+    const double target = std::max(tol_abs, std::max(tol_rel, 1.e-16) * std::max(resnorm0, rhsnorm0));
+    int iters = 0, status = ALAMO_EL_NOT_CONVERGED;
+    double nrm = resnorm0;  // fgmres_solve reads it as the inf-norm of r0
+    if (std::isnan(resnorm0)) return fail(ALAMO_EL_ENAN, "NaN in the initial residual");
+    if (resnorm0 <= target) {
+        status = 0;
+    } else if (o.outer == ALAMO_EL_OUTER_FGMRES) {
+        status = fgmres_solve(h, x, b, target, o, iters, nrm, s);
+        if (status < 0) return status;
+    } else {
+        Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+        Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+        for (int it = 0; it < o.max_iter; ++it) {
+            if (o.precision == ALAMO_EL_PREC_FP32_IR) {
+                // FP32 cycle on the FP64 residual; FP64 correction accumulation
+                ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(l0.d.res, l0.f.res, M0);
+                c32.run(o.cycle);
+                ael::k_add_f32_to_f64<<<nblk(M0), kBlock, 0, s>>>(x, l0.f.cor, M0);
+            } else {
+                c64.run(o.cycle);
+                launch_axpy<double>(x, 1.0, l0.d.cor, M0, s);
+            }
+            launch_residual<double>(h, l0, x, b, l0.d.C, l0.d.res, s);
+            if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 0, &nrm, s))) return rc;
+            iters = it + 1;
+            if (o.verbose) std::fprintf(stderr, "alamo_el: iter %d |r|_inf = %.6e\n", iters, nrm);
+            if (std::isnan(nrm)) return fail(ALAMO_EL_ENAN, "NaN in the residual");
+            if (nrm <= target) { status = 0; break; }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    if (info) {
+        info->status = status;
+        info->iters = iters;
+        info->res0 = resnorm0;
+        info->rhsnorm = rhsnorm0;
+        info->res = nrm;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (status == ALAMO_EL_NOT_CONVERGED) g_err = "MLMG did not converge within max_iter";
+    return status;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Flexible GMRES(m), right preconditioned by one multigrid cycle in the
+// chosen precision (the cycle is a nonlinear, changing preconditioner, so
+// the preconditioned directions Z are stored).  Modified Gram-Schmidt with
+// dot products reduced on the host.  The Krylov basis is FP64.
+int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const alamo_el_opts& o, int& iters,
+                 double& resnorm, hipStream_t s) {
+    Level& l0 = h->lv[0];
+    const long M0 = 3 * l0.L.N;
+    const int m = std::max(2, std::min(o.restart, 100));
+    if (h->krylov_m < m) {
+        for (double* p : {h->kv, h->kz, h->kw, h->kh, h->kpart})
+            if (p) (void)hipFree(p);
+        h->kv = h->kz = h->kw = h->kh = h->kpart = nullptr;
+        ALLOC(h->kv, (long)(m + 1) * M0);
+        ALLOC(h->kz, (long)m * M0);
+        ALLOC(h->kw, M0);
+        ALLOC(h->kh, m + 1);
+        ALLOC(h->kpart, (long)(m + 1) * kDotBlocks);
+        if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
+        HIPCHK(hipHostMalloc((void**)&h->red_host_vec, sizeof(double) * (m + 1)));
+        h->krylov_m = m;
+    }
+    const bool fp32 = o.precision == ALAMO_EL_PREC_FP32_IR;
+    Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+    Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    int rc;
+    iters = 0;
+    // r = b - A x is in l0.d.res on entry
+    while (iters < o.max_iter) {
+        double beta2;
+        if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 1, &beta2, s))) return rc;
+        const double beta = std::sqrt(beta2);
+        if (beta == 0.0) { resnorm = 0.0; return 0; }
+        // V0 = r / beta
+        ael::k_scale_copy<double><<<nblk(M0), kBlock, 0, s>>>(l0.d.res, 1.0 / beta, h->kv, M0);
+        const double inf_at_restart = resnorm;
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m && iters < o.max_iter; ++j) {
+            double* Vj = h->kv + (long)j * M0;
+            double* Zj = h->kz + (long)j * M0;
+            // Z_j = cycle(V_j)
+            if (fp32) {
+                ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(Vj, l0.f.res, M0);
+                c32.run(o.cycle);
+                ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, Zj, M0);
+            } else {
+                ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(Vj, l0.d.res, M0);
+                c64.run(o.cycle);
+                ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, Zj, M0);
+            }
+            // w = A Z_j
+            launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);
+            // classical Gram-Schmidt, two passes (CGS2): h = V^T w; w -= V h
+            for (int i = 0; i <= j; ++i) H[i * m + j] = 0.0;
+            for (int pass = 0; pass < 2; ++pass) {
+                ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, j + 1, M0, h->kw, h->kpart);
+                ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h->kh);
+                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, h->kh, -1.0);
+                HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                for (int i = 0; i <= j; ++i) H[i * m + j] += h->red_host_vec[i];
+            }
+            double hn2;
+            if ((rc = reduce_host<double>(h, h->kw, h->kw, M0, 1, &hn2, s))) return rc;
+            const double hn = std::sqrt(hn2);
+            H[(j + 1) * m + j] = hn;
+            double* Vn = h->kv + (long)(j + 1) * M0;
+            ael::k_scale_copy<double><<<nblk(M0), kBlock, 0, s>>>(h->kw, hn > 0 ? 1.0 / hn : 0.0, Vn, M0);
+            for (int i = 0; i < j; ++i) {
+                const double a = H[i * m + j], c = H[(i + 1) * m + j];
+                H[i * m + j] = cs[i] * a + sn[i] * c;
+                H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
+            }
+            const double a = H[j * m + j], c = H[(j + 1) * m + j];
+            const double rr = std::hypot(a, c);
+            cs[j] = rr > 0 ? a / rr : 1.0;
+            sn[j] = rr > 0 ? c / rr : 0.0;
+            H[j * m + j] = rr;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++iters;
+            if (o.verbose) std::fprintf(stderr, "alamo_el: fgmres %d |r|_2 est = %.6e\n", iters, std::fabs(g[j + 1]));
+            // Test the true inf-norm residual once the 2-norm reduction since
+            // the restart, applied to the inf-norm at the restart, meets the
+            // target; a failed test restarts from the updated x.
+            if (std::fabs(g[j + 1]) / beta * inf_at_restart <= target || hn == 0.0) { ++j; break; }
+        }
+        // x += Z y with H y = g
+        const int k = j;
+        for (int i = k - 1; i >= 0; --i) {
+            double acc = g[i];
+            for (int q = i + 1; q < k; ++q) acc -= H[i * m + q] * y[q];
+            y[i] = acc / H[i * m + i];
+        }
+        for (int i = 0; i < k; ++i) launch_axpy<double>(x, y[i], h->kz + (long)i * M0, M0, s);
+        launch_residual<double>(h, l0, x, b, l0.d.C, l0.d.res, s);
+        if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 0, &resnorm, s))) return rc;
+        if (std::isnan(resnorm)) return fail(ALAMO_EL_ENAN, "NaN in the residual");
+        if (resnorm <= target) return 0;
+    }
+    return ALAMO_EL_NOT_CONVERGED;
+}
+
+}  // namespace

@@ -1,0 +1,818 @@
+// elastic_kernels.h -- HIP/CDNA4 device code of the nodal elastic operator.
+//
+// Kernels are templated on the arithmetic type T (double or float) and the
+// stiffness symmetry SYM (0 isotropic, 1 major-minor, 2 diagonal).  Field
+// layout: one box per level, node index i fastest, vector fields component
+// slowest (v[c*N + node]), stiffness structure-of-arrays (C[s*N + node]).
+// Reference paths are relative to the reference root.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ael {
+
+constexpr int kBlock = 256;
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X (block
+// b runs on XCD b % 8), each XCD with its own L2.  Stencil kernels remap the
+// block index so that every XCD walks one contiguous run of nodes: the y and
+// z neighbours a workgroup reads were then loaded into the same L2 by the
+// workgroups just before it.  Grids are launched with a multiple of 8 blocks
+// so that the remap is a bijection; blocks past the field exit.
+constexpr int kXcd = 8;
+__device__ __forceinline__ long xcd_node() {
+    const long per = (long)gridDim.x / kXcd;
+    const long lb = (long)(blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+    return lb * blockDim.x + threadIdx.x;
+}
+
+template <int SYM> struct NComp;
+template <> struct NComp<0> { static constexpr int value = 2; };
+template <> struct NComp<1> { static constexpr int value = 21; };
+template <> struct NComp<2> { static constexpr int value = 9; };
+
+struct Lvl {
+    int nx, ny, nz;
+    long N;
+    double dx, dy, dz;
+};
+
+__host__ __device__ constexpr int vpair(int a, int b) {
+    return a == b ? (a == 0 ? 0 : (a == 1 ? 3 : 5)) : ((a + b) == 1 ? 1 : ((a + b) == 2 ? 2 : 4));
+}
+// 21 MajorMinor slots = upper triangle (row by row) of the 6x6 matrix over the
+// symmetric pairs (00,01,02,11,12,22): src/Set/Matrix4_MajorMinor.H:327-375.
+__host__ __device__ constexpr int tri6(int P, int Q) {
+    return P <= Q ? P * 6 - P * (P - 1) / 2 + (Q - P) : Q * 6 - Q * (Q - 1) / 2 + (P - Q);
+}
+
+// sigma = C : A  (src/Set/Matrix4_Isotropic.H:137-157,
+// src/Set/Matrix4_MajorMinor.H:676-691, src/Set/Matrix4_Diagonal.H:119-132).
+template <typename T, int SYM>
+__device__ __forceinline__ void contract2(const T* c, const T A[3][3], T s[3][3]) {
+    if constexpr (SYM == 0) {
+        const T lam = c[0], mu = c[1];
+        s[0][0] = (lam + T(2) * mu) * A[0][0] + lam * A[1][1] + lam * A[2][2];
+        s[1][1] = lam * A[0][0] + (lam + T(2) * mu) * A[1][1] + lam * A[2][2];
+        s[2][2] = lam * A[0][0] + lam * A[1][1] + (lam + T(2) * mu) * A[2][2];
+        s[1][2] = s[2][1] = mu * (A[1][2] + A[2][1]);
+        s[2][0] = s[0][2] = mu * (A[2][0] + A[0][2]);
+        s[0][1] = s[1][0] = mu * (A[0][1] + A[1][0]);
+    } else if constexpr (SYM == 1) {
+        const T e[6] = {A[0][0], A[0][1] + A[1][0], A[0][2] + A[2][0], A[1][1], A[1][2] + A[2][1], A[2][2]};
+        constexpr int ra[6] = {0, 0, 0, 1, 1, 2}, rb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int P = 0; P < 6; ++P) {
+            T acc = T(0);
+#pragma unroll
+            for (int Q = 0; Q < 6; ++Q) acc += c[tri6(P, Q)] * e[Q];
+            s[ra[P]][rb[P]] = acc;
+            s[rb[P]][ra[P]] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) s[a][b] = c[3 * a + b] * A[a][b];
+    }
+}
+
+// Column col of C : A only (the grad-C term needs one column per direction).
+template <typename T, int SYM>
+__device__ __forceinline__ void contract2_col(const T* c, const T A[3][3], int col, T out[3]) {
+    if constexpr (SYM == 0) {
+        const T lam = c[0], mu = c[1];
+        const T tr = A[0][0] + A[1][1] + A[2][2];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) out[r] = mu * (A[r][col] + A[col][r]) + (r == col ? lam * tr : T(0));
+    } else if constexpr (SYM == 1) {
+        const T e[6] = {A[0][0], A[0][1] + A[1][0], A[0][2] + A[2][0], A[1][1], A[1][2] + A[2][1], A[2][2]};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            T acc = T(0);
+            const int P = vpair(r, col);
+#pragma unroll
+            for (int Q = 0; Q < 6; ++Q) acc += c[tri6(P, Q)] * e[Q];
+            out[r] = acc;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) out[r] = c[3 * r + col] * A[r][col];
+    }
+}
+
+// f_i = C_ijkl B(k,l,j), B(k,l,j) = u_{k,lj}  (src/Set/Matrix4_Isotropic.H:163-173,
+// src/Set/Matrix4_MajorMinor.H:709-767, src/Set/Matrix4_Diagonal.H:134-143).
+template <typename T, int SYM>
+__device__ __forceinline__ void contract3(const T* c, const T B[3][3][3], T f[3]) {
+    if constexpr (SYM == 0) {
+        const T lam = c[0], mu = c[1];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T r = T(0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) r += mu * (B[i][j][j] + B[j][i][j]) + lam * B[j][j][i];
+            f[i] = r;
+        }
+    } else if constexpr (SYM == 1) {
+        T e[3][6];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            e[j][0] = B[0][0][j];
+            e[j][1] = B[0][1][j] + B[1][0][j];
+            e[j][2] = B[0][2][j] + B[2][0][j];
+            e[j][3] = B[1][1][j];
+            e[j][4] = B[1][2][j] + B[2][1][j];
+            e[j][5] = B[2][2][j];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T r = T(0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int P = vpair(i, j);
+#pragma unroll
+                for (int Q = 0; Q < 6; ++Q) r += c[tri6(P, Q)] * e[j][Q];
+            }
+            f[i] = r;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T r = T(0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) r += c[3 * i + j] * B[i][j][j];
+            f[i] = r;
+        }
+    }
+}
+
+// Boundary region in BC::Operator::Elastic::Elastic::Face order, classified
+// as Constant::operator() does (src/BC/Operator/Elastic/Constant.H:373-400);
+// normals use the truncated constants of src/BC/Operator/Elastic/Elastic.H:79-80.
+template <typename T>
+__device__ __forceinline__ int region_of(int i, int j, int k, const Lvl& L, T nrm[3]) {
+    const bool xl = i == 0, xh = i == L.nx - 1, yl = j == 0, yh = j == L.ny - 1, zl = k == 0, zh = k == L.nz - 1;
+    const bool bx = xl || xh, by = yl || yh, bz = zl || zh;
+    if (!(bx || by || bz)) return -1;
+    const T S3 = T(0.57735026919), S2 = T(0.70710678118);
+    const T sx = xl ? T(-1) : T(1), sy = yl ? T(-1) : T(1), sz = zl ? T(-1) : T(1);
+    if (bx && by && bz) {
+        nrm[0] = sx * S3; nrm[1] = sy * S3; nrm[2] = sz * S3;
+        return 18 + (xh ? 4 : 0) + (yh ? 2 : 0) + (zh ? 1 : 0);
+    }
+    if (by && bz) { nrm[0] = T(0); nrm[1] = sy * S2; nrm[2] = sz * S2; return 6 + (yh ? 2 : 0) + (zh ? 1 : 0); }
+    if (bz && bx) { nrm[0] = sx * S2; nrm[1] = T(0); nrm[2] = sz * S2; return 10 + (zh ? 2 : 0) + (xh ? 1 : 0); }
+    if (bx && by) { nrm[0] = sx * S2; nrm[1] = sy * S2; nrm[2] = T(0); return 14 + (xh ? 2 : 0) + (yh ? 1 : 0); }
+    nrm[0] = nrm[1] = nrm[2] = T(0);
+    if (bx) { nrm[0] = sx; return xl ? 0 : 3; }
+    if (by) { nrm[1] = sy; return yl ? 1 : 4; }
+    nrm[2] = sz;
+    return zl ? 2 : 5;
+}
+
+// Constant::set (src/BC/Operator/Elastic/Constant.H:408-425).
+template <typename T>
+__device__ __forceinline__ void bc_set(const int* __restrict__ t3, const T u[3], const T gu[3][3], const T sg[3][3],
+                                       const T n[3], T f[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int ty = t3[c];
+        T v = T(0);
+        if (ty == 0) v = u[c];
+        else if (ty == 1) v = sg[c][0] * n[0] + sg[c][1] * n[1] + sg[c][2] * n[2];
+        else if (ty == 3) v = gu[c][0] * n[0] + gu[c][1] * n[1] + gu[c][2] * n[2];
+        f[c] = v;
+    }
+}
+
+__device__ __forceinline__ void node_ijk(long id, const Lvl& L, int& i, int& j, int& k) {
+    const long t = id / L.nx;
+    i = (int)(id - t * L.nx);
+    k = (int)(t / L.ny);
+    j = (int)(t - (long)k * L.ny);
+}
+
+// One-sided first derivative at the faces, central inside (src/Numeric/Stencil.H:70-191).
+template <typename T>
+__device__ __forceinline__ T d1(const T* __restrict__ f, long id, long stride, int pos, int n, T h) {
+    if (pos == 0) return (f[id + stride] - f[id]) / h;
+    if (pos == n - 1) return (f[id] - f[id - stride]) / h;
+    return (f[id + stride] - f[id - stride]) * T(0.5) / h;
+}
+
+// Elastic<SYM>::Fapply at one node (src/Operator/Elastic.cpp:165-290),
+// psi unset (psi_avg = 1).
+template <typename T, int SYM>
+__device__ __forceinline__ void apply_node(const T* __restrict__ u, const T* __restrict__ C,
+                                           const int* __restrict__ bct, int uniform, const Lvl& L, int i, int j,
+                                           int k, long id, T out[3]) {
+    constexpr int NC = NComp<SYM>::value;
+    const long N = L.N, sy = L.nx, sz = (long)L.nx * L.ny;
+    const T hx = T(L.dx), hy = T(L.dy), hz = T(L.dz);
+    T c[NC];
+#pragma unroll
+    for (int s = 0; s < NC; ++s) c[s] = C[s * N + id];
+    T nrm[3];
+    const int reg = region_of<T>(i, j, k, L, nrm);
+    T gu[3][3];
+    if (reg >= 0) {
+        T uu[3], sg[3][3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const T* f = u + p * N;
+            uu[p] = f[id];
+            gu[p][0] = d1(f, id, 1, i, L.nx, hx);
+            gu[p][1] = d1(f, id, sy, j, L.ny, hy);
+            gu[p][2] = d1(f, id, sz, k, L.nz, hz);
+        }
+        contract2<T, SYM>(c, gu, sg);
+        bc_set<T>(bct + 3 * reg, uu, gu, sg, nrm, out);
+        return;
+    }
+    T B[3][3][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const T* f = u + p * N;
+        const T c0 = f[id];
+        const T xp = f[id + 1], xm = f[id - 1], yp = f[id + sy], ym = f[id - sy], zp = f[id + sz], zm = f[id - sz];
+        gu[p][0] = (xp - xm) * T(0.5) / hx;
+        gu[p][1] = (yp - ym) * T(0.5) / hy;
+        gu[p][2] = (zp - zm) * T(0.5) / hz;
+        B[p][0][0] = (xp - T(2) * c0 + xm) / hx / hx;
+        B[p][1][1] = (yp - T(2) * c0 + ym) / hy / hy;
+        B[p][2][2] = (zp - T(2) * c0 + zm) / hz / hz;
+        B[p][0][1] = T(0.25) * (f[id + 1 + sy] + f[id - 1 - sy] - f[id + 1 - sy] - f[id - 1 + sy]) / (hx * hy);
+        B[p][0][2] = T(0.25) * (f[id + 1 + sz] + f[id - 1 - sz] - f[id + 1 - sz] - f[id - 1 + sz]) / (hx * hz);
+        B[p][1][2] = T(0.25) * (f[id + sy + sz] + f[id - sy - sz] - f[id + sy - sz] - f[id - sy + sz]) / (hy * hz);
+        B[p][1][0] = B[p][0][1];
+        B[p][2][0] = B[p][0][2];
+        B[p][2][1] = B[p][1][2];
+    }
+    contract3<T, SYM>(c, B, out);
+    if (!uniform) {
+        // f += sum_a ((dC/dx_a) : grad u).col(a)   (src/Operator/Elastic.cpp:245-254)
+        const long st[3] = {1, sy, sz};
+        const T h[3] = {hx, hy, hz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T cg[NC], col[3];
+#pragma unroll
+            for (int s = 0; s < NC; ++s) cg[s] = (C[s * N + id + st[a]] - C[s * N + id - st[a]]) * T(0.5) / h[a];
+            contract2_col<T, SYM>(cg, gu, a, col);
+            out[0] += col[0];
+            out[1] += col[1];
+            out[2] += col[2];
+        }
+    }
+}
+
+template <typename T, int SYM>
+__global__ void __launch_bounds__(kBlock) k_fapply(const T* __restrict__ u, const T* __restrict__ C,
+                                                   const int* __restrict__ bct, int uniform, Lvl L,
+                                                   T* __restrict__ f) {
+    const long id = xcd_node();
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    T o[3];
+    apply_node<T, SYM>(u, C, bct, uniform, L, i, j, k, id, o);
+    f[id] = o[0];
+    f[L.N + id] = o[1];
+    f[2 * L.N + id] = o[2];
+}
+
+// r = b - A x  (src/Operator/Operator.cpp:635-657: apply, then Xpay with -1).
+template <typename T, int SYM>
+__global__ void __launch_bounds__(kBlock) k_residual(const T* __restrict__ x, const T* __restrict__ b,
+                                                     const T* __restrict__ C, const int* __restrict__ bct,
+                                                     int uniform, Lvl L, T* __restrict__ r) {
+    const long id = xcd_node();
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    T o[3];
+    apply_node<T, SYM>(x, C, bct, uniform, L, i, j, k, id, o);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) r[p * L.N + id] = b[p * L.N + id] + T(-1) * o[p];
+}
+
+// One damped Jacobi sweep, out of place (src/Operator/Operator.cpp:109-146):
+// x_out = (1-w) x + w (b - (A x - D x)) / D.
+template <typename T, int SYM>
+__global__ void __launch_bounds__(kBlock) k_jacobi(const T* __restrict__ x, const T* __restrict__ b,
+                                                   const T* __restrict__ diag, const T* __restrict__ C,
+                                                   const int* __restrict__ bct, int uniform, Lvl L, T omega,
+                                                   T* __restrict__ xo) {
+    const long id = xcd_node();
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    T o[3];
+    apply_node<T, SYM>(x, C, bct, uniform, L, i, j, k, id, o);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long q = p * L.N + id;
+        const T xv = x[q], dv = diag[q];
+        const T Rx = o[p] - xv * dv;
+        xo[q] = (T(1) - omega) * xv + omega * (b[q] - Rx) / dv;
+    }
+}
+
+// Elastic<SYM>::Diagonal (src/Operator/Elastic.cpp:321-374) with
+// Gradient_Diagonal (src/Numeric/Stencil.H:900-963).
+template <typename T, int SYM>
+__global__ void __launch_bounds__(kBlock) k_diagonal(const T* __restrict__ C, const int* __restrict__ bct, Lvl L,
+                                                     T* __restrict__ diag) {
+    constexpr int NC = NComp<SYM>::value;
+    const long id = xcd_node();
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    T c[NC];
+#pragma unroll
+    for (int s = 0; s < NC; ++s) c[s] = C[s * L.N + id];
+    T nrm[3];
+    const int reg = region_of<T>(i, j, k, L, nrm);
+    const int pos[3] = {i, j, k}, nn[3] = {L.nx, L.ny, L.nz};
+    const T h[3] = {T(L.dx), T(L.dy), T(L.dz)};
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        T val;
+        if (reg >= 0) {
+            T gu[3][3], sg[3][3], uu[3] = {T(0), T(0), T(0)}, f[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) gu[q][a] = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const bool mn = pos[a] == 0, mx = pos[a] == nn[a] - 1;
+                gu[p][a] = ((!mx ? T(0) : T(1)) - (!mn ? T(0) : T(1))) / ((mn || mx ? T(1) : T(2)) * h[a]);
+            }
+            contract2<T, SYM>(c, gu, sg);
+            uu[p] = T(1);
+            bc_set<T>(bct + 3 * reg, uu, gu, sg, nrm, f);
+            val = f[p];
+        } else {
+            T B[3][3][3], f[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) B[a][b2][d] = T(0);
+#pragma unroll
+            for (int l = 0; l < 3; ++l) B[p][l][l] = T(-2) / h[l] / h[l];
+            contract3<T, SYM>(c, B, f);
+            val = f[p];
+        }
+        diag[p * L.N + id] = val;
+    }
+}
+
+// Nodal full-weighting restriction of ncomp components from level Lf to Lc
+// (src/Operator/Operator.cpp:280-353); the same weights average C between MG
+// levels (src/Operator/Elastic.cpp:786-834).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_restrict(const T* __restrict__ fine, Lvl Lf, T* __restrict__ crse,
+                                                     Lvl Lc, int ncomp) {
+    const long cid = xcd_node();
+    if (cid >= Lc.N) return;
+    int I, J, K;
+    node_ijk(cid, Lc, I, J, K);
+    const int hx = Lc.nx - 1, hy = Lc.ny - 1, hz = Lc.nz - 1;
+    const bool bi = (I == 0 || I == hx), bj = (J == 0 || J == hy), bk = (K == 0 || K == hz);
+    const long sy = Lf.nx, sz = (long)Lf.nx * Lf.ny;
+    const long f0 = (2L * K) * sz + (2L * J) * sy + 2L * I;
+    for (int n = 0; n < ncomp; ++n) {
+        const T* F = fine + n * Lf.N + f0;
+        T v;
+        if (bi && bj && bk) v = F[0];
+        else if (bj && bk) v = T(0.25) * F[-1] + T(0.5) * F[0] + T(0.25) * F[1];
+        else if (bk && bi) v = T(0.25) * F[-sy] + T(0.5) * F[0] + T(0.25) * F[sy];
+        else if (bi && bj) v = T(0.25) * F[-sz] + T(0.5) * F[0] + T(0.25) * F[sz];
+        else if (bi)
+            v = (+F[-sy - sz] + T(2) * F[-sz] + F[sy - sz] + T(2) * F[-sy] + T(4) * F[0] + T(2) * F[sy] + F[-sy + sz] +
+                 T(2) * F[sz] + F[sy + sz]) / T(16);
+        else if (bj)
+            v = (+F[-1 - sz] + T(2) * F[-1] + F[-1 + sz] + T(2) * F[-sz] + T(4) * F[0] + T(2) * F[sz] + F[1 - sz] +
+                 T(2) * F[1] + F[1 + sz]) / T(16);
+        else if (bk)
+            v = (+F[-1 - sy] + T(2) * F[-sy] + F[1 - sy] + T(2) * F[-1] + T(4) * F[0] + T(2) * F[1] + F[-1 + sy] +
+                 T(2) * F[sy] + F[1 + sy]) / T(16);
+        else
+            v = (F[-1 - sy - sz] + F[-1 - sy + sz] + F[-1 + sy - sz] + F[-1 + sy + sz] + F[1 - sy - sz] +
+                 F[1 - sy + sz] + F[1 + sy - sz] + F[1 + sy + sz]) / T(64) +
+                (F[-sy - sz] + F[-sy + sz] + F[sy - sz] + F[sy + sz] + F[-1 - sz] + F[1 - sz] + F[-1 + sz] +
+                 F[1 + sz] + F[-1 - sy] + F[-1 + sy] + F[1 - sy] + F[1 + sy]) / T(32) +
+                (F[-1] + F[-sy] + F[-sz] + F[1] + F[sy] + F[sz]) / T(16) + F[0] / T(8);
+        crse[n * Lc.N + cid] = v;
+    }
+}
+
+// Nodal injection crse(I,J,K) = fine(2I,2J,2K): amrex::average_down on nodal
+// data, which MLLinOp::avgDownResMG applies to the residual in the F-cycle.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_inject(const T* __restrict__ fine, Lvl Lf, T* __restrict__ crse, Lvl Lc,
+                                                   int ncomp) {
+    const long cid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cid >= Lc.N) return;
+    int I, J, K;
+    node_ijk(cid, Lc, I, J, K);
+    const long f0 = (2L * K) * ((long)Lf.nx * Lf.ny) + (2L * J) * Lf.nx + 2L * I;
+    for (int n = 0; n < ncomp; ++n) crse[n * Lc.N + cid] = fine[n * Lf.N + f0];
+}
+
+// Trilinear prolongation, fine += P crse (src/Operator/Operator.cpp:399-428).
+// With zero_first the fine field is overwritten instead (cor = P cor_crse).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_interp(const T* __restrict__ crse, Lvl Lc, T* __restrict__ fine, Lvl Lf,
+                                                   int ncomp, int zero_first) {
+    const long id = xcd_node();
+    if (id >= Lf.N) return;
+    int i, j, k;
+    node_ijk(id, Lf, i, j, k);
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    const bool oi = i & 1, oj = j & 1, ok = k & 1;
+    const long sy = Lc.nx, sz = (long)Lc.nx * Lc.ny;
+    const long c0 = (long)K * sz + (long)J * sy + I;
+    for (int n = 0; n < ncomp; ++n) {
+        const T* Cc = crse + n * Lc.N + c0;
+        T v;
+        if (!oi && !oj && !ok) v = Cc[0];
+        else if (!oj && !ok) v = T(0.5) * (Cc[0] + Cc[1]);
+        else if (!ok && !oi) v = T(0.5) * (Cc[0] + Cc[sy]);
+        else if (!oi && !oj) v = T(0.5) * (Cc[0] + Cc[sz]);
+        else if (!oi) v = T(0.25) * (Cc[0] + Cc[sy] + Cc[sz] + Cc[sy + sz]);
+        else if (!oj) v = T(0.25) * (Cc[0] + Cc[sz] + Cc[1] + Cc[1 + sz]);
+        else if (!ok) v = T(0.25) * (Cc[0] + Cc[1] + Cc[sy] + Cc[1 + sy]);
+        else
+            v = T(0.125) * (Cc[0] + Cc[1] + Cc[sy] + Cc[sz] + Cc[sy + sz] + Cc[1 + sz] + Cc[1 + sy] + Cc[1 + sy + sz]);
+        T* out = fine + n * Lf.N + id;
+        *out = zero_first ? v : *out + v;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill(T* __restrict__ p, T v, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) p[id] = v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_axpy(T* __restrict__ y, T a, const T* __restrict__ x, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) y[id] += a * x[id];
+}
+
+// y = a * x
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_scale_copy(const T* __restrict__ x, T a, T* __restrict__ y, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) y[id] = a * x[id];
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kBlock) k_cast(const Tin* __restrict__ a, Tout* __restrict__ b, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) b[id] = (Tout)a[id];
+}
+
+// y += (double) x  (FP64 correction accumulation of an FP32 correction)
+__global__ void __launch_bounds__(kBlock) k_add_f32_to_f64(double* __restrict__ y, const float* __restrict__ x,
+                                                           long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) y[id] += (double)x[id];
+}
+
+// AoS (BaseFab<Matrix4> order) -> SoA copy of a stiffness field.
+__global__ void __launch_bounds__(kBlock) k_aos_to_soa(const double* __restrict__ a, double* __restrict__ s, long N,
+                                                       int nc) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= N) return;
+    for (int c = 0; c < nc; ++c) s[c * N + id] = a[id * nc + c];
+}
+
+// Two-pass reductions: per-block partials written with vector stores, then
+// one block folds them.  op 0: max |v|, op 1: sum a*b.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_reduce_partial(const T* __restrict__ a, const T* __restrict__ b, long n,
+                                                           int op, double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
+        if (op == 0) acc = fmax(acc, fabs((double)a[id]));
+        else acc += (double)a[id] * (double)b[id];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = op == 0 ? fmax(sh[threadIdx.x], sh[threadIdx.x + s])
+                                                             : sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_reduce_final(const double* __restrict__ partial, int nb, int op,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) acc = op == 0 ? fmax(acc, partial[q]) : acc + partial[q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = op == 0 ? fmax(sh[threadIdx.x], sh[threadIdx.x + s])
+                                                             : sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// BC::Operator::Elastic::Constant::Init (src/BC/Operator/Elastic/Constant.H:197-286).
+__global__ void __launch_bounds__(kBlock) k_bc_init(double* __restrict__ rhs, const int* __restrict__ bct,
+                                                    const double* __restrict__ bcv, int homogeneous, Lvl L) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    double nrm[3];
+    const int reg = region_of<double>(i, j, k, L, nrm);
+    if (reg < 0) return;
+    for (int d = 0; d < 3; ++d) {
+        const int ty = bct[3 * reg + d];
+        rhs[d * L.N + id] = (homogeneous && ty == 0) ? 0.0 : bcv[3 * reg + d];
+    }
+}
+
+// Newton set-up, pass 1 (src/Solver/Nonlocal/Newton.H:168-184): per node
+// grad u (one-sided at faces) and DW = C:(grad u - F0).  C in SoA.
+template <int SYM>
+__global__ void __launch_bounds__(kBlock) k_newton_dw(const double* __restrict__ u, const double* __restrict__ C,
+                                                      const double* __restrict__ F0, Lvl L, double* __restrict__ gu9,
+                                                      double* __restrict__ dw9) {
+    constexpr int NC = NComp<SYM>::value;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    const long sy = L.nx, sz = (long)L.nx * L.ny;
+    double gu[3][3], A[3][3], s[3][3], c[NC];
+    for (int p = 0; p < 3; ++p) {
+        const double* f = u + p * L.N;
+        gu[p][0] = d1(f, id, 1, i, L.nx, L.dx);
+        gu[p][1] = d1(f, id, sy, j, L.ny, L.dy);
+        gu[p][2] = d1(f, id, sz, k, L.nz, L.dz);
+    }
+    for (int p = 0; p < 3; ++p)
+        for (int a = 0; a < 3; ++a) A[p][a] = gu[p][a] - (F0 ? F0[(3 * p + a) * L.N + id] : 0.0);
+    for (int q = 0; q < NC; ++q) c[q] = C[q * L.N + id];
+    contract2<double, SYM>(c, F0 ? A : gu, s);
+    for (int p = 0; p < 3; ++p)
+        for (int a = 0; a < 3; ++a) {
+            gu9[(3 * p + a) * L.N + id] = gu[p][a];
+            dw9[(3 * p + a) * L.N + id] = s[p][a];
+        }
+}
+
+// Newton set-up, pass 2 (src/Solver/Nonlocal/Newton.H:216-231).
+__global__ void __launch_bounds__(kBlock) k_newton_rhs(const double* __restrict__ u, const double* __restrict__ b,
+                                                       const double* __restrict__ gu9, const double* __restrict__ dw9,
+                                                       const int* __restrict__ bct, Lvl L, double* __restrict__ rhs) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    double nrm[3];
+    const int reg = region_of<double>(i, j, k, L, nrm);
+    if (reg >= 0) {
+        double uu[3], gu[3][3], s[3][3], f[3];
+        for (int p = 0; p < 3; ++p) {
+            uu[p] = u[p * L.N + id];
+            for (int a = 0; a < 3; ++a) {
+                gu[p][a] = gu9[(3 * p + a) * L.N + id];
+                s[p][a] = dw9[(3 * p + a) * L.N + id];
+            }
+        }
+        bc_set<double>(bct + 3 * reg, uu, gu, s, nrm, f);
+        for (int d = 0; d < 3; ++d) rhs[d * L.N + id] = b[d * L.N + id] - f[d];
+    } else {
+        // Numeric::Divergence, central (src/Numeric/Stencil.H:588-657)
+        const long st[3] = {1, (long)L.nx, (long)L.nx * L.ny};
+        const double h[3] = {L.dx, L.dy, L.dz};
+        double dv[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; ++a)
+            for (int r = 0; r < 3; ++r)
+                dv[r] += (dw9[(3 * r + a) * L.N + id + st[a]] - dw9[(3 * r + a) * L.N + id - st[a]]) / 2. / h[a];
+        for (int d = 0; d < 3; ++d) rhs[d * L.N + id] = b[d * L.N + id] - dv[d];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense coarsest level
+// ---------------------------------------------------------------------------
+// Column q of the coarsest-level operator: A e_q, for all q at once.  The
+// identity I (M x M, column-major) supplies the unit vectors; Acol receives
+// A column-major (Acol[q*M + row]).
+template <int SYM>
+__global__ void __launch_bounds__(kBlock) k_assemble_dense(const double* __restrict__ I, const double* __restrict__ C,
+                                                           const int* __restrict__ bct, int uniform, Lvl L,
+                                                           double* __restrict__ Acol) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long M = 3 * L.N;
+    if (id >= L.N) return;
+    const long q = blockIdx.y;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    double o[3];
+    apply_node<double, SYM>(I + q * M, C, bct, uniform, L, i, j, k, id, o);
+    for (int c = 0; c < 3; ++c) Acol[q * M + c * L.N + id] = o[c];
+}
+
+// x = Ainv * rhs with the inverse of the coarsest-level operator (column-major),
+// in FP64; rhs and x may be FP32 (the FP32 cycles) or FP64.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bottom_direct(const double* __restrict__ Ainv, const T* __restrict__ rhs,
+                                                          T* __restrict__ x, int M) {
+    extern __shared__ double sb[];
+    for (int q = threadIdx.x; q < M; q += blockDim.x) sb[q] = (double)rhs[q];
+    __syncthreads();
+    for (int row = threadIdx.x; row < M; row += blockDim.x) {
+        double acc = 0.0;
+        for (int c = 0; c < M; ++c) acc += Ainv[(long)c * M + row] * sb[c];
+        x[row] = (T)acc;
+    }
+}
+
+// Block-wide max (op 0) or sum (op 1) for kBlock threads; all threads get it.
+__device__ __forceinline__ double blk_reduce256(double v, int op, double* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = op == 0 ? fmax(v, o) : v + o;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int q = 0; q < kBlock / 64; ++q) r = op == 0 ? fmax(r, red[q]) : r + red[q];
+    return r;
+}
+
+// The reference bottom solve on the dense coarsest operator: MLCGSolver
+// BiCGStab with the diagonal as left preconditioner, then the smoothing of a
+// failed solve (see k_bottom_solve for the stencil form of the same
+// algorithm).  Vectors live in LDS (9 M doubles).  One workgroup of kBlock.
+__global__ void __launch_bounds__(kBlock)
+    k_bottom_bicgstab_dense(double* __restrict__ xg, const double* __restrict__ rhs, const double* __restrict__ Acol,
+                            const double* __restrict__ diag, int M, double omega, int maxiter, double eps_rel, int nuf,
+                            int* __restrict__ status) {
+    extern __shared__ double sm[];
+    __shared__ double red[kBlock / 64];
+    double *x = sm, *r = sm + M, *rh = sm + 2 * M, *p = sm + 3 * M, *v = sm + 4 * M, *s = sm + 5 * M,
+           *t = sm + 6 * M, *b = sm + 7 * M, *d = sm + 8 * M;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto matvec = [&](const double* in, double* out) {
+        __syncthreads();
+        for (int row = tid; row < M; row += nt) {
+            double acc = 0.0;
+            for (int c = 0; c < M; ++c) acc += Acol[(long)c * M + row] * in[c];
+            out[row] = acc;
+        }
+        __syncthreads();
+    };
+    for (int q = tid; q < M; q += nt) { x[q] = 0.0; b[q] = rhs[q]; d[q] = diag[q]; }
+    __syncthreads();
+    matvec(x, v);
+    double loc = 0.0;
+    for (int q = tid; q < M; q += nt) {
+        const double rq = (b[q] + (-1.0) * v[q]) / d[q];
+        r[q] = rq;
+        rh[q] = rq;
+        loc = fmax(loc, fabs(rq));
+    }
+    const double rnorm0 = blk_reduce256(loc, 0, red);
+    double rnorm = rnorm0, rho_1 = 0.0, alpha = 0.0, omg = 0.0;
+    int ret = 0;
+    if (rnorm0 != 0.0) {
+        for (int iter = 1; iter <= maxiter; ++iter) {
+            loc = 0.0;
+            for (int q = tid; q < M; q += nt) loc += rh[q] * r[q];
+            const double rho = blk_reduce256(loc, 1, red);
+            if (rho == 0.0) { ret = 1; break; }
+            if (iter == 1) {
+                for (int q = tid; q < M; q += nt) p[q] = r[q];
+            } else {
+                const double beta = (rho / rho_1) * (alpha / omg);
+                for (int q = tid; q < M; q += nt) p[q] = r[q] + beta * (p[q] + (-omg) * v[q]);
+            }
+            matvec(p, v);
+            loc = 0.0;
+            for (int q = tid; q < M; q += nt) {
+                v[q] /= d[q];
+                loc += rh[q] * v[q];
+            }
+            const double rhTv = blk_reduce256(loc, 1, red);
+            if (rhTv != 0.0) alpha = rho / rhTv;
+            else { ret = 2; break; }
+            loc = 0.0;
+            for (int q = tid; q < M; q += nt) {
+                x[q] += alpha * p[q];
+                const double sq = 1.0 * r[q] + (-alpha) * v[q];
+                s[q] = sq;
+                loc = fmax(loc, fabs(sq));
+            }
+            rnorm = blk_reduce256(loc, 0, red);
+            if (rnorm < eps_rel * rnorm0) break;
+            matvec(s, t);
+            double l1 = 0.0, l2 = 0.0;
+            for (int q = tid; q < M; q += nt) {
+                t[q] /= d[q];
+                l1 += t[q] * t[q];
+                l2 += t[q] * s[q];
+            }
+            const double tt = blk_reduce256(l1, 1, red);
+            const double ts = blk_reduce256(l2, 1, red);
+            if (tt != 0.0) omg = ts / tt;
+            else { ret = 3; break; }
+            loc = 0.0;
+            for (int q = tid; q < M; q += nt) {
+                x[q] += omg * s[q];
+                const double rq = 1.0 * s[q] + (-omg) * t[q];
+                r[q] = rq;
+                loc = fmax(loc, fabs(rq));
+            }
+            rnorm = blk_reduce256(loc, 0, red);
+            if (rnorm < eps_rel * rnorm0) break;
+            if (omg == 0.0) { ret = 4; break; }
+            rho_1 = rho;
+        }
+        if (ret == 0 && rnorm > eps_rel * rnorm0) ret = 8;
+        if (!((ret == 0 || ret == 8) && rnorm < rnorm0))
+            for (int q = tid; q < M; q += nt) x[q] = 0.0;
+        if (ret != 0 && ret != 9)
+            for (int q = tid; q < M; q += nt) x[q] = 0.0;
+        const int nsm = ret == 0 ? 0 : nuf;
+        for (int sw = 0; sw < 2 * nsm; ++sw) {
+            matvec(x, v);
+            for (int q = tid; q < M; q += nt) {
+                const double Rx = v[q] - x[q] * d[q];
+                t[q] = (1.0 - omega) * x[q] + omega * (b[q] - Rx) / d[q];
+            }
+            __syncthreads();
+            for (int q = tid; q < M; q += nt) x[q] = t[q];
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < M; q += nt) xg[q] = x[q];
+    if (tid == 0) status[0] = ret;
+}
+
+// h[i] partials of V_i . w for i < k (V: k vectors of length M, stride M).
+__global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __restrict__ V, int k, long M,
+                                                             const double* __restrict__ w,
+                                                             double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    for (int i = 0; i < k; ++i) {
+        const double* Vi = V + (long)i * M;
+        double acc = 0.0;
+        for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < M; q += (long)gridDim.x * blockDim.x)
+            acc += Vi[q] * w[q];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = kBlock / 2; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(long)i * gridDim.x + blockIdx.x] = sh[0];
+        __syncthreads();
+    }
+}
+
+// h[i] = sum of partials; one block per i.
+__global__ void __launch_bounds__(kBlock) k_multidot_final(const double* __restrict__ partial, int nb,
+                                                           double* __restrict__ h) {
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) acc += partial[(long)blockIdx.x * nb + q];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) h[blockIdx.x] = sh[0];
+}
+
+// w -= sum_i h[i] V_i  (h on the device), then optionally y += sum_i g[i] Z_i.
+__global__ void __launch_bounds__(kBlock) k_multiaxpy(double* __restrict__ w, const double* __restrict__ V, int k,
+                                                      long M, const double* __restrict__ h, double sign) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= M) return;
+    double acc = w[q];
+    for (int i = 0; i < k; ++i) acc += sign * h[i] * V[(long)i * M + q];
+    w[q] = acc;
+}
+
+}  // namespace ael

@@ -1,0 +1,52 @@
+"""Seeded test problems of the elastic solve (numpy, host side).
+
+  inclusion(n_cell)       spherical inclusion in an isotropic matrix:
+                          E=1, nu=0.3 matrix; r=0.2 sphere with E=10, nu=0.3
+                          and dilatational eigenstrain F0 = 1e-3 I; all
+                          boundary regions zero displacement; Affine
+                          isotropic models mixed by a sharp node indicator
+                          (src/Integrator/Mechanics.H:249-253).
+  polycrystal(n_cell)     Voronoi polycrystal of cubic grains (C11=1.7,
+                          C12=1.2, C44=0.75), each rotated by a Haar-uniform
+                          rotation, under uniaxial 1e-3 strain along z
+                          (bc.uniaxial); seeds uniform in the unit cube from
+                          numpy's PCG64 with the given seed.
+
+Each returns a dict with sym, C (ncomp, nz, ny, nx), optional F0
+(9, nz, ny, nx), bc (Constant), dx and n_cell.  The right-hand side is built
+on the device by ElasticOperator.bc_init + newton_rhs, as
+Base::Mechanics::TimeStepBegin does (src/Integrator/Base/Mechanics.H:175-206).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import bc as _bc
+from . import model as _model
+
+
+def inclusion(n_cell: int = 64, E_matrix=1.0, E_incl=10.0, nu=0.3, radius=0.2, center=(0.5, 0.5, 0.5),
+              eigenstrain=1e-3):
+    eta = _model.sphere_indicator(n_cell, center, radius)
+    lm, mm = _model.lame_from_E_nu(E_matrix, nu)
+    li, mi = _model.lame_from_E_nu(E_incl, nu)
+    etas = np.stack([1.0 - eta, eta])
+    C = _model.mix(etas, np.array([[lm, mm], [li, mi]]))
+    F0 = _model.mix(etas, np.array([np.zeros(9), (eigenstrain * np.eye(3)).ravel()]))
+    return dict(sym=_model.ISOTROPIC, C=np.ascontiguousarray(C), F0=np.ascontiguousarray(F0), bc=_bc.Constant(),
+                dx=1.0 / n_cell, n_cell=n_cell, name=f"inclusion-{n_cell + 1}^3")
+
+
+def polycrystal(n_cell: int = 128, ngrains: int = 64, seed: int = 0, C11=1.7, C12=1.2, C44=0.75, strain=1e-3,
+                convention="reference", return_grains=False):
+    rng = np.random.default_rng(seed)
+    seeds = rng.random((ngrains, 3))
+    R = _model.haar_rotations(ngrains, seed + 1)
+    table = np.stack([_model.cubic(C11, C12, C44, R[g], convention=convention) for g in range(ngrains)])
+    gid = _model.voronoi_grain_ids(n_cell, seeds)
+    C = _model.field_from_table(gid, table)
+    out = dict(sym=_model.MAJORMINOR, C=C, F0=None, bc=_bc.uniaxial(2, strain), dx=1.0 / n_cell, n_cell=n_cell,
+               name=f"polycrystal-{n_cell + 1}^3-{ngrains}g")
+    if return_grains:
+        out.update(grain_ids=gid, table=table, seeds=seeds, rotations=R)
+    return out

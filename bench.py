@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Headline benchmark: time to 1e-8 relative residual of the nodal elastic
+solve on a 129^3-node (n_cell = 128) Voronoi polycrystal on one MI355X, plus
+the HBM roofline of the Fapply kernel.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one full solve of the workload from a zero initial guess with the
+fast path (FP32 multigrid cycles inside FP64 flexible GMRES; the residual
+that decides convergence is always the FP64 one).  The plain FP64 multigrid
+with the reference's settings (F-cycle every iteration; also the V-cycle) is
+solved once beside it, and the distance of the fast solution to the plain
+FP64 solution is reported.  Inputs are resident in HBM before any timing
+starts.  The CPU baseline is the FP64 oracle on the host cores, on a bounded
+sample (Fapply repetitions and two F-cycles of the same problem).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("time to 1e-8 residual, 128³ elastic (learned/FP32 vs plain FP64); Fapply GB/s")
+PEAK_HBM_GBS = 8000.0
+# Algorithmic bytes per node of one Fapply in FP64: u (3x8) + C + f (3x8);
+# MajorMinor C holds 21 doubles (168 B), isotropic C 2 doubles (16 B).
+FAPPLY_BYTES_PER_NODE = {0: 24 + 16 + 24, 1: 24 + 168 + 24, 2: 24 + 72 + 24}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-cell", type=int, default=128)
+    ap.add_argument("--grains", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--fapply-reps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference solves")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+    import torch
+
+    from alamo_amd import elastic as el
+    from alamo_amd import problems
+
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    t0 = time.time()
+    p = problems.polycrystal(args.n_cell, ngrains=args.grains, seed=args.seed)
+    log(f"built {p['name']} on the host in {time.time() - t0:.1f}s")
+    op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
+    op.SetBC(p["bc"])
+    C = torch.from_numpy(p["C"]).cuda()
+    op.SetModel(C)
+    shape = op.level_shape(0)
+    nodes = int(np.prod(shape))
+    b = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
+    u0 = torch.zeros_like(b)
+    rhs = op.newton_rhs(C, u0, b)
+    op.prepare()
+    torch.cuda.synchronize()
+
+    # --- Fapply roofline (HIP events on the stream the kernels run on) ------
+    u = torch.randn((3,) + shape, dtype=torch.float64, device="cuda")
+    f = torch.empty_like(u)
+    for _ in range(5):
+        op.Fapply(u, out=f)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.fapply_reps):
+        op.Fapply(u, out=f)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    fapply_ms = ev0.elapsed_time(ev1) / args.fapply_reps
+    bytes_per_launch = FAPPLY_BYTES_PER_NODE[p["sym"]] * nodes
+    fapply_gbs = bytes_per_launch / (fapply_ms * 1e-3) / 1e9
+    log(f"Fapply {fapply_ms * 1e3:.1f} us/launch, {fapply_gbs:.0f} GB/s algorithmic")
+
+    # --- plain FP64 multigrid with the reference settings --------------------
+    plain = {}
+    x64 = None
+    if not args.no_plain:
+        for cyc in ("F", "V"):
+            x = torch.zeros_like(rhs)
+            s = el.Linear(op, cycle=cyc, abort_on_fail=False, max_iter=200)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            s.solve(x, rhs, args.tol, 0.0)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            plain[cyc] = dict(seconds=dt, iters=s.info.iters, converged=s.info.status == 0,
+                              rel_res=s.info.res / max(s.info.rhsnorm, s.info.res0))
+            log(f"plain FP64 {cyc}-cycle: {dt:.3f}s, {s.info.iters} iters, converged={s.info.status == 0}")
+            if cyc == "F" and s.info.status == 0:
+                x64 = x
+
+    # --- fast path: FP32 cycles inside FP64 FGMRES --------------------------
+    def fast_solve():
+        x = torch.zeros_like(rhs)
+        s = el.Linear(op, cycle="F", precision="fp32", outer="fgmres", restart=30, bottom_solver="direct")
+        s.solve(x, rhs, args.tol, 0.0)
+        return x, s
+
+    for _ in range(args.warmup):
+        fast_solve()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x, s = fast_solve()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    fast_t = float(np.mean(times))
+    fast_iters = s.info.iters
+    rel_res = s.info.res / max(s.info.rhsnorm, s.info.res0)
+    err_vs_fp64 = None
+    if x64 is not None:
+        err_vs_fp64 = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+    log(f"fast path: {fast_t:.3f}s per solve, {fast_iters} FGMRES steps, rel res {rel_res:.2e}")
+
+    # --- CPU baseline: the FP64 oracle on the host cores (bounded sample) ----
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        Ch = p["C"]
+        uh = u.cpu().numpy()
+        O.fapply(uh, Ch, p["sym"], p["bc"], p["dx"])
+        reps = []
+        for _ in range(5):
+            t = time.perf_counter()
+            O.fapply(uh, Ch, p["sym"], p["bc"], p["dx"])
+            reps.append(time.perf_counter() - t)
+        cpu_fapply_s = float(np.median(reps))
+        rhs_h = rhs.cpu().numpy()
+        t = time.perf_counter()
+        _, info = O.mlmg_solve(rhs_h, Ch, p["sym"], p["bc"], p["dx"], tol_rel=args.tol, max_iter=2, cycle="F")
+        cpu_cycle_s = (time.perf_counter() - t) / 2.0
+        plain_iters = plain.get("F", {}).get("iters", 44)
+        cpu = dict(value=cpu_cycle_s * plain_iters, unit="s", cores=O.num_threads(), kind="port",
+                   sample=(f"FP64 oracle: 2 F-cycles of the same {p['name']} problem "
+                           f"({cpu_cycle_s:.2f} s/cycle) scaled to the {plain_iters} cycles of the plain FP64 "
+                           f"solve; Fapply median of 5 = {cpu_fapply_s * 1e3:.0f} ms "
+                           f"({bytes_per_launch / cpu_fapply_s / 1e9:.1f} GB/s)"),
+                   fapply_gbs=bytes_per_launch / cpu_fapply_s / 1e9)
+        log(f"cpu baseline: {cpu['sample']}")
+
+    out = {
+        "metric": METRIC,
+        "value": fast_t,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fast_t * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-in-fp64-IR",
+        "data": "synthetic",
+        "config": {"workload": f"{p['name']} cubic C11=1.7 C12=1.2 C44=0.75, Haar rotations, uniaxial 1e-3 in z",
+                   "n_cell": args.n_cell, "nodes": nodes, "tol_rel": args.tol,
+                   "path": "FP32 F-cycle preconditioned FGMRES(30), FP64 residual, Krylov basis and direct coarsest solve"},
+        "fast_path": {"seconds": fast_t, "fgmres_steps": fast_iters, "rel_res": rel_res,
+                      "rel_l2_err_vs_fp64": err_vs_fp64},
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64_F": (plain["F"]["seconds"] / fast_t) if "F" in plain else None,
+        "fapply": {"us_per_launch": fapply_ms * 1e3, "gbs": fapply_gbs, "bytes_per_node": FAPPLY_BYTES_PER_NODE[p["sym"]]},
+        "roofline": {"bound": "hbm", "achieved": fapply_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": fapply_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "k_fapply<double,MajorMinor>"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * alamo_elastic.h -- C-ABI of the MI355X nodal elastic multigrid library
+ * (libalamo_elastic.so).  It replaces, for one GPU, the call path
+ *
+ *   Solver::Nonlocal::Newton<T>::solve            src/Solver/Nonlocal/Newton.H:279
+ *    -> Solver::Nonlocal::Linear::solve           src/Solver/Nonlocal/Linear.H:131
+ *       -> amrex::MLMG -> Operator::Elastic<SYM>  src/Operator/Elastic.H:21-190
+ *
+ * (paths relative to the reference root).  Conventions:
+ *   - Every call returns an int status: 0 success, ALAMO_EL_NOT_CONVERGED (1)
+ *     when a solve stopped at max_iter, negative on errors.  Nothing aborts
+ *     or throws across this boundary; alamo_el_last_error() describes the
+ *     last failure on the calling thread.
+ *   - Fields are caller-allocated DEVICE pointers (hipMalloc'd or from any
+ *     allocator on the current device) unless a name ends in _host.
+ *     One box per level; nodes n = n_cell + 1 per axis; node index i
+ *     fastest, then j, then k; vector fields hold 3 components, component
+ *     slowest: v[c*N + (k*ny + j)*nx + i].
+ *   - Stiffness symmetry tags (template argument SYM of Operator::Elastic,
+ *     src/Operator/Elastic.cpp:928-931):
+ *       ALAMO_EL_ISOTROPIC  2 values per node  (lambda, mu)
+ *       ALAMO_EL_MAJORMINOR 21 values per node (slots of Set::Matrix4<3,MajorMinor>,
+ *                           src/Set/Matrix4_MajorMinor.H:327-375)
+ *       ALAMO_EL_DIAGONAL   9 values per node  (A[a][b] of Set::Matrix4<3,Diagonal>)
+ *     Layout ALAMO_EL_SOA: C[s*N + node]; ALAMO_EL_AOS: C[node*ncomp + s],
+ *     the memory order of amrex::BaseFab<Set::Matrix4<3,SYM>>.
+ *   - Boundary table: 26 regions x 3 components of type and value, region
+ *     order of BC::Operator::Elastic::Elastic::Face
+ *     (src/BC/Operator/Elastic/Elastic.H:35-43); types ALAMO_EL_BC_*.
+ *   - Every launch goes to the hipStream_t passed in (0 = null stream).
+ *     Calls on one handle are not thread safe; distinct handles are.
+ */
+#ifndef ALAMO_ELASTIC_H
+#define ALAMO_ELASTIC_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALAMO_EL_OK 0
+#define ALAMO_EL_NOT_CONVERGED 1
+#define ALAMO_EL_EINVAL (-1)
+#define ALAMO_EL_EHIP (-2)
+#define ALAMO_EL_ENAN (-3)
+#define ALAMO_EL_ESTATE (-4)
+
+#define ALAMO_EL_ISOTROPIC 0
+#define ALAMO_EL_MAJORMINOR 1
+#define ALAMO_EL_DIAGONAL 2
+
+#define ALAMO_EL_SOA 0
+#define ALAMO_EL_AOS 1
+
+#define ALAMO_EL_BC_DISPLACEMENT 0
+#define ALAMO_EL_BC_TRACTION 1
+#define ALAMO_EL_BC_PERIODIC 2
+#define ALAMO_EL_BC_NEUMANN 3
+#define ALAMO_EL_BC_NONE 4
+
+#define ALAMO_EL_PREC_FP64 0      /* plain FP64 multigrid */
+#define ALAMO_EL_PREC_FP32_IR 1   /* FP32 cycles inside FP64 iterative refinement */
+
+#define ALAMO_EL_CYCLE_F 0
+#define ALAMO_EL_CYCLE_V 1
+
+#define ALAMO_EL_OUTER_MLMG 0     /* x += cycle(b - A x) until converged */
+#define ALAMO_EL_OUTER_FGMRES 1   /* flexible GMRES, one cycle as preconditioner */
+
+#define ALAMO_EL_BOTTOM_BICGSTAB 0 /* the reference bottom solver (Linear.H:258) */
+#define ALAMO_EL_BOTTOM_DIRECT 1   /* FP64 inverse of the coarsest operator */
+
+typedef struct alamo_el_s* alamo_el_t;
+
+/* MLMG settings; alamo_el_default_opts fills the reference defaults
+ * (src/Solver/Nonlocal/Linear.H:255-290 and the amrex::MLMG defaults). */
+typedef struct {
+    int max_iter;          /* 200 */
+    int nu1;               /* 2  Fsmooth calls before restriction (pre_smooth)  */
+    int nu2;               /* 2  Fsmooth calls after interpolation (post_smooth) */
+    int cycle;             /* ALAMO_EL_CYCLE_F: setMaxFmgIter(1e8), Linear.H:261 */
+    int bottom_max_iter;   /* 200 */
+    int final_smooth;      /* 8  smooths after a failed bottom solve */
+    double omega;          /* 2/3, src/Operator/Operator.H:123 */
+    double bottom_tol_rel; /* 1e-4 */
+    int precision;         /* ALAMO_EL_PREC_* */
+    int outer;             /* ALAMO_EL_OUTER_* */
+    int restart;           /* FGMRES restart length (default 30) */
+    int verbose;           /* 0 */
+    int bottom_solver;     /* ALAMO_EL_BOTTOM_BICGSTAB */
+} alamo_el_opts;
+
+typedef struct {
+    int status;            /* same as the return value of alamo_el_solve */
+    int iters;             /* outer iterations (MLMG cycles or FGMRES steps) */
+    double res0;           /* |b - A x0|_inf */
+    double rhsnorm;        /* |b|_inf */
+    double res;            /* final |b - A x|_inf (FP64) */
+    double seconds;        /* wall time of the solve on the host clock */
+} alamo_el_info;
+
+const char* alamo_el_last_error(void);
+void alamo_el_default_opts(alamo_el_opts* opts);
+
+/* Operator::Elastic<SYM>::Elastic + define (src/Operator/Elastic.cpp:10-55):
+ * one AMR level, MG hierarchy coarsened by 2 while every axis keeps at least
+ * 2 cells, or to max_coarsening_level (-1 = no limit; LPInfo,
+ * src/Integrator/Base/Mechanics.H:190-193).  dx = cell size on the finest
+ * level.  The handle owns every coefficient, diagonal and work array. */
+int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_coarsening_level,
+                    alamo_el_t* out);
+int alamo_el_destroy(alamo_el_t h);
+int alamo_el_num_levels(alamo_el_t h, int* nlev);
+int alamo_el_level_nodes(alamo_el_t h, int mglev, int nodes[3]);
+
+/* Operator::Elastic::SetBC with a BC::Operator::Elastic::Constant table
+ * (src/Operator/Elastic.H:61-65, src/BC/Operator/Elastic/Constant.H). */
+int alamo_el_set_bc(alamo_el_t h, const int type_host[26 * 3], const double val_host[26 * 3]);
+/* Operator::Elastic::SetUniform (src/Operator/Elastic.H:105). */
+int alamo_el_set_uniform(alamo_el_t h, int uniform);
+/* Operator<Grid::Node>::SetOmega (src/Operator/Operator.H:57). */
+int alamo_el_set_omega(alamo_el_t h, double omega);
+/* Operator::Elastic::SetModel(amrlev, FabArray<BaseFab<Matrix4>>): copies the
+ * caller's per-node stiffness into the operator's own storage for MG level 0
+ * (src/Operator/Elastic.cpp:87-116). */
+int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t stream);
+/* prepareForSolve: averageDownCoeffsSameAmrLevel on every MG level and
+ * Diagonal(true) (src/Operator/Operator.cpp:253-260,
+ * src/Operator/Elastic.cpp:605-626,746-910). */
+int alamo_el_prepare(alamo_el_t h, hipStream_t stream);
+
+/* Operator::Elastic::Fapply, f = A u on MG level mglev
+ * (src/Operator/Elastic.cpp:139-292). */
+int alamo_el_fapply(alamo_el_t h, int mglev, const double* u, double* f, hipStream_t stream);
+/* Operator::Elastic::Diagonal (src/Operator/Elastic.cpp:296-379); also cached
+ * by alamo_el_prepare. */
+int alamo_el_diagonal(alamo_el_t h, int mglev, double* diag, hipStream_t stream);
+/* Operator<Grid::Node>::Fsmooth called nsmooth times: 2*nsmooth damped
+ * Jacobi sweeps of x against b (src/Operator/Operator.cpp:88-154).  Needs
+ * alamo_el_prepare. */
+int alamo_el_fsmooth(alamo_el_t h, int mglev, double* x, const double* b, int nsmooth, hipStream_t stream);
+/* correctionResidual / solutionResidual: r = b - A x
+ * (src/Operator/Operator.cpp:635-657). */
+int alamo_el_residual(alamo_el_t h, int mglev, const double* x, const double* b, double* r, hipStream_t stream);
+/* Operator<Grid::Node>::restriction: crse (level cmglev) = R fine (level
+ * cmglev-1) (src/Operator/Operator.cpp:262-363). */
+int alamo_el_restrict(alamo_el_t h, int cmglev, double* crse, const double* fine, hipStream_t stream);
+/* Operator<Grid::Node>::interpolation: fine (level fmglev) += P crse
+ * (src/Operator/Operator.cpp:365-434). */
+int alamo_el_interp_add(alamo_el_t h, int fmglev, double* fine, const double* crse, hipStream_t stream);
+
+/* BC::Operator::Elastic::Constant::Init on the finest level: boundary nodes
+ * of rhs get the boundary values, or 0 for displacement components when
+ * homogeneous (src/BC/Operator/Elastic/Constant.H:197-286). */
+int alamo_el_bc_init(alamo_el_t h, double* rhs, int homogeneous, hipStream_t stream);
+/* Newton<T>::prepareForSolve right-hand side, psi unset
+ * (src/Solver/Nonlocal/Newton.H:151-231): rhs = b - BC(u, grad u, DW) on the
+ * boundary and b - div DW in the interior, with DW = C:(grad u - F0).  F0 is
+ * NULL or 9 values per node (F0[(3a+b)*N + node]); C uses the layout passed. */
+int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double* F0, const double* u,
+                        const double* b, double* rhs, hipStream_t stream);
+
+/* Solver::Nonlocal::Linear::solve(sol, rhs, tol_rel, tol_abs)
+ * (src/Solver/Nonlocal/Linear.H:131-147): solves A x = b on the finest level
+ * to |b - A x|_inf <= max(tol_abs, tol_rel * max(|b|_inf, |b - A x0|_inf)).
+ * x holds the initial guess on entry.  opts may be NULL (defaults).  Needs
+ * alamo_el_set_model; calls alamo_el_prepare when the model changed. */
+int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, double tol_abs,
+                   const alamo_el_opts* opts, alamo_el_info* info, hipStream_t stream);
+
+/* |v|_inf over 3 components of a level-mglev vector, on the host. */
+int alamo_el_norm_inf(alamo_el_t h, int mglev, const double* v, double* out_host, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

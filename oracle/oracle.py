@@ -1,0 +1,174 @@
+"""ctypes bindings of the FP64 CPU oracle (liboracle_elastic.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline, never as a product path.  Arrays are numpy float64 with shape
+(comp, nz, ny, nx); see elastic_oracle.h for the conventions.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_elastic.so")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++ -fopenmp)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def _nodes(u):
+    nz, ny, nx = u.shape[-3:]
+    return np.array([nx, ny, nz], dtype=np.int32)
+
+
+def _dx(dx):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(dx, dtype=np.float64), (3,)))
+
+
+def _bct(bc):
+    return np.ascontiguousarray(bc.type, dtype=np.int32)
+
+
+def _check(rc, name):
+    if rc < 0:
+        raise ValueError(f"{name} failed with status {rc}")
+    return rc
+
+
+def fapply(u, C, sym, bc, dx, uniform=False):
+    u, C = _f64(u), _f64(C)
+    f = np.empty_like(u)
+    _check(lib().ora_fapply(_i(_nodes(u)), _d(_dx(dx)), sym, _d(C), _i(_bct(bc)), int(uniform), _d(u), _d(f)),
+           "ora_fapply")
+    return f
+
+
+def diagonal(C, sym, bc, dx, shape):
+    C = _f64(C)
+    d = np.empty((3,) + tuple(shape))
+    _check(lib().ora_diagonal(_i(_nodes(d)), _d(_dx(dx)), sym, _d(C), _i(_bct(bc)), _d(d)), "ora_diagonal")
+    return d
+
+
+def fsmooth(x, b, C, sym, bc, dx, diag, omega=2.0 / 3.0, nsmooth=1, uniform=False):
+    x = _f64(x).copy()
+    b, C, diag = _f64(b), _f64(C), _f64(diag)
+    _check(lib().ora_fsmooth(_i(_nodes(x)), _d(_dx(dx)), sym, _d(C), _i(_bct(bc)), int(uniform), _d(diag),
+                             ctypes.c_double(omega), int(nsmooth), _d(x), _d(b)), "ora_fsmooth")
+    return x
+
+
+def residual(x, b, C, sym, bc, dx, uniform=False):
+    x, b, C = _f64(x), _f64(b), _f64(C)
+    r = np.empty_like(x)
+    _check(lib().ora_residual(_i(_nodes(x)), _d(_dx(dx)), sym, _d(C), _i(_bct(bc)), int(uniform), _d(x), _d(b),
+                              _d(r)), "ora_residual")
+    return r
+
+
+def coarse_shape(shape):
+    return tuple((s - 1) // 2 + 1 for s in shape)
+
+
+def restrict(fine):
+    fine = _f64(fine)
+    crse = np.empty((fine.shape[0],) + coarse_shape(fine.shape[1:]))
+    _check(lib().ora_restrict(_i(_nodes(fine)), fine.shape[0], _d(fine), _d(crse)), "ora_restrict")
+    return crse
+
+
+def interp_add(crse, fine):
+    fine = _f64(fine).copy()
+    crse = _f64(crse)
+    _check(lib().ora_interp_add(_i(_nodes(fine)), fine.shape[0], _d(crse), _d(fine)), "ora_interp_add")
+    return fine
+
+
+def average_down_C(Cf, sym):
+    Cf = _f64(Cf)
+    Cc = np.empty((Cf.shape[0],) + coarse_shape(Cf.shape[1:]))
+    _check(lib().ora_average_down_C(_i(_nodes(Cf)), sym, _d(Cf), _d(Cc)), "ora_average_down_C")
+    return Cc
+
+
+def bc_init(shape, bc, homogeneous=False, rhs=None):
+    rhs = np.zeros((3,) + tuple(shape)) if rhs is None else _f64(rhs).copy()
+    bcv = np.ascontiguousarray(bc.val, dtype=np.float64)
+    _check(lib().ora_bc_init(_i(_nodes(rhs)), _i(_bct(bc)), _d(bcv), int(homogeneous), _d(rhs)), "ora_bc_init")
+    return rhs
+
+
+def newton_rhs(u, b, C, sym, bc, dx, F0=None):
+    u, b, C = _f64(u), _f64(b), _f64(C)
+    rhs = np.empty_like(u)
+    f0p = _d(_f64(F0)) if F0 is not None else _dp()
+    _check(lib().ora_newton_rhs(_i(_nodes(u)), _d(_dx(dx)), sym, _d(C), f0p, _i(_bct(bc)), _d(u), _d(b), _d(rhs)),
+           "ora_newton_rhs")
+    return rhs
+
+
+def stress(u, C, sym, dx, F0=None):
+    u, C = _f64(u), _f64(C)
+    sig = np.empty((9,) + u.shape[1:])
+    f0p = _d(_f64(F0)) if F0 is not None else _dp()
+    _check(lib().ora_stress(_i(_nodes(u)), _d(_dx(dx)), sym, _d(C), f0p, _d(u), _d(sig)), "ora_stress")
+    return sig
+
+
+def mlmg_solve(b, C, sym, bc, dx, x0=None, tol_rel=1e-10, tol_abs=0.0, max_iter=200, nu1=2, nu2=2,
+               cycle="F", bottom_max_iter=200, final_smooth=8, max_coarsening_level=-1, omega=2.0 / 3.0,
+               bottom_tol_rel=1e-4, uniform=False, fcycle_inject=True):
+    """Returns (x, info) with info = dict(status, iters, res, res0, rhsnorm, hist)."""
+    b, C = _f64(b), _f64(C)
+    x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
+    opts = np.array([max_iter, nu1, nu2, 0 if cycle == "F" else 1, bottom_max_iter, final_smooth,
+                     max_coarsening_level, int(fcycle_inject)], dtype=np.int32)
+    ropts = np.array([tol_rel, tol_abs, omega, bottom_tol_rel], dtype=np.float64)
+    iters = ctypes.c_int(0)
+    res = np.zeros(3)
+    hist = np.zeros(max_iter + 1)
+    rc = lib().ora_mlmg_solve(_i(_nodes(b)), _d(_dx(dx)), sym, _d(C), _i(_bct(bc)), int(uniform), _i(opts),
+                              _d(ropts), _d(x), _d(b), ctypes.byref(iters), _d(res), _d(hist))
+    _check(rc, "ora_mlmg_solve")
+    return x, dict(status=rc, iters=iters.value, res=res[0], res0=res[1], rhsnorm=res[2],
+                   hist=hist[: iters.value + 1].copy())
+
+
+def num_threads():
+    return lib().ora_num_threads()

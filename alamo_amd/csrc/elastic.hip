@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -21,6 +22,7 @@
 #include "../../include/alamo_elastic.h"
 #include "bottom_kernels.h"
 #include "elastic_kernels.h"
+#include "stencil.h"
 
 using ael::kBlock;
 using ael::Lvl;
@@ -62,6 +64,7 @@ struct Level {
     Lvl L;
     Vecs<double> d;
     Vecs<float> f;
+    unsigned char* gmask = nullptr;  // per node: axes along which C varies (k_grad_mask)
 };
 
 template <typename T> Vecs<T>& vecs(Level& l);
@@ -113,36 +116,62 @@ constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M do
 // ---------------------------------------------------------------------------
 // Kernel dispatch
 // ---------------------------------------------------------------------------
-template <typename T>
-void launch_fapply(alamo_el_t h, const Level& l, const T* u, const T* C, T* f, hipStream_t s) {
+// Stencil kernel variant: the XCD-slab per-node kernel with the grad-C mask
+// (default), the same without the mask, or the linear per-node kernel.
+// ALAMO_EL_STENCIL=slab|slab-nomask|node selects one; all three give the same
+// result bit for bit.
+enum { kVariantSlab = 0, kVariantSlabNoMask = 1, kVariantNode = 2 };
+
+int stencil_variant() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = std::getenv("ALAMO_EL_STENCIL");
+        v = kVariantSlab;
+        if (e && std::strcmp(e, "slab-nomask") == 0) v = kVariantSlabNoMask;
+        if (e && std::strcmp(e, "node") == 0) v = kVariantNode;
+    }
+    return v;
+}
+
+template <typename T, int OP>
+void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const T* diag, const T* C, T* out,
+                    hipStream_t s) {
     const Lvl L = l.L;
+    const T w = (T)h->omega;
+    const int u = h->uniform;
+    const int var = stencil_variant();
+    if (var == kVariantNode) {
+        const unsigned g = nblk8(L.N);
+        switch (h->sym) {
+            case 0: ael::k_stencil_node<T, 0, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, out); break;
+            case 1: ael::k_stencil_node<T, 1, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, out); break;
+            default: ael::k_stencil_node<T, 2, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, out); break;
+        }
+        return;
+    }
+    const unsigned g = ael::slab_blocks(L, kBlock);
+    const unsigned char* gm = var == kVariantSlab ? l.gmask : nullptr;
     switch (h->sym) {
-        case 0: ael::k_fapply<T, 0><<<nblk8(L.N), kBlock, 0, s>>>(u, C, h->bct_dev, h->uniform, L, f); break;
-        case 1: ael::k_fapply<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(u, C, h->bct_dev, h->uniform, L, f); break;
-        default: ael::k_fapply<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(u, C, h->bct_dev, h->uniform, L, f); break;
+        case 0: ael::k_stencil_slab<T, 0, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, out); break;
+        case 1: ael::k_stencil_slab<T, 1, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, out); break;
+        default: ael::k_stencil_slab<T, 2, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, out); break;
     }
 }
 
 template <typename T>
+void launch_fapply(alamo_el_t h, const Level& l, const T* u, const T* C, T* f, hipStream_t s) {
+    launch_stencil<T, 0>(h, l, u, (const T*)nullptr, (const T*)nullptr, C, f, s);
+}
+
+template <typename T>
 void launch_residual(alamo_el_t h, const Level& l, const T* x, const T* b, const T* C, T* r, hipStream_t s) {
-    const Lvl L = l.L;
-    switch (h->sym) {
-        case 0: ael::k_residual<T, 0><<<nblk8(L.N), kBlock, 0, s>>>(x, b, C, h->bct_dev, h->uniform, L, r); break;
-        case 1: ael::k_residual<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(x, b, C, h->bct_dev, h->uniform, L, r); break;
-        default: ael::k_residual<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(x, b, C, h->bct_dev, h->uniform, L, r); break;
-    }
+    launch_stencil<T, 1>(h, l, x, b, (const T*)nullptr, C, r, s);
 }
 
 template <typename T>
 void launch_jacobi(alamo_el_t h, const Level& l, const T* x, const T* b, const T* diag, const T* C, T* xo,
                    hipStream_t s) {
-    const Lvl L = l.L;
-    const T w = (T)h->omega;
-    switch (h->sym) {
-        case 0: ael::k_jacobi<T, 0><<<nblk8(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, h->uniform, L, w, xo); break;
-        case 1: ael::k_jacobi<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, h->uniform, L, w, xo); break;
-        default: ael::k_jacobi<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, h->uniform, L, w, xo); break;
-    }
+    launch_stencil<T, 2>(h, l, x, b, diag, C, xo, s);
 }
 
 template <typename T>
@@ -398,6 +427,14 @@ int assemble_dense(alamo_el_t h, hipStream_t s) {
     return 0;
 }
 
+void build_gmask(alamo_el_t h, Level& l, hipStream_t s) {
+    switch (h->sym) {
+        case 0: ael::k_grad_mask<2><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, l.gmask); break;
+        case 1: ael::k_grad_mask<21><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, l.gmask); break;
+        default: ael::k_grad_mask<9><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, l.gmask); break;
+    }
+}
+
 int prepare(alamo_el_t h, hipStream_t s) {
     if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
     // averageDownCoeffsSameAmrLevel (src/Operator/Elastic.cpp:746-910)
@@ -405,6 +442,7 @@ int prepare(alamo_el_t h, hipStream_t s) {
         Level& c = h->lv[l];
         Level& f = h->lv[l - 1];
         ael::k_restrict<double><<<nblk8(c.L.N), kBlock, 0, s>>>(f.d.C, f.L, c.d.C, c.L, h->nc);
+        build_gmask(h, c, s);
     }
     // Diagonal(true) on every MG level (src/Operator/Operator.cpp:16-29)
     for (auto& l : h->lv) launch_diagonal<double>(h, l, l.d.C, l.d.diag, s);
@@ -468,6 +506,9 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
     if (ncomp_of(sym) < 0) return fail(ALAMO_EL_EINVAL, "unknown stiffness symmetry");
     for (int a = 0; a < 3; ++a)
         if (n_cell[a] < 2 || !(dx[a] > 0)) return fail(ALAMO_EL_EINVAL, "n_cell must be >= 2 and dx > 0");
+    // The stencil kernels address each field component with a 32-bit byte offset.
+    if ((long)(n_cell[0] + 1) * (n_cell[1] + 1) * (n_cell[2] + 1) >= (1L << 29))
+        return fail(ALAMO_EL_EINVAL, "box too large for one handle (at most 2^29 nodes)");
     alamo_el_t h = new alamo_el_s();
     *out = nullptr;
     h->sym = sym;
@@ -496,6 +537,7 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
         const long N = l.L.N;
         int rc;
         if ((rc = alloc((void**)&l.d.C, sizeof(double) * h->nc * N))) return cleanup(rc);
+        if ((rc = alloc((void**)&l.gmask, N))) return cleanup(rc);
         for (double** p : {&l.d.diag, &l.d.cor, &l.d.cor_hold, &l.d.res, &l.d.rescor, &l.d.tmp})
             if ((rc = alloc((void**)p, sizeof(double) * 3 * N))) return cleanup(rc);
     }
@@ -524,6 +566,7 @@ int alamo_el_destroy(alamo_el_t h) {
             if (p) (void)hipFree(p);
         for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp})
             if (p) (void)hipFree(p);
+        if (l.gmask) (void)hipFree(l.gmask);
     }
     for (void* p : {(void*)h->bct_dev, (void*)h->bcv_dev, (void*)h->red_partial, (void*)h->red_out,
                     (void*)h->bottom_work, (void*)h->bottom_status, (void*)h->gu9, (void*)h->dw9, (void*)h->csoa,
@@ -589,6 +632,8 @@ int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t s)
     } else {
         return fail(ALAMO_EL_EINVAL, "bad layout");
     }
+    build_gmask(h, l, s);
+    HIPCHK(hipGetLastError());
     h->model_set = true;
     h->prepared = false;
     h->prepared32 = false;

@@ -202,14 +202,171 @@ __device__ __forceinline__ T d1(const T* __restrict__ f, long id, long stride, i
 }
 
 // Elastic<SYM>::Fapply at one node (src/Operator/Elastic.cpp:165-290),
+// Pair index P <-> (pair_a(P), pair_b(P)) of (00,01,02,11,12,22), and the
+// MajorMinor slot s <-> (slot_p(s), slot_q(s)), P <= Q, of the 6x6 pair matrix.
+__host__ __device__ constexpr int pair_a(int P) { return P < 3 ? 0 : (P < 5 ? 1 : 2); }
+__host__ __device__ constexpr int pair_b(int P) { return P < 3 ? P : (P < 5 ? P - 2 : 2); }
+// The r with vpair(r, a) == P, or -1 when a is not an index of pair P.
+__host__ __device__ constexpr int pair_other(int P, int a) {
+    return pair_a(P) == a ? pair_b(P) : (pair_b(P) == a ? pair_a(P) : -1);
+}
+__host__ __device__ constexpr int slot_p(int s) { return s < 6 ? 0 : s < 11 ? 1 : s < 15 ? 2 : s < 18 ? 3 : s < 20 ? 4 : 5; }
+__host__ __device__ constexpr int slot_q(int s) {
+    return s < 6 ? s : s < 11 ? s - 5 : s < 15 ? s - 9 : s < 18 ? s - 12 : s < 20 ? s - 14 : 5;
+}
+constexpr bool slot_table_ok() {
+    for (int s = 0; s < 21; ++s)
+        if (tri6(slot_p(s), slot_q(s)) != s || slot_p(s) > slot_q(s)) return false;
+    for (int P = 0; P < 6; ++P)
+        if (vpair(pair_a(P), pair_b(P)) != P) return false;
+    return true;
+}
+static_assert(slot_table_ok(), "slot/pair tables disagree with tri6/vpair");
+
+// Slot s of dC/dx_a enters column a of (dC/dx_a) : grad u.
+__host__ __device__ constexpr bool slot_in_col(int s, int a) {
+    return pair_other(slot_p(s), a) >= 0 || pair_other(slot_q(s), a) >= 0;
+}
+
+// Interior row of Fapply for MajorMinor C, the same sums as contract3 and
+// contract2_col but streamed over the 21 slots: each slot is loaded once,
+// used for every term it enters, and dropped, so only the 18 second
+// derivative combinations e[j][Q], the 6 symmetric gradients e1[Q] and the
+// result stay live (the load of slot s+1 overlaps the math of slot s).
+// Load base[idx] with a 32-bit element index: with base uniform, the address
+// is a scalar base plus a 32-bit byte offset (no 64-bit vector address math).
+// Every array indexed this way holds fewer than 2^32 bytes (checked at create).
+template <typename T>
+__device__ __forceinline__ T ld32(const T* __restrict__ base, unsigned idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* __restrict__ base, unsigned idx, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(idx * (unsigned)sizeof(T))) = v;
+}
+
+// Interior row of Fapply for MajorMinor C, the same sums as contract3 and
+// contract2_col but streamed over the 21 slots: each slot is loaded once,
+// used for every term it enters, and dropped, so only the 18 second
+// derivative combinations e[j][Q], the 6 symmetric gradients e1[Q] and the
+// result stay live.  Grid spacings enter through reciprocals computed once.
+template <typename T>
+__device__ __forceinline__ void apply_interior_mm(const T* __restrict__ u, const T* __restrict__ C, const Lvl& L,
+                                                  unsigned o, T out[3], unsigned gmask, bool uniform) {
+    const unsigned N = (unsigned)L.N, sy = (unsigned)L.nx, sz = (unsigned)(L.nx * L.ny);
+    const T ihx = T(1) / T(L.dx), ihy = T(1) / T(L.dy), ihz = T(1) / T(L.dz);
+    const T hx2 = T(0.5) * ihx, hy2 = T(0.5) * ihy, hz2 = T(0.5) * ihz;
+    const T ixx = ihx * ihx, iyy = ihy * ihy, izz = ihz * ihz;
+    const T ixy = T(0.25) * ihx * ihy, ixz = T(0.25) * ihx * ihz, iyz = T(0.25) * ihy * ihz;
+    T gu[3][3], B[3][3][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const T* f = u + (size_t)p * N;
+        const T c0 = ld32(f, o);
+        const T xp = ld32(f, o + 1), xm = ld32(f, o - 1), yp = ld32(f, o + sy), ym = ld32(f, o - sy);
+        const T zp = ld32(f, o + sz), zm = ld32(f, o - sz);
+        gu[p][0] = (xp - xm) * hx2;
+        gu[p][1] = (yp - ym) * hy2;
+        gu[p][2] = (zp - zm) * hz2;
+        B[p][0][0] = (xp - T(2) * c0 + xm) * ixx;
+        B[p][1][1] = (yp - T(2) * c0 + ym) * iyy;
+        B[p][2][2] = (zp - T(2) * c0 + zm) * izz;
+        B[p][0][1] = (ld32(f, o + 1 + sy) + ld32(f, o - 1 - sy) - ld32(f, o + 1 - sy) - ld32(f, o - 1 + sy)) * ixy;
+        B[p][0][2] = (ld32(f, o + 1 + sz) + ld32(f, o - 1 - sz) - ld32(f, o + 1 - sz) - ld32(f, o - 1 + sz)) * ixz;
+        B[p][1][2] = (ld32(f, o + sy + sz) + ld32(f, o - sy - sz) - ld32(f, o + sy - sz) - ld32(f, o - sy + sz)) * iyz;
+        B[p][1][0] = B[p][0][1];
+        B[p][2][0] = B[p][0][2];
+        B[p][2][1] = B[p][1][2];
+    }
+    T e[3][6], e1[6];
+#pragma unroll
+    for (int Q = 0; Q < 6; ++Q) {
+        const int qa = pair_a(Q), qb = pair_b(Q);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) e[j][Q] = qa == qb ? B[qa][qa][j] : B[qa][qb][j] + B[qb][qa][j];
+        e1[Q] = qa == qb ? gu[qa][qa] : gu[qa][qb] + gu[qb][qa];
+    }
+    out[0] = out[1] = out[2] = T(0);
+    // C : grad grad u  (src/Set/Matrix4_MajorMinor.H:709-767)
+#pragma unroll
+    for (int s = 0; s < 21; ++s) {
+        const int P = slot_p(s), Q = slot_q(s);
+        const T c = ld32(C + (size_t)s * N, o);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (pair_other(P, j) >= 0) out[pair_other(P, j)] += c * e[j][Q];
+            if (Q != P && pair_other(Q, j) >= 0) out[pair_other(Q, j)] += c * e[j][P];
+        }
+    }
+    if (uniform) return;
+    // + sum_a ((dC/dx_a) : grad u).col(a)  (src/Operator/Elastic.cpp:245-254)
+    const unsigned st[3] = {1u, sy, sz};
+    const T h2[3] = {hx2, hy2, hz2};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!((gmask >> a) & 1u)) continue;
+#pragma unroll
+        for (int s = 0; s < 21; ++s) {
+            if (!slot_in_col(s, a)) continue;
+            const int P = slot_p(s), Q = slot_q(s);
+            const T* Cs = C + (size_t)s * N;
+            const T d = (ld32(Cs, o + st[a]) - ld32(Cs, o - st[a])) * h2[a];
+            if (pair_other(P, a) >= 0) out[pair_other(P, a)] += d * e1[Q];
+            if (Q != P && pair_other(Q, a) >= 0) out[pair_other(Q, a)] += d * e1[P];
+        }
+    }
+}
+
+// sigma = C : A for MajorMinor C, streamed over the slots (contract2).
+template <typename T>
+__device__ __forceinline__ void contract2_mm_stream(const T* __restrict__ C, long N, long id, const T A[3][3],
+                                                    T sg[3][3]) {
+    T e1[6], sp[6];
+#pragma unroll
+    for (int Q = 0; Q < 6; ++Q) {
+        const int qa = pair_a(Q), qb = pair_b(Q);
+        e1[Q] = qa == qb ? A[qa][qa] : A[qa][qb] + A[qb][qa];
+        sp[Q] = T(0);
+    }
+#pragma unroll
+    for (int s = 0; s < 21; ++s) {
+        const int P = slot_p(s), Q = slot_q(s);
+        const T c = C[s * N + id];
+        sp[P] += c * e1[Q];
+        if (Q != P) sp[Q] += c * e1[P];
+    }
+#pragma unroll
+    for (int P = 0; P < 6; ++P) sg[pair_a(P)][pair_b(P)] = sg[pair_b(P)][pair_a(P)] = sp[P];
+}
+
 // psi unset (psi_avg = 1).
 template <typename T, int SYM>
 __device__ __forceinline__ void apply_node(const T* __restrict__ u, const T* __restrict__ C,
                                            const int* __restrict__ bct, int uniform, const Lvl& L, int i, int j,
-                                           int k, long id, T out[3]) {
+                                           int k, long id, T out[3], unsigned gmask = 7u) {
     constexpr int NC = NComp<SYM>::value;
     const long N = L.N, sy = L.nx, sz = (long)L.nx * L.ny;
     const T hx = T(L.dx), hy = T(L.dy), hz = T(L.dz);
+    if constexpr (SYM == 1) {
+        T nrm[3];
+        const int reg = region_of<T>(i, j, k, L, nrm);
+        if (reg < 0) {
+            apply_interior_mm<T>(u, C, L, (unsigned)id, out, gmask, uniform != 0);
+            return;
+        }
+        T uu[3], gu[3][3], sg[3][3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const T* f = u + p * N;
+            uu[p] = f[id];
+            gu[p][0] = d1(f, id, 1, i, L.nx, hx);
+            gu[p][1] = d1(f, id, sy, j, L.ny, hy);
+            gu[p][2] = d1(f, id, sz, k, L.nz, hz);
+        }
+        contract2_mm_stream<T>(C, N, id, gu, sg);
+        bc_set<T>(bct + 3 * reg, uu, gu, sg, nrm, out);
+        return;
+    }
     T c[NC];
 #pragma unroll
     for (int s = 0; s < NC; ++s) c[s] = C[s * N + id];
@@ -256,6 +413,9 @@ __device__ __forceinline__ void apply_node(const T* __restrict__ u, const T* __r
         const T h[3] = {hx, hy, hz};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
+            // Bit a of gmask clear: C(+a) and C(-a) are equal slot by slot, so
+            // the central difference and its contribution are exactly zero.
+            if (!((gmask >> a) & 1u)) continue;
             T cg[NC], col[3];
 #pragma unroll
             for (int s = 0; s < NC; ++s) cg[s] = (C[s * N + id + st[a]] - C[s * N + id - st[a]]) * T(0.5) / h[a];
@@ -264,58 +424,6 @@ __device__ __forceinline__ void apply_node(const T* __restrict__ u, const T* __r
             out[1] += col[1];
             out[2] += col[2];
         }
-    }
-}
-
-template <typename T, int SYM>
-__global__ void __launch_bounds__(kBlock) k_fapply(const T* __restrict__ u, const T* __restrict__ C,
-                                                   const int* __restrict__ bct, int uniform, Lvl L,
-                                                   T* __restrict__ f) {
-    const long id = xcd_node();
-    if (id >= L.N) return;
-    int i, j, k;
-    node_ijk(id, L, i, j, k);
-    T o[3];
-    apply_node<T, SYM>(u, C, bct, uniform, L, i, j, k, id, o);
-    f[id] = o[0];
-    f[L.N + id] = o[1];
-    f[2 * L.N + id] = o[2];
-}
-
-// r = b - A x  (src/Operator/Operator.cpp:635-657: apply, then Xpay with -1).
-template <typename T, int SYM>
-__global__ void __launch_bounds__(kBlock) k_residual(const T* __restrict__ x, const T* __restrict__ b,
-                                                     const T* __restrict__ C, const int* __restrict__ bct,
-                                                     int uniform, Lvl L, T* __restrict__ r) {
-    const long id = xcd_node();
-    if (id >= L.N) return;
-    int i, j, k;
-    node_ijk(id, L, i, j, k);
-    T o[3];
-    apply_node<T, SYM>(x, C, bct, uniform, L, i, j, k, id, o);
-#pragma unroll
-    for (int p = 0; p < 3; ++p) r[p * L.N + id] = b[p * L.N + id] + T(-1) * o[p];
-}
-
-// One damped Jacobi sweep, out of place (src/Operator/Operator.cpp:109-146):
-// x_out = (1-w) x + w (b - (A x - D x)) / D.
-template <typename T, int SYM>
-__global__ void __launch_bounds__(kBlock) k_jacobi(const T* __restrict__ x, const T* __restrict__ b,
-                                                   const T* __restrict__ diag, const T* __restrict__ C,
-                                                   const int* __restrict__ bct, int uniform, Lvl L, T omega,
-                                                   T* __restrict__ xo) {
-    const long id = xcd_node();
-    if (id >= L.N) return;
-    int i, j, k;
-    node_ijk(id, L, i, j, k);
-    T o[3];
-    apply_node<T, SYM>(x, C, bct, uniform, L, i, j, k, id, o);
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-        const long q = p * L.N + id;
-        const T xv = x[q], dv = diag[q];
-        const T Rx = o[p] - xv * dv;
-        xo[q] = (T(1) - omega) * xv + omega * (b[q] - Rx) / dv;
     }
 }
 

@@ -1,0 +1,127 @@
+// stencil.h -- the nodal elastic operator kernels used by every MG level.
+//
+// One thread per node.  The launch is XCD-slab ordered (slab_node) so that the
+// planes a node reads stay in one XCD's L2, and the optional grad-C mask skips
+// the neighbour loads of axes along which C does not change.
+//
+// OP 0: out = A x            (Elastic::Fapply, src/Operator/Elastic.cpp:139-292)
+// OP 1: out = b - A x        (correctionResidual, src/Operator/Operator.cpp:647-657)
+// OP 2: out = (1-w) x + w (b - (A x - D x)) / D   (one Fsmooth sweep,
+//                              src/Operator/Operator.cpp:109-146)
+#pragma once
+#include "elastic_kernels.h"
+
+namespace ael {
+
+template <typename T, int OP>
+__device__ __forceinline__ T finish_op(T Ax, T xv, T bv, T dv, T omega) {
+    if constexpr (OP == 0) return Ax;
+    else if constexpr (OP == 1) return bv + T(-1) * Ax;
+    else return (T(1) - omega) * xv + omega * (bv - (Ax - xv * dv)) / dv;
+}
+
+// One thread per node over the whole box (XCD-remapped), any OP.
+template <typename T, int SYM, int OP>
+__global__ void __launch_bounds__(kBlock)
+    k_stencil_node(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
+                   const T* __restrict__ C, const int* __restrict__ bct, int uniform, Lvl L, T omega,
+                   T* __restrict__ out) {
+    const long id = xcd_node();
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    T o[3];
+    apply_node<T, SYM>(x, C, bct, uniform, L, i, j, k, id, o);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long qq = p * L.N + id;
+        const T xv = (OP == 2) ? x[qq] : T(0);
+        const T bv = (OP >= 1) ? b[qq] : T(0);
+        const T dv = (OP == 2) ? diag[qq] : T(1);
+        out[qq] = finish_op<T, OP>(o[p], xv, bv, dv, omega);
+    }
+}
+
+// XCD-slab node mapping.  Workgroups are dispatched to the kXcd XCDs round
+// robin (workgroup b runs on XCD b % kXcd), each XCD with its own L2.  XCD x
+// is given the rows j in [x*ny/kXcd, (x+1)*ny/kXcd) of every plane and walks
+// that slab in (k, j, i) order, so a node's k-1 and k+1 neighbours were read
+// a few workgroups earlier, or will be read a few later, on the same L2: u and
+// C are fetched from HBM about once (plus the two rows each slab shares with
+// its neighbours) instead of once per plane that reads them.
+__device__ __forceinline__ bool slab_node(const Lvl& L, int& i, int& j, int& k, long& id) {
+    const int x = (int)(blockIdx.x % kXcd);
+    const long lb = blockIdx.x / kXcd;
+    const int j0 = (int)((long)x * L.ny / kXcd), j1 = (int)((long)(x + 1) * L.ny / kXcd);
+    const long per_plane = (long)(j1 - j0) * L.nx;
+    const long t = lb * blockDim.x + threadIdx.x;
+    if (t >= per_plane * L.nz) return false;
+    k = (int)(t / per_plane);
+    const long r = t - (long)k * per_plane;
+    const int jj = (int)(r / L.nx);
+    i = (int)(r - (long)jj * L.nx);
+    j = j0 + jj;
+    id = ((long)k * L.ny + j) * L.nx + i;
+    return true;
+}
+
+// Workgroups of the XCD-slab launch: kXcd times the blocks of the tallest slab.
+inline unsigned slab_blocks(const Lvl& L, int block) {
+    const long rows = (L.ny + kXcd - 1) / kXcd;
+    const long per_xcd = (rows * L.nx * L.nz + block - 1) / block;
+    return (unsigned)(per_xcd * kXcd);
+}
+
+// One thread per node, XCD-slab order, any OP.  gmask (optional, one byte per
+// node) holds bit a set where C(+a) != C(-a) in some slot; the grad-C
+// contribution of a clear axis is exactly zero and its 2 x NC neighbour loads
+// are skipped.
+template <typename T, int SYM, int OP>
+__global__ void __launch_bounds__(kBlock)
+    k_stencil_slab(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
+                   const T* __restrict__ C, const int* __restrict__ bct, int uniform, Lvl L, T omega,
+                   const unsigned char* __restrict__ gmask, T* __restrict__ out) {
+    int i, j, k;
+    long id;
+    if (!slab_node(L, i, j, k, id)) return;
+    T o[3];
+    const unsigned gm = gmask ? (unsigned)gmask[id] : 7u;
+    apply_node<T, SYM>(x, C, bct, uniform, L, i, j, k, id, o, gm);
+    const unsigned n32 = (unsigned)id;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const size_t off = (size_t)p * L.N;
+        const T xv = (OP == 2) ? ld32(x + off, n32) : T(0);
+        const T bv = (OP >= 1) ? ld32(b + off, n32) : T(0);
+        const T dv = (OP == 2) ? ld32(diag + off, n32) : T(1);
+        st32(out + off, n32, finish_op<T, OP>(o[p], xv, bv, dv, omega));
+    }
+}
+
+// gmask of every interior node: bit a set iff some slot of C differs between
+// the two neighbours along axis a (bitwise compare of the FP64 values, so the
+// mask also holds for any rounding of C to FP32).  Boundary nodes get 7.
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_grad_mask(const double* __restrict__ C, Lvl L,
+                                                      unsigned char* __restrict__ gmask) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    if (i == 0 || j == 0 || k == 0 || i == L.nx - 1 || j == L.ny - 1 || k == L.nz - 1) {
+        gmask[id] = 7;
+        return;
+    }
+    const long st[3] = {1, (long)L.nx, (long)L.nx * L.ny};
+    unsigned m = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        bool diff = false;
+        for (int s = 0; s < NC; ++s)
+            diff |= __double_as_longlong(C[s * L.N + id + st[a]]) != __double_as_longlong(C[s * L.N + id - st[a]]);
+        m |= diff ? (1u << a) : 0u;
+    }
+    gmask[id] = (unsigned char)m;
+}
+
+}  // namespace ael

@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--fapply-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference solves")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -53,8 +55,77 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+FAPPLY_KERNEL = "k_stencil_slab<double, 1, 0>"
+COPY_BYTES = 1 << 28  # calibration copy: 256 MiB read and 256 MiB written
+
+
+def pmc_child(args):
+    """Workload of one traffic pass under rocprofv3 --pmc: the calibration copy
+    (an elementwise kernel of known size) and 3 Fapply launches."""
+    import torch
+
+    from alamo_amd import elastic as el
+    from alamo_amd import problems
+    p = problems.polycrystal(args.n_cell, ngrains=args.grains, seed=args.seed)
+    op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
+    op.SetBC(p["bc"])
+    op.SetModel(torch.from_numpy(p["C"]).cuda())
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    torch.add(a, 1.0, out=c)
+    u = torch.randn((3,) + op.level_shape(0), dtype=torch.float64, device="cuda")
+    f = torch.empty_like(u)
+    for _ in range(3):
+        op.Fapply(u, out=f)
+    torch.cuda.synchronize()
+
+
+def measure_traffic(args):
+    """HBM bytes per Fapply launch from two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE), each scaled by what the counter reads for the calibration
+    copy.  Returns a dict, or None when rocprofv3 is unavailable or fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    out = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="alamo_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cmd = ["timeout", "-s", "KILL", "120", "rocprofv3", "--pmc", ctr, "--output-format", "csv", "-d", d,
+               "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
+               "--n-cell", str(args.n_cell), "--grains", str(args.grains), "--seed", str(args.seed)]
+        log(f"traffic pass {ctr}")
+        r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, cwd=d)
+        files = [os.path.join(dp, fn) for dp, _, fns in os.walk(d) for fn in fns if fn.endswith("counter_collection.csv")]
+        if r.returncode != 0 or not files:
+            log(f"traffic pass {ctr} failed (rc {r.returncode}): {r.stderr[-300:]}")
+            return None
+        fap, cal = [], []
+        for row in csv.DictReader(open(files[0])):
+            if row["Counter_Name"] != ctr:
+                continue
+            if FAPPLY_KERNEL in row["Kernel_Name"]:
+                fap.append(float(row["Counter_Value"]))
+            elif "elementwise" in row["Kernel_Name"]:
+                cal.append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if not fap or not cal:
+            log(f"traffic pass {ctr}: kernels not found")
+            return None
+        kib = float(np.mean(fap))
+        cal_kib = max(cal)  # the 256 MiB copy is the largest elementwise kernel of the pass
+        factor = (COPY_BYTES / 1024.0) / cal_kib
+        out[ctr] = dict(raw_kib=kib, calib_kib=cal_kib, factor=factor, bytes=kib * 1024.0 * factor)
+    return out
+
+
 def main():
     args = parse()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
         sys.exit(2)
@@ -132,12 +203,20 @@ def main():
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
     fast_t = float(np.mean(times))
+    fast_iters_all = s.info.iters
     fast_iters = s.info.iters
     rel_res = s.info.res / max(s.info.rhsnorm, s.info.res0)
     err_vs_fp64 = None
     if x64 is not None:
         err_vs_fp64 = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
     log(f"fast path: {fast_t:.3f}s per solve, {fast_iters} FGMRES steps, rel res {rel_res:.2e}")
+
+    traffic = None if args.no_traffic else measure_traffic(args)
+    traffic_bytes = None
+    if traffic:
+        traffic_bytes = traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]
+        log(f"Fapply HBM traffic {traffic_bytes / 1e6:.1f} MB per launch "
+            f"({traffic_bytes / bytes_per_launch:.2f}x the algorithmic {bytes_per_launch / 1e6:.1f} MB)")
 
     # --- CPU baseline: the FP64 oracle on the host cores (bounded sample) ----
     cpu = None
@@ -147,7 +226,7 @@ def main():
         uh = u.cpu().numpy()
         O.fapply(uh, Ch, p["sym"], p["bc"], p["dx"])
         reps = []
-        for _ in range(5):
+        for _ in range(20):
             t = time.perf_counter()
             O.fapply(uh, Ch, p["sym"], p["bc"], p["dx"])
             reps.append(time.perf_counter() - t)
@@ -160,7 +239,7 @@ def main():
         cpu = dict(value=cpu_cycle_s * plain_iters, unit="s", cores=O.num_threads(), kind="port",
                    sample=(f"FP64 oracle: 2 F-cycles of the same {p['name']} problem "
                            f"({cpu_cycle_s:.2f} s/cycle) scaled to the {plain_iters} cycles of the plain FP64 "
-                           f"solve; Fapply median of 5 = {cpu_fapply_s * 1e3:.0f} ms "
+                           f"solve; Fapply median of 20 = {cpu_fapply_s * 1e3:.0f} ms "
                            f"({bytes_per_launch / cpu_fapply_s / 1e9:.1f} GB/s)"),
                    fapply_gbs=bytes_per_launch / cpu_fapply_s / 1e9)
         log(f"cpu baseline: {cpu['sample']}")
@@ -184,10 +263,14 @@ def main():
         "fast_path": {"seconds": fast_t, "fgmres_steps": fast_iters, "rel_res": rel_res,
                       "rel_l2_err_vs_fp64": err_vs_fp64},
         "plain_fp64": plain,
+        "speedup_vs_plain_fp64_V": (plain["V"]["seconds"] / fast_t) if "V" in plain else None,
         "speedup_vs_plain_fp64_F": (plain["F"]["seconds"] / fast_t) if "F" in plain else None,
         "fapply": {"us_per_launch": fapply_ms * 1e3, "gbs": fapply_gbs, "bytes_per_node": FAPPLY_BYTES_PER_NODE[p["sym"]]},
         "roofline": {"bound": "hbm", "achieved": fapply_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": fapply_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "k_fapply<double,MajorMinor>"},
+                     "frac": fapply_gbs / PEAK_HBM_GBS, "traffic": traffic_bytes,
+                     "kernel": "k_stencil_slab<double,MajorMinor,Fapply>",
+                     "algorithmic_bytes": bytes_per_launch,
+                     "traffic_passes": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -17,8 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libalamo_elastic.so")
 ARCH = os.environ.get("ALAMO_AMD_ARCH", "gfx950")
 
-SOURCES = ["elastic.hip"]
-HEADERS = ["elastic_kernels.h", "bottom_kernels.h"]
+SOURCES = ["elastic.hip", "unet.hip"]
+HEADERS = ["elastic_kernels.h", "bottom_kernels.h", "stencil.h", "common.h", "unet_kernels.h"]
 
 # Packed FP32 instructions are disabled for all device code: hipcc -O3 forms
 # them from ordinary float code and some of their forms have given wrong

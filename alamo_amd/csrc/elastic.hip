@@ -21,20 +21,19 @@
 
 #include "../../include/alamo_elastic.h"
 #include "bottom_kernels.h"
+#include "common.h"
 #include "elastic_kernels.h"
 #include "stencil.h"
 
 using ael::kBlock;
 using ael::Lvl;
 
+thread_local std::string alamo_detail::g_err;
+
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
+using alamo_detail::fail;
+using alamo_detail::g_err;
 
 #define HIPCHK(expr)                                                                         \
     do {                                                                                     \

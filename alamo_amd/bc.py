@@ -94,6 +94,27 @@ def uniaxial(axis: int, strain: float, length: float = 1.0) -> Constant:
     return bc
 
 
+def shear(normal: int, direction: int, strain: float, length: float = 1.0) -> Constant:
+    """Simple shear of a box: every region on the lo face of ``normal`` is
+    clamped, every region on its hi face moves by strain*length along
+    ``direction`` with the other components held at 0, and the remaining
+    faces are traction free."""
+    if normal == direction:
+        raise ValueError("shear needs direction != normal")
+    names = "xyz"
+    bc = Constant()
+    bc.type[:] = TRACTION
+    bc.val[:] = 0.0
+    for r, name in enumerate(FACES):
+        parts = _split(name)
+        if names[normal] + "lo" in parts:
+            bc.type[r, :] = DISPLACEMENT
+        elif names[normal] + "hi" in parts:
+            bc.type[r, :] = DISPLACEMENT
+            bc.val[r, direction] = strain * length
+    return bc
+
+
 def tension_test(type_: str = "uniaxial_stress_clamp", disp: float | None = None, trac: float | None = None) -> Constant:
     """BC::Operator::Elastic::TensionTest::Parse (src/BC/Operator/Elastic/TensionTest.H:48-306)
     for one load value: tension along x with ``disp`` (displacement of the

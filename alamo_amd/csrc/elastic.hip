@@ -20,9 +20,11 @@
 #include <vector>
 
 #include "../../include/alamo_elastic.h"
+#include "../../include/alamo_unet.h"
 #include "bottom_kernels.h"
 #include "common.h"
 #include "elastic_kernels.h"
+#include "learned_kernels.h"
 #include "stencil.h"
 
 using ael::kBlock;
@@ -104,6 +106,15 @@ struct alamo_el_s {
     double* kv = nullptr;  // (m+1) vectors
     double* kz = nullptr;  // m vectors
     double* kw = nullptr;
+    // learned coarse-level solver (alamo_el_set_learned); the network is the caller's
+    alamo_nn_t nn = nullptr;
+    int nn_level = -1;
+    float* nn_in = nullptr;    // network input [nc + 30][N_L]
+    float* nn_out = nullptr;   // network output [3][N_L]
+    double* nn_bc = nullptr;   // level-L right-hand side
+    double* nn_t = nullptr;    // scaled right-hand side, capacity nn_cap nodes
+    double* nn_fmean = nullptr;  // 18 face means of nn_t
+    long nn_cap = 0;
 };
 
 namespace {
@@ -469,10 +480,138 @@ int prepare32(alamo_el_t h, hipStream_t s) {
 
 bool bad_level(alamo_el_t h, int l) { return !h || l < 0 || l >= (int)h->lv.size(); }
 
+// ---------------------------------------------------------------------------
+// Learned coarse-level solver (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// Level-L right-hand side of a fine-level b: the interior rows restricted by
+// the MG full weighting (src/Operator/Operator.cpp:262-363) level by level,
+// the boundary rows injected from the fine node each coarse node sits on.
+int coarse_rhs(alamo_el_t h, int L, const double* b, double* bc, hipStream_t s) {
+    Level& l0 = h->lv[0];
+    if (L == 0) {
+        launch_copy<double>(bc, b, 3 * l0.L.N, s);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    ael::k_interior_only<<<nblk(l0.L.N), kBlock, 0, s>>>(b, l0.L, l0.d.tmp);
+    const double* src = l0.d.tmp;
+    for (int l = 1; l <= L; ++l) {
+        double* dst = l == L ? bc : h->lv[l].d.tmp;
+        ael::k_restrict<double><<<nblk8(h->lv[l].L.N), kBlock, 0, s>>>(src, h->lv[l - 1].L, dst, h->lv[l].L, 3);
+        src = dst;
+    }
+    ael::k_inject_boundary<<<nblk(h->lv[L].L.N), kBlock, 0, s>>>(b, l0.L, 1 << L, bc, h->lv[L].L);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Network inputs beside the nc stiffness slots (learned_kernels.h k_features).
+constexpr int kLearnedExtra = 30;
+
+int ensure_nn_scratch(alamo_el_t h, long N) {
+    if (h->nn_cap >= N) return 0;
+    if (h->nn_t) (void)hipFree(h->nn_t);
+    h->nn_t = nullptr;
+    h->nn_cap = 0;
+    HIPCHK(hipMalloc((void**)&h->nn_t, sizeof(double) * 3 * N));
+    if (!h->nn_fmean) HIPCHK(hipMalloc((void**)&h->nn_fmean, sizeof(double) * 18));
+    h->nn_cap = N;
+    return 0;
+}
+
+// Network input of level L from its right-hand side bc (learned_kernels.h);
+// *scale = s, 0 when bc is zero.
+int learned_input(alamo_el_t h, int L, const double* bc, float* feat, double* scale, hipStream_t s) {
+    Level& l = h->lv[L];
+    const long N = l.L.N;
+    int rc;
+    if ((rc = ensure_nn_scratch(h, N))) return rc;
+    double csum;
+    if ((rc = reduce_host<double>(h, l.d.C, l.d.C, N, 2, &csum, s))) return rc;
+    const double alpha = csum / (double)N;
+    if (!(alpha > 0)) return fail(ALAMO_EL_EINVAL, "learned input: the mean of stiffness slot 0 is not positive");
+    ael::k_prescale<<<nblk(N), kBlock, 0, s>>>(bc, l.L, 1.0 / alpha, h->nn_t);
+    double smax;
+    if ((rc = reduce_host<double>(h, h->nn_t, h->nn_t, 3 * N, 0, &smax, s))) return rc;
+    const double sc = smax > 0 ? smax : 1.0;
+    ael::k_face_means<<<18, kBlock, 0, s>>>(h->nn_t, l.L, h->nn_fmean);
+    ael::k_features<<<nblk(N), kBlock, 0, s>>>(l.d.C, h->nc, h->nn_t, h->bct_dev, h->nn_fmean, l.L, 1.0 / alpha,
+                                                1.0 / sc, feat);
+    HIPCHK(hipGetLastError());
+    *scale = smax;
+    return 0;
+}
+
+// x = P^L (s * net(input(coarse_rhs(b)))), trilinear interpolation level by
+// level (src/Operator/Operator.cpp:365-434); x = 0 when b has no level-L part.
+int learned_apply(alamo_el_t h, const double* b, double* x, hipStream_t s) {
+    const int L = h->nn_level;
+    Level& lL = h->lv[L];
+    const long NL = lL.L.N;
+    int rc;
+    if ((rc = coarse_rhs(h, L, b, h->nn_bc, s))) return rc;
+    double sc;
+    if ((rc = learned_input(h, L, h->nn_bc, h->nn_in, &sc, s))) return rc;
+    if (sc == 0.0) {
+        launch_fill<double>(x, 0.0, 3 * h->lv[0].L.N, s);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    if ((rc = alamo_nn_forward(h->nn, h->nn_in, h->nn_out, s))) return rc;
+    ael::k_scale_output<<<nblk(3 * NL), kBlock, 0, s>>>(h->nn_out, 3 * NL, sc, L == 0 ? x : lL.d.cor);
+    for (int l = L - 1; l >= 0; --l) {
+        double* dst = l == 0 ? x : h->lv[l].d.cor;
+        ael::k_interp<double><<<nblk8(h->lv[l].L.N), kBlock, 0, s>>>(h->lv[l + 1].d.cor, h->lv[l + 1].L, dst,
+                                                                      h->lv[l].L, 3, 1);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+constexpr int kLearnedSmooth = 2;  // Fsmooth calls applied to a learned correction
+
+// One judged learned correction.  On entry l0.d.res = b - A x and *rnorm is
+// its inf-norm.  The network proposes e = P^L N(R^L r), smoothed by
+// kLearnedSmooth Fsmooth calls on A e = r; the step length minimizes the
+// 2-norm of r - g A e, and the step is taken only when it lowers |r|_inf (the
+// multigrid residual is the judge).  *taken tells which.
+int learned_step(alamo_el_t h, double* x, double* rnorm, int* taken, hipStream_t s) {
+    Level& l0 = h->lv[0];
+    const long M0 = 3 * l0.L.N;
+    double* r = l0.d.res;
+    double* e = l0.d.cor_hold;
+    double* w = l0.d.rescor;
+    int rc;
+    *taken = 0;
+    if ((rc = learned_apply(h, r, e, s))) return rc;
+    for (int q = 0; q < kLearnedSmooth; ++q) {
+        launch_jacobi<double>(h, l0, e, r, l0.d.diag, l0.d.C, l0.d.tmp, s);
+        launch_jacobi<double>(h, l0, l0.d.tmp, r, l0.d.diag, l0.d.C, e, s);
+    }
+    launch_fapply<double>(h, l0, e, l0.d.C, w, s);
+    double rw, ww;
+    if ((rc = reduce_host<double>(h, r, w, M0, 1, &rw, s))) return rc;
+    if ((rc = reduce_host<double>(h, w, w, M0, 1, &ww, s))) return rc;
+    if (!(ww > 0.0) || std::isnan(rw)) return 0;
+    const double g = rw / ww;
+    // w = r - g A e, the residual after the step
+    ael::k_scale_copy<double><<<nblk(M0), kBlock, 0, s>>>(w, -g, w, M0);
+    launch_axpy<double>(w, 1.0, r, M0, s);
+    double rn;
+    if ((rc = reduce_host<double>(h, w, w, M0, 0, &rn, s))) return rc;
+    if (!(rn < *rnorm)) return 0;
+    launch_axpy<double>(x, g, e, M0, s);
+    launch_copy<double>(r, w, M0, s);
+    HIPCHK(hipGetLastError());
+    *rnorm = rn;
+    *taken = 1;
+    return 0;
+}
+
 // Flexible GMRES(m) with one multigrid cycle as right preconditioner; the
 // stopping test is the inf-norm of the true FP64 residual.
 int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const alamo_el_opts& o, int& iters,
-                 double& resnorm, hipStream_t s);
+                 double& resnorm, int& learned_taken, hipStream_t s);
 
 }  // namespace
 
@@ -494,6 +633,7 @@ void alamo_el_default_opts(alamo_el_opts* o) {
     o->omega = 2.0 / 3.0;
     o->bottom_tol_rel = 1e-4;
     o->precision = ALAMO_EL_PREC_FP64;
+    o->learned = 0;
     o->outer = ALAMO_EL_OUTER_MLMG;
     o->restart = 30;
     o->verbose = 0;
@@ -570,7 +710,8 @@ int alamo_el_destroy(alamo_el_t h) {
     for (void* p : {(void*)h->bct_dev, (void*)h->bcv_dev, (void*)h->red_partial, (void*)h->red_out,
                     (void*)h->bottom_work, (void*)h->bottom_status, (void*)h->gu9, (void*)h->dw9, (void*)h->csoa,
                     (void*)h->kv, (void*)h->kz, (void*)h->kw, (void*)h->kh, (void*)h->kpart,
-                    (void*)h->dense_I, (void*)h->dense_A, (void*)h->dense_Ainv})
+                    (void*)h->dense_I, (void*)h->dense_A, (void*)h->dense_Ainv, (void*)h->nn_in, (void*)h->nn_out,
+                    (void*)h->nn_bc, (void*)h->nn_t, (void*)h->nn_fmean})
         if (p) (void)hipFree(p);
     if (h->red_host) (void)hipHostFree(h->red_host);
     if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
@@ -749,6 +890,65 @@ int alamo_el_norm_inf(alamo_el_t h, int mglev, const double* v, double* out_host
     return reduce_host<double>(h, v, v, 3 * h->lv[mglev].L.N, 0, out_host, s);
 }
 
+int alamo_el_level_model(alamo_el_t h, int mglev, double* C, hipStream_t s) {
+    if (bad_level(h, mglev) || !C) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    int rc;
+    if (mglev > 0 && !h->prepared && (rc = prepare(h, s))) return rc;
+    const Level& l = h->lv[mglev];
+    HIPCHK(hipMemcpyAsync(C, l.d.C, sizeof(double) * h->nc * l.L.N, hipMemcpyDeviceToDevice, s));
+    return 0;
+}
+
+int alamo_el_coarse_rhs(alamo_el_t h, int mglev, const double* b, double* bc, hipStream_t s) {
+    if (bad_level(h, mglev) || !b || !bc || b == bc) return fail(ALAMO_EL_EINVAL, "bad argument");
+    return coarse_rhs(h, mglev, b, bc, s);
+}
+
+int alamo_el_learned_input(alamo_el_t h, int mglev, const double* bc, float* features, double* scale_host,
+                           hipStream_t s) {
+    if (bad_level(h, mglev) || !bc || !features || !scale_host) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    int rc;
+    if (mglev > 0 && !h->prepared && (rc = prepare(h, s))) return rc;
+    return learned_input(h, mglev, bc, features, scale_host, s);
+}
+
+int alamo_el_set_learned(alamo_el_t h, alamo_nn_t nn, int mglev) {
+    if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
+    for (void* p : {(void*)h->nn_in, (void*)h->nn_out, (void*)h->nn_bc})
+        if (p) (void)hipFree(p);
+    h->nn_in = nullptr;
+    h->nn_out = nullptr;
+    h->nn_bc = nullptr;
+    h->nn = nullptr;
+    h->nn_level = -1;
+    if (!nn) return 0;
+    if (bad_level(h, mglev)) return fail(ALAMO_EL_EINVAL, "bad level");
+    int nodes[3], n_in, n_out;
+    int rc;
+    if ((rc = alamo_nn_shape(nn, nodes, &n_in, &n_out))) return rc;
+    const Lvl& L = h->lv[mglev].L;
+    if (nodes[0] != L.nx || nodes[1] != L.ny || nodes[2] != L.nz || n_in != h->nc + kLearnedExtra || n_out != 3)
+        return fail(ALAMO_EL_EINVAL, "network grid or channels do not match the level (needs nc + 30 inputs, 3 outputs)");
+    HIPCHK(hipMalloc((void**)&h->nn_in, sizeof(float) * n_in * L.N));
+    HIPCHK(hipMalloc((void**)&h->nn_out, sizeof(float) * 3 * L.N));
+    HIPCHK(hipMalloc((void**)&h->nn_bc, sizeof(double) * 3 * L.N));
+    if ((rc = ensure_nn_scratch(h, L.N))) return rc;
+    h->nn = nn;
+    h->nn_level = mglev;
+    return 0;
+}
+
+int alamo_el_learned_guess(alamo_el_t h, const double* b, double* x, hipStream_t s) {
+    if (!h || !b || !x) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (!h->nn) return fail(ALAMO_EL_ESTATE, "no network set (alamo_el_set_learned)");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    int rc;
+    if (!h->prepared && (rc = prepare(h, s))) return rc;
+    return learned_apply(h, b, x, s);
+}
+
 int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, double tol_abs,
                    const alamo_el_opts* opts_in, alamo_el_info* info, hipStream_t s) {
     if (!h || !x || !b) return fail(ALAMO_EL_EINVAL, "null argument");
@@ -773,15 +973,32 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
     int iters = 0, status = ALAMO_EL_NOT_CONVERGED;
     double nrm = resnorm0;  // fgmres_solve reads it as the inf-norm of r0
     if (std::isnan(resnorm0)) return fail(ALAMO_EL_ENAN, "NaN in the initial residual");
-    if (resnorm0 <= target) {
+    // Learned initial guess: a judged learned correction of the caller's x
+    // (learned_step); the stopping target stays the one of the caller's x.
+    int learned_taken = 0;
+    const bool learned_pre = (o.learned & ALAMO_EL_LEARNED_PRECOND) && h->nn;
+    if ((o.learned & ALAMO_EL_LEARNED_GUESS) && h->nn && resnorm0 > target) {
+        int taken;
+        if ((rc = learned_step(h, x, &nrm, &taken, s))) return rc;
+        learned_taken += taken;
+        if (o.verbose) std::fprintf(stderr, "alamo_el: learned guess %s, |r|_inf %.6e\n", taken ? "taken" : "rejected", nrm);
+    }
+    if (nrm <= target) {
         status = 0;
     } else if (o.outer == ALAMO_EL_OUTER_FGMRES) {
-        status = fgmres_solve(h, x, b, target, o, iters, nrm, s);
+        status = fgmres_solve(h, x, b, target, o, iters, nrm, learned_taken, s);
         if (status < 0) return status;
     } else {
         Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
         Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
         for (int it = 0; it < o.max_iter; ++it) {
+            if (learned_pre && it > 0) {
+                // learned coarse correction between cycles, judged like the guess
+                int taken;
+                if ((rc = learned_step(h, x, &nrm, &taken, s))) return rc;
+                learned_taken += taken;
+                if (nrm <= target) { iters = it; status = 0; break; }
+            }
             if (o.precision == ALAMO_EL_PREC_FP32_IR) {
                 // FP32 cycle on the FP64 residual; FP64 correction accumulation
                 ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(l0.d.res, l0.f.res, M0);
@@ -802,6 +1019,7 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
     HIPCHK(hipGetLastError());
     if (info) {
         info->status = status;
+        info->learned_steps = learned_taken;
         info->iters = iters;
         info->res0 = resnorm0;
         info->rhsnorm = rhsnorm0;
@@ -821,7 +1039,7 @@ namespace {
 // the preconditioned directions Z are stored).  Modified Gram-Schmidt with
 // dot products reduced on the host.  The Krylov basis is FP64.
 int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const alamo_el_opts& o, int& iters,
-                 double& resnorm, hipStream_t s) {
+                 double& resnorm, int& learned_taken, hipStream_t s) {
     Level& l0 = h->lv[0];
     const long M0 = 3 * l0.L.N;
     const int m = std::max(2, std::min(o.restart, 100));
@@ -845,7 +1063,16 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
     int rc;
     iters = 0;
     // r = b - A x is in l0.d.res on entry
+    bool first = true;
     while (iters < o.max_iter) {
+        if ((o.learned & ALAMO_EL_LEARNED_PRECOND) && h->nn && !first) {
+            // learned coarse correction at each restart, judged like the guess
+            int taken;
+            if ((rc = learned_step(h, x, &resnorm, &taken, s))) return rc;
+            learned_taken += taken;
+            if (resnorm <= target) return 0;
+        }
+        first = false;
         double beta2;
         if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 1, &beta2, s))) return rc;
         const double beta = std::sqrt(beta2);

@@ -603,7 +603,7 @@ __global__ void __launch_bounds__(kBlock) k_aos_to_soa(const double* __restrict_
 }
 
 // Two-pass reductions: per-block partials written with vector stores, then
-// one block folds them.  op 0: max |v|, op 1: sum a*b.
+// one block folds them.  op 0: max |v|, op 1: sum a*b, op 2: sum a.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_reduce_partial(const T* __restrict__ a, const T* __restrict__ b, long n,
                                                            int op, double* __restrict__ partial) {
@@ -611,7 +611,8 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partial(const T* __restrict__
     double acc = 0.0;
     for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
         if (op == 0) acc = fmax(acc, fabs((double)a[id]));
-        else acc += (double)a[id] * (double)b[id];
+        else if (op == 1) acc += (double)a[id] * (double)b[id];
+        else acc += (double)a[id];
     }
     sh[threadIdx.x] = acc;
     __syncthreads();

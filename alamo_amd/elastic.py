@@ -44,12 +44,13 @@ class Opts(ctypes.Structure):
                 ("cycle", ctypes.c_int), ("bottom_max_iter", ctypes.c_int), ("final_smooth", ctypes.c_int),
                 ("omega", ctypes.c_double), ("bottom_tol_rel", ctypes.c_double), ("precision", ctypes.c_int),
                 ("outer", ctypes.c_int), ("restart", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("bottom_solver", ctypes.c_int)]
+                ("bottom_solver", ctypes.c_int), ("learned", ctypes.c_int)]
 
 
 class Info(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iters", ctypes.c_int), ("res0", ctypes.c_double),
-                ("rhsnorm", ctypes.c_double), ("res", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("rhsnorm", ctypes.c_double), ("res", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("learned_steps", ctypes.c_int)]
 
 
 EXPORTS = [
@@ -57,8 +58,10 @@ EXPORTS = [
     "alamo_el_num_levels", "alamo_el_level_nodes", "alamo_el_set_bc", "alamo_el_set_uniform",
     "alamo_el_set_omega", "alamo_el_set_model", "alamo_el_prepare", "alamo_el_fapply", "alamo_el_diagonal",
     "alamo_el_fsmooth", "alamo_el_residual", "alamo_el_restrict", "alamo_el_interp_add", "alamo_el_bc_init",
-    "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf",
+    "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
+    "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess",
 ]
+LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
 _lib = None
 
@@ -97,6 +100,11 @@ def lib():
         L.alamo_el_num_levels.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.alamo_el_level_nodes.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.alamo_el_default_opts.argtypes = [ctypes.POINTER(Opts)]
+        L.alamo_el_level_model.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.alamo_el_coarse_rhs.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.alamo_el_learned_input.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(ctypes.c_double), vp]
+        L.alamo_el_set_learned.argtypes = [vp, vp, ctypes.c_int]
+        L.alamo_el_learned_guess.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -256,6 +264,45 @@ class ElasticOperator:
         _check(lib().alamo_el_norm_inf(self._h, mglev, self._v(v, mglev), ctypes.byref(out), _stream()), "alamo_el_norm_inf")
         return out.value
 
+    # --- learned coarse-level solver (no counterpart in the reference) -----
+    def level_model(self, mglev: int):
+        """Stiffness of MG level mglev, SoA (ncomp, nz, ny, nx)."""
+        torch = _torch()
+        C = torch.empty((self.ncomp,) + self.level_shape(mglev), dtype=torch.float64, device="cuda")
+        _check(lib().alamo_el_level_model(self._h, mglev, _ptr(C), _stream()), "alamo_el_level_model")
+        return C
+
+    def coarse_rhs(self, b, mglev: int):
+        """Level-mglev right-hand side of the finest-level b."""
+        torch = _torch()
+        bc = torch.empty((3,) + self.level_shape(mglev), dtype=torch.float64, device="cuda")
+        _check(lib().alamo_el_coarse_rhs(self._h, mglev, self._v(b, 0), self._v(bc, mglev), _stream()),
+               "alamo_el_coarse_rhs")
+        return bc
+
+    def learned_input(self, bc, mglev: int):
+        """(features float32 (ncomp + 30, nz, ny, nx), scale s) of the
+        level-mglev right-hand side bc; the network predicts u / s."""
+        torch = _torch()
+        f = torch.empty((self.ncomp + 30,) + self.level_shape(mglev), dtype=torch.float32, device="cuda")
+        sc = ctypes.c_double()
+        _check(lib().alamo_el_learned_input(self._h, mglev, self._v(bc, mglev), ctypes.c_void_p(f.data_ptr()),
+                                            ctypes.byref(sc), _stream()), "alamo_el_learned_input")
+        return f, sc.value
+
+    def set_learned(self, net, mglev: int):
+        """Attach a learned.UNet working on MG level mglev (None detaches)."""
+        _check(lib().alamo_el_set_learned(self._h, net.h if net is not None else None, int(mglev)),
+               "alamo_el_set_learned")
+        self._net = net  # the handle keeps a pointer: keep the network alive
+
+    def learned_guess(self, b):
+        torch = _torch()
+        x = torch.empty_like(b)
+        _check(lib().alamo_el_learned_guess(self._h, self._v(b, 0), self._v(x, 0), _stream()),
+               "alamo_el_learned_guess")
+        return x
+
 
 def default_opts() -> Opts:
     o = Opts()
@@ -269,11 +316,13 @@ class Linear:
     reference's solver.* inputs (Linear::Parse, Linear.H:296-382) plus
     ``cycle`` ("F" = setMaxFmgIter(1e8), "V"), ``precision`` ("fp64",
     "fp32"), ``outer`` ("mlmg", "fgmres") and ``bottom_solver`` ("bicgstab",
-    the reference's; "direct", the FP64 inverse of the coarsest operator)."""
+    the reference's; "direct", the FP64 inverse of the coarsest operator) and
+    ``learned`` ("", "guess", "guess+precond": the learned coarse-level solver
+    attached with ElasticOperator.set_learned)."""
 
     def __init__(self, op: ElasticOperator, max_iter=200, pre_smooth=2, post_smooth=2, bottom_max_iter=200,
                  final_smooth=8, bottom_tol_rel=1e-4, omega=2.0 / 3.0, cycle="F", precision="fp64", outer="mlmg",
-                 restart=30, verbose=0, abort_on_fail=True, bottom_solver="bicgstab"):
+                 restart=30, verbose=0, abort_on_fail=True, bottom_solver="bicgstab", learned=""):
         self.op = op
         o = default_opts()
         o.max_iter, o.nu1, o.nu2 = int(max_iter), int(pre_smooth), int(post_smooth)
@@ -284,6 +333,8 @@ class Linear:
         o.outer = {"mlmg": OUTER_MLMG, "fgmres": OUTER_FGMRES}[outer]
         o.restart, o.verbose = int(restart), int(verbose)
         o.bottom_solver = {"bicgstab": 0, "direct": 1}[bottom_solver]
+        o.learned = {"": 0, "guess": LEARNED_GUESS, "precond": LEARNED_PRECOND,
+                     "guess+precond": LEARNED_GUESS | LEARNED_PRECOND}[learned]
         self.opts = o
         self.abort_on_fail = abort_on_fail
         self.info = Info()

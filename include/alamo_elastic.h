@@ -71,7 +71,15 @@ extern "C" {
 #define ALAMO_EL_BOTTOM_BICGSTAB 0 /* the reference bottom solver (Linear.H:258) */
 #define ALAMO_EL_BOTTOM_DIRECT 1   /* FP64 inverse of the coarsest operator */
 
+/* Learned coarse correction (alamo_el_set_learned): the network proposes
+ * e = P^L N(R^L r) for the residual r, smoothed by 2 Fsmooth calls; the step
+ * x += g e takes the g that minimizes |r - g A e|_2 and is kept only when it
+ * lowers |r|_inf (the multigrid residual is the judge). */
+#define ALAMO_EL_LEARNED_GUESS 1   /* one learned correction of the initial x */
+#define ALAMO_EL_LEARNED_PRECOND 2 /* one between cycles (MLMG) or at each FGMRES restart */
+
 typedef struct alamo_el_s* alamo_el_t;
+typedef struct alamo_nn_s* alamo_nn_t; /* alamo_unet.h */
 
 /* MLMG settings; alamo_el_default_opts fills the reference defaults
  * (src/Solver/Nonlocal/Linear.H:255-290 and the amrex::MLMG defaults). */
@@ -89,6 +97,7 @@ typedef struct {
     int restart;           /* FGMRES restart length (default 30) */
     int verbose;           /* 0 */
     int bottom_solver;     /* ALAMO_EL_BOTTOM_BICGSTAB */
+    int learned;           /* 0, or ALAMO_EL_LEARNED_* bits (needs alamo_el_set_learned) */
 } alamo_el_opts;
 
 typedef struct {
@@ -98,6 +107,7 @@ typedef struct {
     double rhsnorm;        /* |b|_inf */
     double res;            /* final |b - A x|_inf (FP64) */
     double seconds;        /* wall time of the solve on the host clock */
+    int learned_steps;     /* learned corrections taken (initial guess included) */
 } alamo_el_info;
 
 const char* alamo_el_last_error(void);
@@ -171,6 +181,28 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
 
 /* |v|_inf over 3 components of a level-mglev vector, on the host. */
 int alamo_el_norm_inf(alamo_el_t h, int mglev, const double* v, double* out_host, hipStream_t stream);
+
+/* --- Learned coarse-level solver.  No counterpart in the reference: these
+ * connect the network of alamo_unet.h to the multigrid hierarchy. --- */
+/* Copy of the stiffness of MG level mglev (SoA, as set by alamo_el_set_model
+ * and averaged down by alamo_el_prepare). */
+int alamo_el_level_model(alamo_el_t h, int mglev, double* C, hipStream_t stream);
+/* Level-mglev right-hand side of a finest-level b: interior rows restricted
+ * by the MG full weighting level by level, boundary rows injected. */
+int alamo_el_coarse_rhs(alamo_el_t h, int mglev, const double* b, double* bc, hipStream_t stream);
+/* Network input on level mglev from its right-hand side bc: float32
+ * [nc + 30][N]: C / alpha (alpha = mean of slot 0), interior bc / (alpha s),
+ * boundary bc / s, displacement-row indicators, node coordinates scaled to
+ * [0, 1], and 18 constant channels holding the mean of bc / s over each face
+ * (xlo, ylo, zlo, xhi, yhi, zhi) and component; *scale_host = s (the largest
+ * scaled magnitude, 0 for bc = 0).  The network predicts u / s. */
+int alamo_el_learned_input(alamo_el_t h, int mglev, const double* bc, float* features, double* scale_host,
+                           hipStream_t stream);
+/* Attach a network (NULL detaches) working on MG level mglev: its grid is
+ * that level's nodes, nc + 30 inputs, 3 outputs.  The handle does not own it. */
+int alamo_el_set_learned(alamo_el_t h, alamo_nn_t nn, int mglev);
+/* x = P^L (s * net(input(coarse_rhs(b)))) on the finest level. */
+int alamo_el_learned_guess(alamo_el_t h, const double* b, double* x, hipStream_t stream);
 
 #ifdef __cplusplus
 }

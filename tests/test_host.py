@@ -30,9 +30,13 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(path)
     names = declared("alamo_elastic.h")
     assert "alamo_el_solve" in names and "alamo_el_fapply" in names
-    for name in names:
+    nn_names = declared("alamo_unet.h")
+    assert "alamo_nn_forward" in nn_names and "alamo_nn_backward" in nn_names
+    for name in names + nn_names:
         assert hasattr(lib, name), name
+    from alamo_amd import learned
     assert set(elastic.EXPORTS) <= set(names)
+    assert set(learned.NN_EXPORTS) <= set(nn_names)
 
 
 def test_default_opts_are_the_reference_defaults():

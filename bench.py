@@ -6,8 +6,10 @@ the HBM roofline of the Fapply kernel.
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 One step = one full solve of the workload from a zero initial guess with the
-fast path (FP32 multigrid cycles inside FP64 flexible GMRES; the residual
-that decides convergence is always the FP64 one).  The plain FP64 multigrid
+learned + FP32 path: the trained network's judged initial correction
+(alamo_amd/weights), then FP32 multigrid cycles inside FP64 flexible GMRES;
+the residual that decides convergence is always the FP64 one.  The same
+solve without the network is timed beside it (the ablation).  The plain FP64 multigrid
 with the reference's settings (F-cycle every iteration; also the V-cycle) is
 solved once beside it, and the distance of the fast solution to the plain
 FP64 solution is reported.  Inputs are resident in HBM before any timing
@@ -47,6 +49,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference solves")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "alamo_amd", "weights", "unet_polycrystal129_level2.npz"))
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -185,31 +188,44 @@ def main():
             if cyc == "F" and s.info.status == 0:
                 x64 = x
 
-    # --- fast path: FP32 cycles inside FP64 FGMRES --------------------------
-    def fast_solve():
+    # --- learned + FP32 path, and FP32 alone -----------------------------------
+    learned = ""
+    if os.path.exists(args.weights) and args.n_cell == 128:
+        from alamo_amd import train
+        net, meta = train.load(args.weights)
+        op.set_learned(net, meta["level"])
+        learned = "guess"
+    else:
+        log("no trained network for this grid: the learned step is off")
+
+    def fast_solve(mode):
         x = torch.zeros_like(rhs)
-        s = el.Linear(op, cycle="F", precision="fp32", outer="fgmres", restart=30, bottom_solver="direct")
+        s = el.Linear(op, cycle="F", precision="fp32", outer="fgmres", restart=30, bottom_solver="direct",
+                      learned=mode)
         s.solve(x, rhs, args.tol, 0.0)
         return x, s
 
-    for _ in range(args.warmup):
-        fast_solve()
-    torch.cuda.synchronize()
-    times = []
-    for _ in range(args.steps):
+    def timed(mode):
+        for _ in range(args.warmup):
+            fast_solve(mode)
         torch.cuda.synchronize()
-        t = time.perf_counter()
-        x, s = fast_solve()
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t)
-    fast_t = float(np.mean(times))
-    fast_iters_all = s.info.iters
-    fast_iters = s.info.iters
-    rel_res = s.info.res / max(s.info.rhsnorm, s.info.res0)
-    err_vs_fp64 = None
-    if x64 is not None:
-        err_vs_fp64 = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
-    log(f"fast path: {fast_t:.3f}s per solve, {fast_iters} FGMRES steps, rel res {rel_res:.2e}")
+        times = []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            x, s = fast_solve(mode)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64)) if x64 is not None else None
+        return dict(seconds=float(np.mean(times)), fgmres_steps=s.info.iters, learned_steps=s.info.learned_steps,
+                    rel_res=s.info.res / max(s.info.rhsnorm, s.info.res0), rel_l2_err_vs_fp64=err)
+
+    fp32_only = timed("")
+    log(f"FP32 path: {fp32_only['seconds']:.3f}s per solve, {fp32_only['fgmres_steps']} FGMRES steps")
+    head = timed(learned) if learned else fp32_only
+    fast_t = head["seconds"]
+    log(f"learned + FP32 path: {fast_t:.3f}s per solve, {head['fgmres_steps']} FGMRES steps, "
+        f"{head['learned_steps']} learned steps taken, rel res {head['rel_res']:.2e}")
 
     traffic = None if args.no_traffic else measure_traffic(args)
     traffic_bytes = None
@@ -255,15 +271,18 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-in-fp64-IR",
+        "dtype": "fp32-in-fp64-IR" + ("+bf16-net" if learned else ""),
         "data": "synthetic",
         "config": {"workload": f"{p['name']} cubic C11=1.7 C12=1.2 C44=0.75, Haar rotations, uniaxial 1e-3 in z",
                    "n_cell": args.n_cell, "nodes": nodes, "tol_rel": args.tol,
-                   "path": "FP32 F-cycle preconditioned FGMRES(30), FP64 residual, Krylov basis and direct coarsest solve"},
-        "fast_path": {"seconds": fast_t, "fgmres_steps": fast_iters, "rel_res": rel_res,
-                      "rel_l2_err_vs_fp64": err_vs_fp64},
+                   "path": ("learned judged initial correction (U-Net, BF16 MFMA, on the 33^3 level) + " if learned
+                            else "") + "FP32 F-cycle preconditioned FGMRES(30), FP64 residual, Krylov basis "
+                                       "and direct coarsest solve"},
+        "learned_fp32": head if learned else None,
+        "fp32_only": fp32_only,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64_V": (plain["V"]["seconds"] / fast_t) if "V" in plain else None,
+        "network_gain_vs_fp32_only": fp32_only["seconds"] / fast_t if learned else None,
         "speedup_vs_plain_fp64_F": (plain["F"]["seconds"] / fast_t) if "F" in plain else None,
         "fapply": {"us_per_launch": fapply_ms * 1e3, "gbs": fapply_gbs, "bytes_per_node": FAPPLY_BYTES_PER_NODE[p["sym"]]},
         "roofline": {"bound": "hbm", "achieved": fapply_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -1,0 +1,142 @@
+// amr_kernels.h -- device code of the composite (AMR) solve: nested single-box
+// patches, refinement ratio 2, nodal.
+//
+// A patch of AMR level l covers the parent-level cells [clo, chi] (global
+// indices).  Its handle holds the fine nodes 2 clo .. 2 chi plus one ghost
+// node layer on each face that lies inside the domain; `origin` is the
+// global level-l node index of handle node 0.  Ghost nodes carry values
+// interpolated from the parent level (the "ghostnodes" coarse-fine
+// treatment), the nodes 2 clo .. 2 chi are the patch's unknowns.
+#pragma once
+#include "elastic_kernels.h"
+
+namespace ael {
+
+struct Patch {
+    int ox, oy, oz;        // origin: global fine node index of handle node 0
+    int clo[3], chi[3];    // covered parent cells [clo, chi] (global parent indices)
+    int gl[3], gh[3];      // 1 where the lo / hi face of the handle is a ghost layer
+};
+
+__device__ __forceinline__ bool ghost_node(const Lvl& L, const Patch& P, int i, int j, int k) {
+    return (P.gl[0] && i == 0) || (P.gh[0] && i == L.nx - 1) || (P.gl[1] && j == 0) || (P.gh[1] && j == L.ny - 1) ||
+           (P.gl[2] && k == 0) || (P.gh[2] && k == L.nz - 1);
+}
+
+// v = 0 on ghost nodes (mode 0) or v = src on ghost nodes (mode 1).
+__global__ void __launch_bounds__(kBlock) k_patch_ghost(double* __restrict__ v, const double* __restrict__ src, Lvl L,
+                                                        Patch P, int mode) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    if (!ghost_node(L, P, i, j, k)) return;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) v[p * L.N + id] = mode ? src[p * L.N + id] : 0.0;
+}
+
+// Trilinear interpolation of the parent field c (level Lc, whole parent
+// handle, parent origin po) onto the patch handle field f:
+// mode 0: f = P c everywhere, mode 1: f += P c everywhere, mode 2: f = P c on
+// ghost nodes only (src/Operator/Operator.cpp:365-434 weights).
+__global__ void __launch_bounds__(kBlock) k_patch_interp(const double* __restrict__ c, Lvl Lc, int pox, int poy,
+                                                         int poz, double* __restrict__ f, Lvl Lf, Patch P, int mode) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= Lf.N) return;
+    int i, j, k;
+    node_ijk(id, Lf, i, j, k);
+    if (mode == 2 && !ghost_node(Lf, P, i, j, k)) return;
+    const int gx = P.ox + i, gy = P.oy + j, gz = P.oz + k;  // global fine node
+    const int xs[2] = {(gx >> 1) - pox, ((gx + 1) >> 1) - pox};
+    const int ys[2] = {(gy >> 1) - poy, ((gy + 1) >> 1) - poy};
+    const int zs[2] = {(gz >> 1) - poz, ((gz + 1) >> 1) - poz};
+    const int nxs = (gx & 1) ? 2 : 1, nys = (gy & 1) ? 2 : 1, nzs = (gz & 1) ? 2 : 1;
+    const double w = 1.0 / (double)(nxs * nys * nzs);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        double acc = 0.0;
+        for (int a = 0; a < nzs; ++a)
+            for (int b = 0; b < nys; ++b)
+                for (int d = 0; d < nxs; ++d) acc += c[p * Lc.N + ((long)zs[a] * Lc.ny + ys[b]) * Lc.nx + xs[d]];
+        acc *= w;
+        double* q = f + p * Lf.N + id;
+        *q = mode == 1 ? *q + acc : acc;
+    }
+}
+
+// averageDownSolutionRHS (src/Operator/Operator.cpp:436-448, nodal
+// average_down = injection): parent nodes clo..chi take the patch values.
+__global__ void __launch_bounds__(kBlock) k_patch_inject(const double* __restrict__ f, Lvl Lf, Patch P,
+                                                         double* __restrict__ c, Lvl Lc, int pox, int poy, int poz) {
+    const int nx = P.chi[0] - P.clo[0] + 1, ny = P.chi[1] - P.clo[1] + 1, nz = P.chi[2] - P.clo[2] + 1;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)nx * ny * nz) return;
+    const int a = (int)(id % nx), b = (int)((id / nx) % ny), d = (int)(id / ((long)nx * ny));
+    const int I = P.clo[0] + a, J = P.clo[1] + b, K = P.clo[2] + d;  // global parent node
+    const long fi = ((long)(2 * K - P.oz) * Lf.ny + (2 * J - P.oy)) * Lf.nx + (2 * I - P.ox);
+    const long ci = ((long)(K - poz) * Lc.ny + (J - poy)) * Lc.nx + (I - pox);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) c[p * Lc.N + ci] = f[p * Lf.N + fi];
+}
+
+// Operator::reflux (src/Operator/Operator.cpp:526-633): the parent residual
+// at the nodes clo..chi (covered and coarse-fine nodes) is replaced by the
+// full-weighting restriction of the patch residual.  At a domain face the
+// stencil falls back to the face, edge and corner weights of the MG
+// restriction (Operator.cpp:280-353), since no fine node exists outside the
+// domain; inside the domain the fine ghost layer supplies the outer nodes.
+__global__ void __launch_bounds__(kBlock) k_patch_reflux(const double* __restrict__ rf, Lvl Lf, Patch P,
+                                                         double* __restrict__ rc, Lvl Lc, int pox, int poy, int poz,
+                                                         int dnx, int dny, int dnz) {
+    const int nx = P.chi[0] - P.clo[0] + 1, ny = P.chi[1] - P.clo[1] + 1, nz = P.chi[2] - P.clo[2] + 1;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)nx * ny * nz) return;
+    const int a = (int)(id % nx), b = (int)((id / nx) % ny), d = (int)(id / ((long)nx * ny));
+    const int I = P.clo[0] + a, J = P.clo[1] + b, K = P.clo[2] + d;  // global parent node
+    // domain faces of the parent level: dnx = parent-level domain cells
+    const bool bi = (I == 0 || I == dnx), bj = (J == 0 || J == dny), bk = (K == 0 || K == dnz);
+    const long sy = Lf.nx, sz = (long)Lf.nx * Lf.ny;
+    const long f0 = ((long)(2 * K - P.oz) * Lf.ny + (2 * J - P.oy)) * Lf.nx + (2 * I - P.ox);
+    const long ci = ((long)(K - poz) * Lc.ny + (J - poy)) * Lc.nx + (I - pox);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const double* F = rf + p * Lf.N + f0;
+        double v;
+        if (bi && bj && bk) v = F[0];
+        else if (bj && bk) v = 0.25 * F[-1] + 0.5 * F[0] + 0.25 * F[1];
+        else if (bk && bi) v = 0.25 * F[-sy] + 0.5 * F[0] + 0.25 * F[sy];
+        else if (bi && bj) v = 0.25 * F[-sz] + 0.5 * F[0] + 0.25 * F[sz];
+        else if (bi)
+            v = (F[-sy - sz] + 2.0 * F[-sz] + F[sy - sz] + 2.0 * F[-sy] + 4.0 * F[0] + 2.0 * F[sy] + F[-sy + sz] +
+                 2.0 * F[sz] + F[sy + sz]) / 16.0;
+        else if (bj)
+            v = (F[-1 - sz] + 2.0 * F[-1] + F[-1 + sz] + 2.0 * F[-sz] + 4.0 * F[0] + 2.0 * F[sz] + F[1 - sz] +
+                 2.0 * F[1] + F[1 + sz]) / 16.0;
+        else if (bk)
+            v = (F[-1 - sy] + 2.0 * F[-sy] + F[1 - sy] + 2.0 * F[-1] + 4.0 * F[0] + 2.0 * F[1] + F[-1 + sy] +
+                 2.0 * F[sy] + F[1 + sy]) / 16.0;
+        else
+            v = (F[-1 - sy - sz] + F[-1 - sy + sz] + F[-1 + sy - sz] + F[-1 + sy + sz] + F[1 - sy - sz] +
+                 F[1 - sy + sz] + F[1 + sy - sz] + F[1 + sy + sz]) / 64.0 +
+                (F[-sy - sz] + F[-sy + sz] + F[sy - sz] + F[sy + sz] + F[-1 - sz] + F[1 - sz] + F[-1 + sz] +
+                 F[1 + sz] + F[-1 - sy] + F[-1 + sy] + F[1 - sy] + F[1 + sy]) / 32.0 +
+                (F[-1] + F[-sy] + F[-sz] + F[1] + F[sy] + F[sz]) / 16.0 + F[0] / 8.0;
+        rc[p * Lc.N + ci] = v;
+    }
+}
+
+// out = v with the parent nodes strictly inside the patch (covered by the
+// finer level, excluded from the composite norm) set to 0.
+__global__ void __launch_bounds__(kBlock) k_mask_covered(const double* __restrict__ v, Lvl L, int pox, int poy,
+                                                         int poz, Patch P, double* __restrict__ out) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    const int I = pox + i, J = poy + j, K = poz + k;
+    const bool covered = I > P.clo[0] && I < P.chi[0] && J > P.clo[1] && J < P.chi[1] && K > P.clo[2] && K < P.chi[2];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) out[p * L.N + id] = covered ? 0.0 : v[p * L.N + id];
+}
+
+}  // namespace ael

@@ -24,6 +24,7 @@
 #include "bottom_kernels.h"
 #include "common.h"
 #include "elastic_kernels.h"
+#include "amr_kernels.h"
 #include "learned_kernels.h"
 #include "stencil.h"
 
@@ -115,6 +116,10 @@ struct alamo_el_s {
     double* nn_t = nullptr;    // scaled right-hand side, capacity nn_cap nodes
     double* nn_fmean = nullptr;  // 18 face means of nn_t
     long nn_cap = 0;
+    // AMR placement (alamo_amr_patch_create); a base handle has no parent
+    alamo_el_t amr_parent = nullptr;
+    int amr_dom[3] = {0, 0, 0};  // domain cells at this handle's level
+    ael::Patch amr{};            // origin and covered parent cells (origin 0 for a base handle)
 };
 
 namespace {
@@ -691,6 +696,7 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
     if (hipHostMalloc((void**)&h->red_host, sizeof(double) * 2) != hipSuccess)
         return cleanup(fail(ALAMO_EL_EHIP, "hipHostMalloc failed"));
     for (int q = 0; q < 78; ++q) { h->bct_host[q] = ALAMO_EL_BC_DISPLACEMENT; h->bcv_host[q] = 0.0; }
+    for (int a = 0; a < 3; ++a) h->amr_dom[a] = n_cell[a];
     if (hipMemcpy(h->bct_dev, h->bct_host, sizeof(h->bct_host), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->bcv_dev, h->bcv_host, sizeof(h->bcv_host), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(ALAMO_EL_EHIP, "hipMemcpy of the boundary table failed"));
@@ -1151,3 +1157,260 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Composite AMR solve (src/Operator/Operator.cpp:230-251, 436-448, 526-633)
+// ---------------------------------------------------------------------------
+namespace {
+
+// Faces (xlo, ylo, zlo, xhi, yhi, zhi) on which boundary region r lies, in the
+// region numbering of region_of (BC::Operator::Elastic::Elastic::Face).
+void region_faces(int r, bool f[6]) {
+    for (int q = 0; q < 6; ++q) f[q] = false;
+    if (r < 6) { f[r] = true; return; }
+    if (r < 10) { const int c = r - 6; f[(c & 2) ? 4 : 1] = true; f[(c & 1) ? 5 : 2] = true; return; }
+    if (r < 14) { const int c = r - 10; f[(c & 2) ? 5 : 2] = true; f[(c & 1) ? 3 : 0] = true; return; }
+    if (r < 18) { const int c = r - 14; f[(c & 2) ? 3 : 0] = true; f[(c & 1) ? 4 : 1] = true; return; }
+    const int c = r - 18;
+    f[(c & 4) ? 3 : 0] = true;
+    f[(c & 2) ? 4 : 1] = true;
+    f[(c & 1) ? 5 : 2] = true;
+}
+
+int amr_reduce_max(alamo_el_t h, const double* v, long n, double* out, hipStream_t s) {
+    return reduce_host<double>(h, v, v, n, 0, out, s);
+}
+
+struct AmrCtx {
+    int nlev;
+    const alamo_el_t* h;
+    double* const* x;
+    const double* const* b;
+    hipStream_t s;
+    Level& L(int l) { return h[l]->lv[0]; }
+    const ael::Patch& P(int l) { return h[l]->amr; }
+    int pox(int l) { return h[l - 1]->amr.ox; }
+    int poy(int l) { return h[l - 1]->amr.oy; }
+    int poz(int l) { return h[l - 1]->amr.oz; }
+
+    void reflux(int l, const double* rf, double* rc) {  // fine level l into parent l-1
+        const ael::Patch& Pl = P(l);
+        const long n = (long)(Pl.chi[0] - Pl.clo[0] + 1) * (Pl.chi[1] - Pl.clo[1] + 1) * (Pl.chi[2] - Pl.clo[2] + 1);
+        const int* dom = h[l - 1]->amr_dom;
+        ael::k_patch_reflux<<<nblk(n), kBlock, 0, s>>>(rf, L(l).L, Pl, rc, L(l - 1).L, pox(l), poy(l), poz(l), dom[0],
+                                                        dom[1], dom[2]);
+    }
+    void interp(int l, const double* c, double* f, int mode) {
+        ael::k_patch_interp<<<nblk(L(l).L.N), kBlock, 0, s>>>(c, L(l - 1).L, pox(l), poy(l), poz(l), f, L(l).L, P(l),
+                                                               mode);
+    }
+    void ghost(int l, double* v, const double* src, int mode) {
+        ael::k_patch_ghost<<<nblk(L(l).L.N), kBlock, 0, s>>>(v, src, L(l).L, P(l), mode);
+    }
+    void sync_solution() {  // average down, then refresh the ghost layers
+        for (int l = nlev - 1; l >= 1; --l) {
+            const ael::Patch& Pl = P(l);
+            const long n = (long)(Pl.chi[0] - Pl.clo[0] + 1) * (Pl.chi[1] - Pl.clo[1] + 1) * (Pl.chi[2] - Pl.clo[2] + 1);
+            ael::k_patch_inject<<<nblk(n), kBlock, 0, s>>>(x[l], L(l).L, Pl, x[l - 1], L(l - 1).L, pox(l), poy(l), poz(l));
+        }
+        for (int l = 1; l < nlev; ++l) interp(l, x[l - 1], x[l], 2);
+    }
+    // Composite residual on every level (finest first, reflux into each
+    // parent) and its masked inf-norm.
+    int residuals(double* norm) {
+        for (int l = nlev - 1; l >= 0; --l) {
+            Level& lv = L(l);
+            launch_residual<double>(h[l], lv, x[l], b[l], lv.d.C, lv.d.res, s);
+            if (l > 0) ghost(l, lv.d.res, nullptr, 0);
+            if (l + 1 < nlev) reflux(l + 1, L(l + 1).d.res, lv.d.res);
+        }
+        return masked_norm(norm, true);
+    }
+    // max |v| over the composite grid: each level's res (use_res) or b, ghost
+    // nodes excluded, parent nodes strictly covered by the child excluded.
+    int masked_norm(double* norm, bool use_res) {
+        *norm = 0.0;
+        int rc;
+        for (int l = 0; l < nlev; ++l) {
+            Level& lv = L(l);
+            const long M = 3 * lv.L.N;
+            const double* v = use_res ? lv.d.res : b[l];
+            double* t = lv.d.rescor;
+            if (l + 1 < nlev) {
+                ael::k_mask_covered<<<nblk(lv.L.N), kBlock, 0, s>>>(v, lv.L, h[l]->amr.ox, h[l]->amr.oy, h[l]->amr.oz,
+                                                                     P(l + 1), t);
+            } else {
+                launch_copy<double>(t, v, M, s);
+            }
+            if (l > 0) ghost(l, t, nullptr, 0);
+            double m;
+            if ((rc = amr_reduce_max(h[l], t, M, &m, s))) return rc;
+            if (std::isnan(m)) return fail(ALAMO_EL_ENAN, "NaN in the composite residual");
+            *norm = std::max(*norm, m);
+        }
+        return 0;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int alamo_amr_patch_create(alamo_el_t parent, const int clo[3], const int chi[3], alamo_el_t* out) {
+    if (!parent || !clo || !chi || !out) return fail(ALAMO_EL_EINVAL, "null argument");
+    *out = nullptr;
+    ael::Patch P{};
+    const Lvl& PL = parent->lv[0].L;
+    const int pn[3] = {PL.nx, PL.ny, PL.nz};
+    const int po[3] = {parent->amr.ox, parent->amr.oy, parent->amr.oz};
+    int ncell[3];
+    double dx[3] = {PL.dx / 2, PL.dy / 2, PL.dz / 2};
+    for (int a = 0; a < 3; ++a) {
+        if (!(0 <= clo[a] && clo[a] < chi[a] && chi[a] <= parent->amr_dom[a]))
+            return fail(ALAMO_EL_EINVAL, "patch cells must satisfy 0 <= clo < chi <= domain cells");
+        P.clo[a] = clo[a];
+        P.chi[a] = chi[a];
+        P.gl[a] = clo[a] > 0;
+        P.gh[a] = chi[a] < parent->amr_dom[a];
+        // proper nesting: the patch and its ghost layer lie on the parent's unknowns
+        const int ulo = po[a] + parent->amr.gl[a], uhi = po[a] + pn[a] - 1 - parent->amr.gh[a];
+        if (clo[a] - P.gl[a] < ulo || chi[a] + P.gh[a] > uhi)
+            return fail(ALAMO_EL_EINVAL, "patch not properly nested in its parent (needs one parent node of margin)");
+        ncell[a] = 2 * (chi[a] - clo[a]) + P.gl[a] + P.gh[a];
+    }
+    P.ox = 2 * clo[0] - P.gl[0];
+    P.oy = 2 * clo[1] - P.gl[1];
+    P.oz = 2 * clo[2] - P.gl[2];
+    alamo_el_t h;
+    int rc;
+    if ((rc = alamo_el_create(ncell, dx, parent->sym, 0, &h))) return rc;
+    h->amr_parent = parent;
+    h->amr = P;
+    for (int a = 0; a < 3; ++a) h->amr_dom[a] = 2 * parent->amr_dom[a];
+    h->uniform = parent->uniform;
+    h->omega = parent->omega;
+    // boundary rows: the parent's table on domain faces, ghost (displacement,
+    // value from the parent level) on every region touching an inner face
+    int t[78];
+    double v[78];
+    const bool inner[6] = {P.gl[0] != 0, P.gl[1] != 0, P.gl[2] != 0, P.gh[0] != 0, P.gh[1] != 0, P.gh[2] != 0};
+    for (int r = 0; r < 26; ++r) {
+        bool f[6];
+        region_faces(r, f);
+        bool g = false;
+        for (int q = 0; q < 6; ++q) g = g || (f[q] && inner[q]);
+        for (int c = 0; c < 3; ++c) {
+            t[3 * r + c] = g ? ALAMO_EL_BC_DISPLACEMENT : parent->bct_host[3 * r + c];
+            v[3 * r + c] = g ? 0.0 : parent->bcv_host[3 * r + c];
+        }
+    }
+    if ((rc = alamo_el_set_bc(h, t, v))) {
+        alamo_el_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost_hi[3]) {
+    if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
+    const int o[3] = {h->amr.ox, h->amr.oy, h->amr.oz};
+    for (int a = 0; a < 3; ++a) {
+        if (origin) origin[a] = o[a];
+        if (ghost_lo) ghost_lo[a] = h->amr.gl[a];
+        if (ghost_hi) ghost_hi[a] = h->amr.gh[a];
+    }
+    return 0;
+}
+
+int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const double* const* b, double tol_rel,
+                    double tol_abs, const alamo_el_opts* opts_in, alamo_el_info* info, hipStream_t s) {
+    if (nlev < 1 || !hs || !x || !b) return fail(ALAMO_EL_EINVAL, "bad argument");
+    for (int l = 0; l < nlev; ++l) {
+        if (!hs[l] || !x[l] || !b[l]) return fail(ALAMO_EL_EINVAL, "null level");
+        if ((l == 0) != (hs[l]->amr_parent == nullptr) || (l > 0 && hs[l]->amr_parent != hs[l - 1]))
+            return fail(ALAMO_EL_EINVAL, "levels must be a base handle followed by its nested patches");
+        if (!hs[l]->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    }
+    if (nlev == 1) return alamo_el_solve(hs[0], x[0], b[0], tol_rel, tol_abs, opts_in, info, s);
+    alamo_el_opts o;
+    alamo_el_default_opts(&o);
+    if (opts_in) o = *opts_in;
+    if (o.max_iter < 0 || o.nu1 < 0 || o.nu2 < 0 || !(o.omega > 0)) return fail(ALAMO_EL_EINVAL, "bad solver options");
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    for (int l = 0; l < nlev; ++l) {
+        hs[l]->omega = o.omega;
+        if (!hs[l]->prepared && (rc = prepare(hs[l], s))) return rc;
+    }
+    const bool fp32 = o.precision == ALAMO_EL_PREC_FP32_IR;
+    if (fp32 && (rc = prepare32(hs[0], s))) return rc;
+    AmrCtx A{nlev, hs, x, b, s};
+    A.sync_solution();
+    double r0, bn;
+    if ((rc = A.residuals(&r0))) return rc;
+    if ((rc = A.masked_norm(&bn, false))) return rc;
+    const double target = std::max(tol_abs, std::max(tol_rel, 1.e-16) * std::max(r0, bn));
+    Cycle<double> c64{hs[0], s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+    Cycle<float> c32{hs[0], s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+    double nrm = r0;
+    int iters = 0, status = nrm <= target ? 0 : ALAMO_EL_NOT_CONVERGED;
+    const long M0 = 3 * A.L(0).L.N;
+    for (int it = 0; it < o.max_iter && status != 0; ++it) {
+        // down: pre-smooth each patch on its composite residual, reflux the
+        // remaining residual into the parent
+        for (int l = nlev - 1; l >= 1; --l) {
+            Level& lv = A.L(l);
+            const long M = 3 * lv.L.N;
+            launch_fill<double>(lv.d.cor, 0.0, M, s);
+            A.ghost(l, lv.d.res, lv.d.cor, 1);
+            for (int q = 0; q < 2 * o.nu1; ++q) {
+                double* src = (q % 2 == 0) ? lv.d.cor : lv.d.tmp;
+                double* dst = (q % 2 == 0) ? lv.d.tmp : lv.d.cor;
+                launch_jacobi<double>(hs[l], lv, src, lv.d.res, lv.d.diag, lv.d.C, dst, s);
+            }
+            launch_residual<double>(hs[l], lv, lv.d.cor, lv.d.res, lv.d.C, lv.d.rescor, s);
+            A.ghost(l, lv.d.rescor, nullptr, 0);
+            A.reflux(l, lv.d.rescor, A.L(l - 1).d.res);
+        }
+        // base level: one multigrid cycle on its composite residual
+        if (fp32) {
+            ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(A.L(0).d.res, A.L(0).f.res, M0);
+            c32.run(o.cycle);
+            ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(A.L(0).f.cor, A.L(0).d.cor, M0);
+        } else {
+            c64.run(o.cycle);
+        }
+        // up: add the interpolated parent correction, post-smooth
+        for (int l = 1; l < nlev; ++l) {
+            Level& lv = A.L(l);
+            A.interp(l, A.L(l - 1).d.cor, lv.d.cor, 1);
+            A.ghost(l, lv.d.res, lv.d.cor, 1);
+            for (int q = 0; q < 2 * o.nu2; ++q) {
+                double* src = (q % 2 == 0) ? lv.d.cor : lv.d.tmp;
+                double* dst = (q % 2 == 0) ? lv.d.tmp : lv.d.cor;
+                launch_jacobi<double>(hs[l], lv, src, lv.d.res, lv.d.diag, lv.d.C, dst, s);
+            }
+        }
+        for (int l = 0; l < nlev; ++l) launch_axpy<double>(x[l], 1.0, A.L(l).d.cor, 3 * A.L(l).L.N, s);
+        A.sync_solution();
+        if ((rc = A.residuals(&nrm))) return rc;
+        iters = it + 1;
+        if (o.verbose) std::fprintf(stderr, "alamo_amr: iter %d |r|_inf = %.6e\n", iters, nrm);
+        if (nrm <= target) status = 0;
+    }
+    HIPCHK(hipGetLastError());
+    if (info) {
+        info->status = status;
+        info->iters = iters;
+        info->res0 = r0;
+        info->rhsnorm = bn;
+        info->res = nrm;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        info->learned_steps = 0;
+    }
+    if (status == ALAMO_EL_NOT_CONVERGED) g_err = "composite MLMG did not converge within max_iter";
+    return status;
+}
+
+}  // extern "C"

@@ -610,15 +610,19 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partial(const T* __restrict__
     __shared__ double sh[kBlock];
     double acc = 0.0;
     for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
-        if (op == 0) acc = fmax(acc, fabs((double)a[id]));
+        // max |v| propagates NaN (fmax alone would drop it and hide a blow-up)
+        if (op == 0) acc = (isnan((double)a[id]) || isnan(acc)) ? NAN : fmax(acc, fabs((double)a[id]));
         else if (op == 1) acc += (double)a[id] * (double)b[id];
         else acc += (double)a[id];
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = op == 0 ? fmax(sh[threadIdx.x], sh[threadIdx.x + s])
-                                                             : sh[threadIdx.x] + sh[threadIdx.x + s];
+        if ((int)threadIdx.x < s)
+            sh[threadIdx.x] = op == 0 ? ((isnan(sh[threadIdx.x]) || isnan(sh[threadIdx.x + s]))
+                                             ? NAN
+                                             : fmax(sh[threadIdx.x], sh[threadIdx.x + s]))
+                                      : sh[threadIdx.x] + sh[threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
@@ -628,12 +632,16 @@ __global__ void __launch_bounds__(kBlock) k_reduce_final(const double* __restric
                                                          double* __restrict__ out) {
     __shared__ double sh[kBlock];
     double acc = 0.0;
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) acc = op == 0 ? fmax(acc, partial[q]) : acc + partial[q];
+    for (int q = threadIdx.x; q < nb; q += blockDim.x)
+        acc = op == 0 ? ((isnan(partial[q]) || isnan(acc)) ? NAN : fmax(acc, partial[q])) : acc + partial[q];
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = op == 0 ? fmax(sh[threadIdx.x], sh[threadIdx.x + s])
-                                                             : sh[threadIdx.x] + sh[threadIdx.x + s];
+        if ((int)threadIdx.x < s)
+            sh[threadIdx.x] = op == 0 ? ((isnan(sh[threadIdx.x]) || isnan(sh[threadIdx.x + s]))
+                                             ? NAN
+                                             : fmax(sh[threadIdx.x], sh[threadIdx.x + s]))
+                                      : sh[threadIdx.x] + sh[threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = sh[0];

@@ -59,7 +59,8 @@ EXPORTS = [
     "alamo_el_set_omega", "alamo_el_set_model", "alamo_el_prepare", "alamo_el_fapply", "alamo_el_diagonal",
     "alamo_el_fsmooth", "alamo_el_residual", "alamo_el_restrict", "alamo_el_interp_add", "alamo_el_bc_init",
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
-    "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess",
+    "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
+    "alamo_amr_patch_info", "alamo_amr_solve",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -105,6 +106,11 @@ def lib():
         L.alamo_el_learned_input.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(ctypes.c_double), vp]
         L.alamo_el_set_learned.argtypes = [vp, vp, ctypes.c_int]
         L.alamo_el_learned_guess.argtypes = [vp, vp, vp, vp]
+        L.alamo_amr_patch_create.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(vp)]
+        L.alamo_amr_patch_info.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
+        L.alamo_amr_solve.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                      ctypes.c_double, ctypes.c_double, ctypes.POINTER(Opts), ctypes.POINTER(Info), vp]
         _lib = L
     return _lib
 
@@ -148,6 +154,9 @@ class ElasticOperator:
         _check(lib().alamo_el_create(n_cell.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                      dxa.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(sym),
                                      int(max_coarsening_level), ctypes.byref(h)), "alamo_el_create")
+        self._init_from(h, sym, n_cell, dxa)
+
+    def _init_from(self, h, sym, n_cell, dxa):
         self._h = h
         self.sym = int(sym)
         self.ncomp = _model.NCOMP[self.sym]
@@ -157,6 +166,32 @@ class ElasticOperator:
         _check(lib().alamo_el_num_levels(h, ctypes.byref(nl)), "alamo_el_num_levels")
         self.num_levels = nl.value
         self._bc = _bc.Constant()
+        self.parent = None
+        self.origin = (0, 0, 0)       # global node index (x, y, z) of node 0
+        self.ghost_lo = (0, 0, 0)
+        self.ghost_hi = (0, 0, 0)
+
+    def refine(self, clo, chi):
+        """Patch of the next AMR level covering this level's cells [clo, chi]
+        (x, y, z global indices): alamo_amr_patch_create.  The patch inherits
+        the boundary table on domain faces; set its model and right-hand side
+        over its own nodes (self.origin locates them)."""
+        lo = np.ascontiguousarray(clo, dtype=np.int32)
+        hi = np.ascontiguousarray(chi, dtype=np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().alamo_amr_patch_create(self._h, lo.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            hi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(h)),
+               "alamo_amr_patch_create")
+        p = ElasticOperator.__new__(ElasticOperator)
+        o, gl, gh = (ctypes.c_int * 3)(), (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        _check(lib().alamo_amr_patch_info(h, o, gl, gh), "alamo_amr_patch_info")
+        n_cell = [2 * (int(hi[a]) - int(lo[a])) + gl[a] + gh[a] for a in range(3)]
+        p._init_from(h, self.sym, n_cell, np.asarray(self.dx) / 2.0)
+        p.parent = self
+        p.origin, p.ghost_lo, p.ghost_hi = tuple(o), tuple(gl), tuple(gh)
+        p.clo, p.chi = tuple(int(v) for v in lo), tuple(int(v) for v in hi)
+        p._bc = self._bc
+        return p
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -348,6 +383,25 @@ class Linear:
         if rc == NOT_CONVERGED and self.abort_on_fail:
             raise SolverError(f"MLMG failed to converge: |r| = {self.info.res:.3e} after {self.info.iters} iterations")
         return self.info.res
+
+
+def amr_solve(levels, xs, bs, tol_rel: float, tol_abs: float, **kw):
+    """Composite solve over a base operator and its nested patches
+    (alamo_amr_solve); xs, bs are per-level fields, solved in place.  Keyword
+    options as for Linear.  Returns the Info structure."""
+    if not (len(levels) == len(xs) == len(bs)):
+        raise AlamoError("levels, xs and bs must have the same length")
+    solver = Linear(levels[0], **kw)
+    n = len(levels)
+    H = (ctypes.c_void_p * n)(*[lv._h.value for lv in levels])
+    X = (ctypes.c_void_p * n)(*[lv._v(x, 0).value for lv, x in zip(levels, xs)])
+    B = (ctypes.c_void_p * n)(*[lv._v(b, 0).value for lv, b in zip(levels, bs)])
+    rc = lib().alamo_amr_solve(n, H, X, B, float(tol_rel), float(tol_abs), ctypes.byref(solver.opts),
+                               ctypes.byref(solver.info), _stream())
+    _check(rc, "alamo_amr_solve")
+    if rc == NOT_CONVERGED and solver.abort_on_fail:
+        raise SolverError(f"composite MLMG failed to converge: |r| = {solver.info.res:.3e}")
+    return solver.info
 
 
 class Newton(Linear):

@@ -50,3 +50,109 @@ def polycrystal(n_cell: int = 128, ngrains: int = 64, seed: int = 0, C11=1.7, C1
     if return_grains:
         out.update(grain_ids=gid, table=table, seeds=seeds, rotations=R)
     return out
+
+
+def sphere_eta(x, y, z, center, radius, width=0.0):
+    """Indicator of a sphere at the points (x, y, z) (broadcastable): sharp
+    (r < R) for width 0, else the diffuse 0.5 (1 - tanh((r - R) / width))."""
+    r2 = (x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2
+    if width <= 0:
+        return (r2 < radius * radius).astype(np.float64)
+    return 0.5 * (1.0 - np.tanh((np.sqrt(r2) - radius) / width))
+
+
+def inclusions_on_box(origin, shape, dx, spheres, E_matrix=1.0, nu=0.3, eigenstrain=1e-3, width=0.0):
+    """Isotropic C (2, nz, ny, nx) and eigenstrain F0 (9, nz, ny, nx) at the
+    nodes (origin + index) * dx of a box, for spheres [(center, radius, E)]:
+    matrix E_matrix, each sphere its own E, all with Poisson ratio nu, and
+    dilatational eigenstrain in the spheres; models are mixed by the sphere
+    indicators (src/Integrator/Mechanics.H:249-253), sharp for width 0 as in
+    inclusion(), diffuse over `width` otherwise.  The field is a function of
+    position, so every AMR level samples the same material."""
+    nz, ny, nx = shape
+    x = ((origin[0] + np.arange(nx)) * dx)[None, None, :]
+    y = ((origin[1] + np.arange(ny)) * dx)[None, :, None]
+    z = ((origin[2] + np.arange(nz)) * dx)[:, None, None]
+    lam, mu = _model.lame_from_E_nu(E_matrix, nu)
+    C = np.empty((2, nz, ny, nx))
+    C[0], C[1] = lam, mu
+    eig = np.zeros(shape)
+    for center, radius, E in spheres:
+        eta = np.broadcast_to(sphere_eta(x, y, z, center, radius, width), shape)
+        li, mi = _model.lame_from_E_nu(E, nu)
+        C[0] = (1.0 - eta) * C[0] + eta * li
+        C[1] = (1.0 - eta) * C[1] + eta * mi
+        eig = np.maximum(eig, eta)
+    F0 = np.zeros((9, nz, ny, nx))
+    for a in range(3):
+        F0[4 * a] = eigenstrain * eig
+    return np.ascontiguousarray(C), np.ascontiguousarray(F0)
+
+
+def random_spheres(n=3, seed=0, r_range=(0.05, 0.15), E_range=(2.0, 20.0)):
+    """n spheres with seeded centres inside the cube (kept a radius from the
+    faces), radii ~ U(r_range) and moduli ~ U(E_range)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        r = rng.uniform(*r_range)
+        c = rng.uniform(r, 1.0 - r, size=3)
+        out.append((tuple(float(v) for v in c), float(r), float(rng.uniform(*E_range))))
+    return out
+
+
+def tag_box(n_cell, level_dx, origin_cells, box_cells, spheres, threshold=0.1, buffer=2, width=0.0):
+    """Cells of one level tagged by |grad eta| * |dx| > threshold
+    (src/Integrator/Mechanics.H:298-313) for the sharp sphere indicator,
+    searched inside box_cells = (lo, hi) (global cells), returned as the
+    bounding box (clo, chi) grown by `buffer` cells and clipped to the box,
+    or None when nothing is tagged.  One box per level: the clustering of
+    tags into several boxes (AmrCore) is not restated."""
+    lo, hi = box_cells
+    shape = tuple(hi[a] - lo[a] + 1 for a in (2, 1, 0))  # nodes of the searched cells
+    eta = np.zeros(shape)
+    x = (lo[0] + np.arange(shape[2])) * level_dx
+    y = (lo[1] + np.arange(shape[1])) * level_dx
+    z = (lo[2] + np.arange(shape[0])) * level_dx
+    for center, radius, _ in spheres:
+        eta = np.maximum(eta, sphere_eta(x[None, None, :], y[None, :, None], z[:, None, None], center, radius,
+                                         width))
+    # cell gradient from the 8 corner nodes, times |dx| = sqrt(3) dx
+    gx = (eta[:, :, 1:] - eta[:, :, :-1])[:-1, :-1, :] + (eta[:, :, 1:] - eta[:, :, :-1])[1:, 1:, :]
+    gy = (eta[:, 1:, :] - eta[:, :-1, :])[:-1, :, :-1] + (eta[:, 1:, :] - eta[:, :-1, :])[1:, :, 1:]
+    gz = (eta[1:, :, :] - eta[:-1, :, :])[:, :-1, :-1] + (eta[1:, :, :] - eta[:-1, :, :])[:, 1:, 1:]
+    grad = np.sqrt((gx / 2) ** 2 + (gy / 2) ** 2 + (gz / 2) ** 2) / level_dx
+    tags = grad * np.sqrt(3.0) * level_dx > threshold
+    if not tags.any():
+        return None
+    k, j, i = np.nonzero(tags)
+    clo = [int(i.min()) + lo[0] - buffer, int(j.min()) + lo[1] - buffer, int(k.min()) + lo[2] - buffer]
+    chi = [int(i.max()) + 1 + lo[0] + buffer, int(j.max()) + 1 + lo[1] + buffer, int(k.max()) + 1 + lo[2] + buffer]
+    clo = [max(clo[a], lo[a]) for a in range(3)]
+    chi = [min(chi[a], hi[a]) for a in range(3)]
+    return clo, chi
+
+
+def amr_hierarchy(n0: int, spheres, levels: int = 3, buffer: int = 2, width: float = 0.0):
+    """Patch boxes [(clo, chi), ...] of AMR levels 1..levels-1 over a base grid
+    of n0 cells: each level tags its cells by the sphere-indicator gradient
+    (tag_box) inside the region where a child patch and its ghost layer stay
+    on the parent's unknowns, and takes the bounding box."""
+    boxes = []
+    lo, hi = [0, 0, 0], [n0, n0, n0]      # searchable cells of the current level
+    dom = [n0, n0, n0]
+    for l in range(levels - 1):
+        dx = 1.0 / dom[0]
+        tb = tag_box(dom[0], dx, lo, (lo, hi), spheres, buffer=buffer, width=width)
+        if tb is None:
+            break
+        clo, chi = tb
+        # ghost layer (inner faces) must sit on the parent's unknowns
+        clo = [max(clo[a], lo[a] + (1 if lo[a] > 0 else 0)) for a in range(3)]
+        chi = [min(chi[a], hi[a] - (1 if hi[a] < dom[a] else 0)) for a in range(3)]
+        if any(chi[a] - clo[a] < 2 for a in range(3)):
+            break
+        boxes.append((tuple(clo), tuple(chi)))
+        dom = [2 * d for d in dom]
+        lo, hi = [2 * c for c in clo], [2 * c for c in chi]
+    return boxes

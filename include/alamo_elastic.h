@@ -204,6 +204,31 @@ int alamo_el_set_learned(alamo_el_t h, alamo_nn_t nn, int mglev);
 /* x = P^L (s * net(input(coarse_rhs(b)))) on the finest level. */
 int alamo_el_learned_guess(alamo_el_t h, const double* b, double* x, hipStream_t stream);
 
+/* --- Composite AMR solve (refinement ratio 2, one box per AMR level):
+ * Operator::reflux, fixUpResidualMask and averageDownSolutionRHS
+ * (src/Operator/Operator.cpp:230-251, 436-448, 526-633) with MLMG's sweep over
+ * AMR levels. --- */
+/* Patch of the next AMR level covering the parent's cells [clo, chi] (global
+ * parent-level cell indices, clo < chi <= domain cells).  The new handle holds
+ * the fine nodes 2 clo .. 2 chi plus one ghost node layer on each face inside
+ * the domain (values interpolated from the parent, the "ghostnodes" treatment);
+ * its boundary table is the parent's on domain faces and displacement on ghost
+ * faces.  The patch and its ghost layer must lie on the parent's unknowns.  Set
+ * its model with alamo_el_set_model over the handle's nodes
+ * (alamo_amr_patch_info gives the global node index of node 0). */
+int alamo_amr_patch_create(alamo_el_t parent, const int clo[3], const int chi[3], alamo_el_t* out);
+int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost_hi[3]);
+/* Solve the composite system over levels[0] (the base handle) and its nested
+ * patches levels[1..nlev-1]: x[l], b[l] are fields of each handle.  One
+ * iteration pre-smooths every patch (nu1 Fsmooth calls), refluxes the residual
+ * into its parent, runs one cycle (opts->cycle, opts->precision) on the base
+ * level, interpolates the corrections up and post-smooths (nu2); then the
+ * solution is averaged down and the ghost layers refreshed.  Stops on the
+ * composite inf-norm (parent nodes strictly covered by a patch excluded,
+ * coarse-fine nodes included). */
+int alamo_amr_solve(int nlev, const alamo_el_t* levels, double* const* x, const double* const* b, double tol_rel,
+                    double tol_abs, const alamo_el_opts* opts, alamo_el_info* info, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,143 @@
+"""Measurements of the secondary configurations (the headline one is bench.py).
+
+  python -m alamo_amd.measure --config 5 [--fields 4]   257^3 polycrystals, 256 grains
+  python -m alamo_amd.measure --config 3                 3-level AMR, 65^3 base
+
+Config 5: unseen seeded Voronoi fields (256 grains, cubic C11=1.7, C12=1.2,
+C44=0.75, Haar orientations, uniaxial 1e-3 along z) at 257^3 nodes solved to
+1e-8 with the plain FP64 MLMG (V and F cycles), the FP32 path (FP32 F-cycle
+inside FP64 FGMRES) and the learned + FP32 path (the trained network on the
+33^3 level, judged initial correction); times, iterations and the relative
+L2 error against an FP64 solve to 1e-10.
+
+Config 3: three seeded spherical inclusions (r ~ U[0.05, 0.15], E ~
+U[2, 20], dilatational eigenstrain 1e-3, interfaces two base cells wide),
+zero-displacement faces, base 65^3 nodes, two patch levels from the
+indicator-gradient tags; composite V-cycle solve to 1e-8 in FP64 and with
+FP32 base cycles.
+
+Results go to stdout as one JSON object per configuration.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def _log(msg):
+    print(f"[measure] {msg}", file=sys.stderr, flush=True)
+
+
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t
+
+
+def config5(fields=4, n_cell=256, weights=None):
+    import torch
+
+    from . import bc as B
+    from . import elastic as el
+    from . import model as M
+    from . import train
+    net = None
+    if weights and os.path.exists(weights):
+        net, meta = train.load(weights)
+    rows = []
+    for f in range(fields):
+        seed = 900000 + f
+        C = train.polycrystal_field(n_cell, 256, seed, 1.7, 1.2, 0.75)
+        op = el.ElasticOperator(n_cell, 1.0 / n_cell, M.MAJORMINOR)
+        op.SetBC(B.uniaxial(2, 1e-3))
+        op.SetModel(C)
+        op.prepare()
+        b = op.bc_init(torch.zeros((3,) + op.level_shape(0), dtype=torch.float64, device="cuda"))
+        level = op.num_levels - 1
+        while level > 0 and op.level_shape(level)[0] < 33:
+            level -= 1
+        if net is not None and op.level_shape(level) == (33, 33, 33):
+            op.set_learned(net, level)
+        ref = torch.zeros_like(b)
+        el.Linear(op, cycle="F", outer="fgmres", abort_on_fail=False).solve(ref, b, 1e-10, 0.0)
+        row = dict(field=seed, grains=256, nodes=int(np.prod(op.level_shape(0))), network_level=level)
+        runs = [("plain_fp64_V", dict(cycle="V")), ("plain_fp64_F", dict(cycle="F")),
+                ("fp32", dict(cycle="F", precision="fp32", outer="fgmres", bottom_solver="direct"))]
+        if net is not None:
+            runs.append(("learned_fp32", dict(cycle="F", precision="fp32", outer="fgmres", bottom_solver="direct",
+                                              learned="guess")))
+        for name, kw in runs:
+            x = torch.zeros_like(b)
+            s = el.Linear(op, abort_on_fail=False, max_iter=400, **kw)
+            _, dt = _timed(lambda: s.solve(x, b, 1e-8, 0.0))
+            err = float(torch.linalg.norm(x - ref) / torch.linalg.norm(ref))
+            row[name] = dict(seconds=dt, iters=s.info.iters, converged=s.info.status == 0,
+                             learned_steps=s.info.learned_steps, rel_l2_err_vs_fp64=err)
+            _log(f"field {seed}: {name}: {dt:.3f}s, {s.info.iters} iterations, err vs FP64 {err:.2e}")
+        rows.append(row)
+        del op, C
+        torch.cuda.empty_cache()
+    return dict(config=5, workload=f"polycrystal-{n_cell + 1}^3-256g uniaxial 1e-3 z, to 1e-8", fields=rows)
+
+
+def config3(n0=64, seed=0):
+    import torch
+
+    from . import bc as B
+    from . import elastic as el
+    from . import model as M
+    from . import problems as P
+    spheres = P.random_spheres(3, seed=seed)
+    width = 2.0 / n0
+    boxes = P.amr_hierarchy(n0, spheres, width=width)
+    ops = [el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC)]
+    ops[0].SetBC(B.Constant())
+    for clo, chi in boxes:
+        ops.append(ops[-1].refine(clo, chi))
+    bs = []
+    for op in ops:
+        shape = op.level_shape(0)
+        C, F0 = P.inclusions_on_box(op.origin, shape, op.dx[0], spheres, width=width)
+        Cd = torch.from_numpy(C).cuda()
+        op.SetModel(Cd)
+        b0 = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
+        bs.append(op.newton_rhs(Cd, torch.zeros_like(b0), b0, F0=torch.from_numpy(F0).cuda()))
+    out = dict(config=3, workload=f"3-level AMR, base {n0 + 1}^3, 3 inclusions (seed {seed})",
+               spheres=[dict(center=c, radius=r, E=E) for c, r, E in spheres],
+               patches=[dict(clo=list(lo), chi=list(hi)) for lo, hi in boxes],
+               level_nodes=[int(np.prod(op.level_shape(0))) for op in ops])
+    sols = {}
+    for prec in ("fp64", "fp32"):
+        xs = [torch.zeros_like(b) for b in bs]
+        info, dt = _timed(lambda: el.amr_solve(ops, xs, bs, 1e-8, 0.0, max_iter=400, cycle="V", precision=prec,
+                                               abort_on_fail=False))
+        sols[prec] = xs
+        out[prec] = dict(seconds=dt, iters=info.iters, converged=info.status == 0, res=info.res)
+        _log(f"config 3 {prec}: {dt:.3f}s, {info.iters} composite V-cycles, converged={info.status == 0}")
+    out["fp32_vs_fp64_rel_l2"] = [float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+                                  for a, b in zip(sols["fp32"], sols["fp64"])]
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--config", type=int, choices=(3, 5), required=True)
+    ap.add_argument("--fields", type=int, default=4)
+    ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                                                      "unet_polycrystal129_level2.npz"))
+    a = ap.parse_args(argv)
+    r = config5(a.fields, weights=a.weights) if a.config == 5 else config3()
+    print(json.dumps(r), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -164,12 +164,12 @@ void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const 
         }
         return;
     }
-    const unsigned g = ael::slab_blocks(L, kBlock);
+    const unsigned g = ael::slab_blocks(L, kBlock), nbb = ael::slab_boundary_blocks(L, kBlock);
     const unsigned char* gm = var == kVariantSlab ? l.gmask : nullptr;
     switch (h->sym) {
-        case 0: ael::k_stencil_slab<T, 0, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, out); break;
-        case 1: ael::k_stencil_slab<T, 1, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, out); break;
-        default: ael::k_stencil_slab<T, 2, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, out); break;
+        case 0: ael::k_stencil_slab<T, 0, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, out); break;
+        case 1: ael::k_stencil_slab<T, 1, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, out); break;
+        default: ael::k_stencil_slab<T, 2, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, out); break;
     }
 }
 

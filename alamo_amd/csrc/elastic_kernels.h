@@ -194,14 +194,17 @@ __device__ __forceinline__ void node_ijk(long id, const Lvl& L, int& i, int& j, 
 }
 
 // One-sided first derivative at the faces, central inside (src/Numeric/Stencil.H:70-191).
+// Branch-free: both loads are always issued (and always in the box), so a
+// wave with lanes on different faces waits on memory once, not once per case;
+// the one-sided factor 1 is exact, so each case rounds as (a - b) / h or
+// (a - b) * 0.5 / h.
 template <typename T>
 __device__ __forceinline__ T d1(const T* __restrict__ f, long id, long stride, int pos, int n, T h) {
-    if (pos == 0) return (f[id + stride] - f[id]) / h;
-    if (pos == n - 1) return (f[id] - f[id - stride]) / h;
-    return (f[id + stride] - f[id - stride]) * T(0.5) / h;
+    const bool lo = pos == 0, hi = pos == n - 1;
+    const T a = f[hi ? id : id + stride], b = f[lo ? id : id - stride];
+    return (a - b) * (lo || hi ? T(1) : T(0.5)) / h;
 }
 
-// Elastic<SYM>::Fapply at one node (src/Operator/Elastic.cpp:165-290),
 // Pair index P <-> (pair_a(P), pair_b(P)) of (00,01,02,11,12,22), and the
 // MajorMinor slot s <-> (slot_p(s), slot_q(s)), P <= Q, of the 6x6 pair matrix.
 __host__ __device__ constexpr int pair_a(int P) { return P < 3 ? 0 : (P < 5 ? 1 : 2); }
@@ -228,11 +231,6 @@ __host__ __device__ constexpr bool slot_in_col(int s, int a) {
     return pair_other(slot_p(s), a) >= 0 || pair_other(slot_q(s), a) >= 0;
 }
 
-// Interior row of Fapply for MajorMinor C, the same sums as contract3 and
-// contract2_col but streamed over the 21 slots: each slot is loaded once,
-// used for every term it enters, and dropped, so only the 18 second
-// derivative combinations e[j][Q], the 6 symmetric gradients e1[Q] and the
-// result stay live (the load of slot s+1 overlaps the math of slot s).
 // Load base[idx] with a 32-bit element index: with base uniform, the address
 // is a scalar base plus a 32-bit byte offset (no 64-bit vector address math).
 // Every array indexed this way holds fewer than 2^32 bytes (checked at create).
@@ -243,6 +241,30 @@ __device__ __forceinline__ T ld32(const T* __restrict__ base, unsigned idx) {
 template <typename T>
 __device__ __forceinline__ void st32(T* __restrict__ base, unsigned idx, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(idx * (unsigned)sizeof(T))) = v;
+}
+
+// + sum_a ((dC/dx_a) : grad u).col(a)  (src/Operator/Elastic.cpp:245-254) for
+// MajorMinor C at interior node o, for the axes whose bit is set in gmask;
+// e1 = symmetric gradient pairs of u at the node.  Accumulates into out.
+template <typename T>
+__device__ __forceinline__ void gradc_mm(const T* __restrict__ C, const Lvl& L, unsigned o, unsigned gmask,
+                                         const T e1[6], T out[3]) {
+    const unsigned N = (unsigned)L.N, sy = (unsigned)L.nx, sz = (unsigned)(L.nx * L.ny);
+    const unsigned st[3] = {1u, sy, sz};
+    const T h2[3] = {T(0.5) * (T(1) / T(L.dx)), T(0.5) * (T(1) / T(L.dy)), T(0.5) * (T(1) / T(L.dz))};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!((gmask >> a) & 1u)) continue;
+#pragma unroll
+        for (int s = 0; s < 21; ++s) {
+            if (!slot_in_col(s, a)) continue;
+            const int P = slot_p(s), Q = slot_q(s);
+            const T* Cs = C + (size_t)s * N;
+            const T d = (ld32(Cs, o + st[a]) - ld32(Cs, o - st[a])) * h2[a];
+            if (pair_other(P, a) >= 0) out[pair_other(P, a)] += d * e1[Q];
+            if (Q != P && pair_other(Q, a) >= 0) out[pair_other(Q, a)] += d * e1[P];
+        }
+    }
 }
 
 // Interior row of Fapply for MajorMinor C, the same sums as contract3 and
@@ -299,22 +321,7 @@ __device__ __forceinline__ void apply_interior_mm(const T* __restrict__ u, const
         }
     }
     if (uniform) return;
-    // + sum_a ((dC/dx_a) : grad u).col(a)  (src/Operator/Elastic.cpp:245-254)
-    const unsigned st[3] = {1u, sy, sz};
-    const T h2[3] = {hx2, hy2, hz2};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (!((gmask >> a) & 1u)) continue;
-#pragma unroll
-        for (int s = 0; s < 21; ++s) {
-            if (!slot_in_col(s, a)) continue;
-            const int P = slot_p(s), Q = slot_q(s);
-            const T* Cs = C + (size_t)s * N;
-            const T d = (ld32(Cs, o + st[a]) - ld32(Cs, o - st[a])) * h2[a];
-            if (pair_other(P, a) >= 0) out[pair_other(P, a)] += d * e1[Q];
-            if (Q != P && pair_other(Q, a) >= 0) out[pair_other(Q, a)] += d * e1[P];
-        }
-    }
+    gradc_mm<T>(C, L, o, gmask, e1, out);
 }
 
 // sigma = C : A for MajorMinor C, streamed over the slots (contract2).
@@ -339,92 +346,99 @@ __device__ __forceinline__ void contract2_mm_stream(const T* __restrict__ C, lon
     for (int P = 0; P < 6; ++P) sg[pair_a(P)][pair_b(P)] = sg[pair_b(P)][pair_a(P)] = sp[P];
 }
 
-// psi unset (psi_avg = 1).
+// Boundary row of Fapply at node (i, j, k) of region reg with normal nrm:
+// one-sided gradients, sigma = C : grad u, and the Constant BC row.
+template <typename T, int SYM>
+__device__ __forceinline__ void apply_boundary(const T* __restrict__ u, const T* __restrict__ C,
+                                               const int* __restrict__ bct, const Lvl& L, int i, int j, int k,
+                                               long id, int reg, const T nrm[3], T out[3]) {
+    const long N = L.N, sy = L.nx, sz = (long)L.nx * L.ny;
+    const T hx = T(L.dx), hy = T(L.dy), hz = T(L.dz);
+    T uu[3], gu[3][3], sg[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const T* f = u + p * N;
+        uu[p] = f[id];
+        gu[p][0] = d1(f, id, 1, i, L.nx, hx);
+        gu[p][1] = d1(f, id, sy, j, L.ny, hy);
+        gu[p][2] = d1(f, id, sz, k, L.nz, hz);
+    }
+    if constexpr (SYM == 1) {
+        contract2_mm_stream<T>(C, N, id, gu, sg);
+    } else {
+        constexpr int NC = NComp<SYM>::value;
+        T c[NC];
+#pragma unroll
+        for (int s = 0; s < NC; ++s) c[s] = C[s * N + id];
+        contract2<T, SYM>(c, gu, sg);
+    }
+    bc_set<T>(bct + 3 * reg, uu, gu, sg, nrm, out);
+}
+
+// Interior row of Fapply at node id (no face of the box through it).
+template <typename T, int SYM>
+__device__ __forceinline__ void apply_interior(const T* __restrict__ u, const T* __restrict__ C, int uniform,
+                                               const Lvl& L, long id, T out[3], unsigned gmask) {
+    if constexpr (SYM == 1) {
+        apply_interior_mm<T>(u, C, L, (unsigned)id, out, gmask, uniform != 0);
+    } else {
+        constexpr int NC = NComp<SYM>::value;
+        const long N = L.N, sy = L.nx, sz = (long)L.nx * L.ny;
+        const T hx = T(L.dx), hy = T(L.dy), hz = T(L.dz);
+        T c[NC];
+#pragma unroll
+        for (int s = 0; s < NC; ++s) c[s] = C[s * N + id];
+        T gu[3][3], B[3][3][3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const T* f = u + p * N;
+            const T c0 = f[id];
+            const T xp = f[id + 1], xm = f[id - 1], yp = f[id + sy], ym = f[id - sy], zp = f[id + sz], zm = f[id - sz];
+            gu[p][0] = (xp - xm) * T(0.5) / hx;
+            gu[p][1] = (yp - ym) * T(0.5) / hy;
+            gu[p][2] = (zp - zm) * T(0.5) / hz;
+            B[p][0][0] = (xp - T(2) * c0 + xm) / hx / hx;
+            B[p][1][1] = (yp - T(2) * c0 + ym) / hy / hy;
+            B[p][2][2] = (zp - T(2) * c0 + zm) / hz / hz;
+            B[p][0][1] = T(0.25) * (f[id + 1 + sy] + f[id - 1 - sy] - f[id + 1 - sy] - f[id - 1 + sy]) / (hx * hy);
+            B[p][0][2] = T(0.25) * (f[id + 1 + sz] + f[id - 1 - sz] - f[id + 1 - sz] - f[id - 1 + sz]) / (hx * hz);
+            B[p][1][2] = T(0.25) * (f[id + sy + sz] + f[id - sy - sz] - f[id + sy - sz] - f[id - sy + sz]) / (hy * hz);
+            B[p][1][0] = B[p][0][1];
+            B[p][2][0] = B[p][0][2];
+            B[p][2][1] = B[p][1][2];
+        }
+        contract3<T, SYM>(c, B, out);
+        if (!uniform) {
+            // f += sum_a ((dC/dx_a) : grad u).col(a)   (src/Operator/Elastic.cpp:245-254)
+            const long st[3] = {1, sy, sz};
+            const T h[3] = {hx, hy, hz};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                // Bit a of gmask clear: C(+a) and C(-a) are equal slot by slot, so
+                // the central difference and its contribution are exactly zero.
+                if (!((gmask >> a) & 1u)) continue;
+                T cg[NC], col[3];
+#pragma unroll
+                for (int s = 0; s < NC; ++s) cg[s] = (C[s * N + id + st[a]] - C[s * N + id - st[a]]) * T(0.5) / h[a];
+                contract2_col<T, SYM>(cg, gu, a, col);
+                out[0] += col[0];
+                out[1] += col[1];
+                out[2] += col[2];
+            }
+        }
+    }
+}
+
+// Elastic<SYM>::Fapply at one node (src/Operator/Elastic.cpp:165-290); psi
+// unset (psi_avg = 1).
 template <typename T, int SYM>
 __device__ __forceinline__ void apply_node(const T* __restrict__ u, const T* __restrict__ C,
                                            const int* __restrict__ bct, int uniform, const Lvl& L, int i, int j,
                                            int k, long id, T out[3], unsigned gmask = 7u) {
-    constexpr int NC = NComp<SYM>::value;
-    const long N = L.N, sy = L.nx, sz = (long)L.nx * L.ny;
-    const T hx = T(L.dx), hy = T(L.dy), hz = T(L.dz);
-    if constexpr (SYM == 1) {
-        T nrm[3];
-        const int reg = region_of<T>(i, j, k, L, nrm);
-        if (reg < 0) {
-            apply_interior_mm<T>(u, C, L, (unsigned)id, out, gmask, uniform != 0);
-            return;
-        }
-        T uu[3], gu[3][3], sg[3][3];
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            const T* f = u + p * N;
-            uu[p] = f[id];
-            gu[p][0] = d1(f, id, 1, i, L.nx, hx);
-            gu[p][1] = d1(f, id, sy, j, L.ny, hy);
-            gu[p][2] = d1(f, id, sz, k, L.nz, hz);
-        }
-        contract2_mm_stream<T>(C, N, id, gu, sg);
-        bc_set<T>(bct + 3 * reg, uu, gu, sg, nrm, out);
-        return;
-    }
-    T c[NC];
-#pragma unroll
-    for (int s = 0; s < NC; ++s) c[s] = C[s * N + id];
     T nrm[3];
     const int reg = region_of<T>(i, j, k, L, nrm);
-    T gu[3][3];
-    if (reg >= 0) {
-        T uu[3], sg[3][3];
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            const T* f = u + p * N;
-            uu[p] = f[id];
-            gu[p][0] = d1(f, id, 1, i, L.nx, hx);
-            gu[p][1] = d1(f, id, sy, j, L.ny, hy);
-            gu[p][2] = d1(f, id, sz, k, L.nz, hz);
-        }
-        contract2<T, SYM>(c, gu, sg);
-        bc_set<T>(bct + 3 * reg, uu, gu, sg, nrm, out);
-        return;
-    }
-    T B[3][3][3];
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-        const T* f = u + p * N;
-        const T c0 = f[id];
-        const T xp = f[id + 1], xm = f[id - 1], yp = f[id + sy], ym = f[id - sy], zp = f[id + sz], zm = f[id - sz];
-        gu[p][0] = (xp - xm) * T(0.5) / hx;
-        gu[p][1] = (yp - ym) * T(0.5) / hy;
-        gu[p][2] = (zp - zm) * T(0.5) / hz;
-        B[p][0][0] = (xp - T(2) * c0 + xm) / hx / hx;
-        B[p][1][1] = (yp - T(2) * c0 + ym) / hy / hy;
-        B[p][2][2] = (zp - T(2) * c0 + zm) / hz / hz;
-        B[p][0][1] = T(0.25) * (f[id + 1 + sy] + f[id - 1 - sy] - f[id + 1 - sy] - f[id - 1 + sy]) / (hx * hy);
-        B[p][0][2] = T(0.25) * (f[id + 1 + sz] + f[id - 1 - sz] - f[id + 1 - sz] - f[id - 1 + sz]) / (hx * hz);
-        B[p][1][2] = T(0.25) * (f[id + sy + sz] + f[id - sy - sz] - f[id + sy - sz] - f[id - sy + sz]) / (hy * hz);
-        B[p][1][0] = B[p][0][1];
-        B[p][2][0] = B[p][0][2];
-        B[p][2][1] = B[p][1][2];
-    }
-    contract3<T, SYM>(c, B, out);
-    if (!uniform) {
-        // f += sum_a ((dC/dx_a) : grad u).col(a)   (src/Operator/Elastic.cpp:245-254)
-        const long st[3] = {1, sy, sz};
-        const T h[3] = {hx, hy, hz};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            // Bit a of gmask clear: C(+a) and C(-a) are equal slot by slot, so
-            // the central difference and its contribution are exactly zero.
-            if (!((gmask >> a) & 1u)) continue;
-            T cg[NC], col[3];
-#pragma unroll
-            for (int s = 0; s < NC; ++s) cg[s] = (C[s * N + id + st[a]] - C[s * N + id - st[a]]) * T(0.5) / h[a];
-            contract2_col<T, SYM>(cg, gu, a, col);
-            out[0] += col[0];
-            out[1] += col[1];
-            out[2] += col[2];
-        }
-    }
+    if (reg >= 0) apply_boundary<T, SYM>(u, C, bct, L, i, j, k, id, reg, nrm, out);
+    else apply_interior<T, SYM>(u, C, uniform, L, id, out, gmask);
 }
 
 // Elastic<SYM>::Diagonal (src/Operator/Elastic.cpp:321-374) with

@@ -1,8 +1,9 @@
 // stencil.h -- the nodal elastic operator kernels used by every MG level.
 //
-// One thread per node.  The launch is XCD-slab ordered (slab_node) so that the
-// planes a node reads stay in one XCD's L2, and the optional grad-C mask skips
-// the neighbour loads of axes along which C does not change.
+// One thread per node.  Boundary nodes get workgroups of their own; interior
+// nodes are XCD-slab ordered (slab_interior_node) so that the planes a node
+// reads stay in one XCD's L2, and the optional grad-C mask skips the
+// neighbour loads of axes along which C does not change.
 //
 // OP 0: out = A x            (Elastic::Fapply, src/Operator/Elastic.cpp:139-292)
 // OP 1: out = b - A x        (correctionResidual, src/Operator/Operator.cpp:647-657)
@@ -42,51 +43,113 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// XCD-slab node mapping.  Workgroups are dispatched to the kXcd XCDs round
-// robin (workgroup b runs on XCD b % kXcd), each XCD with its own L2.  XCD x
-// is given the rows j in [x*ny/kXcd, (x+1)*ny/kXcd) of every plane and walks
-// that slab in (k, j, i) order, so a node's k-1 and k+1 neighbours were read
-// a few workgroups earlier, or will be read a few later, on the same L2: u and
-// C are fetched from HBM about once (plus the two rows each slab shares with
-// its neighbours) instead of once per plane that reads them.
-__device__ __forceinline__ bool slab_node(const Lvl& L, int& i, int& j, int& k, long& id) {
-    const int x = (int)(blockIdx.x % kXcd);
-    const long lb = blockIdx.x / kXcd;
-    const int j0 = (int)((long)x * L.ny / kXcd), j1 = (int)((long)(x + 1) * L.ny / kXcd);
-    const long per_plane = (long)(j1 - j0) * L.nx;
+// Launch layout of the stencil kernels.  The first nbb workgroups (a
+// multiple of kXcd) take the nodes on the faces of the box, the rest the
+// interior box [1, nx-2] x [1, ny-2] x [1, nz-2].  Boundary rows need
+// one-sided gradients and the BC table; a wave that mixed them with interior
+// nodes would run both code paths, and with rows of nx nodes straddling
+// 64-lane waves about half of all waves would.
+//
+// Interior nodes are XCD-slab ordered.  Workgroups are dispatched to the kXcd
+// XCDs round robin (workgroup b runs on XCD b % kXcd), each XCD with its own
+// L2.  XCD x is given the interior rows j in [x*nj/kXcd, (x+1)*nj/kXcd) of
+// every plane and walks that slab in (k, j, i) order, so a node's k-1 and k+1
+// neighbours were read a few workgroups earlier, or will be read a few later,
+// on the same L2: u and C are fetched from HBM about once (plus the rows each
+// slab shares with its neighbours) instead of once per plane that reads them.
+__host__ __device__ inline long interior_count(const Lvl& L) {
+    return L.nx > 2 && L.ny > 2 && L.nz > 2 ? (long)(L.nx - 2) * (L.ny - 2) * (L.nz - 2) : 0;
+}
+__host__ __device__ inline long boundary_count(const Lvl& L) { return L.N - interior_count(L); }
+
+// The t-th boundary node, t < boundary_count(L): the planes k = 0 and
+// k = nz-1 whole, then for each inner plane its ring of face nodes.
+__device__ __forceinline__ void boundary_node(const Lvl& L, long t, int& i, int& j, int& k) {
+    const long plane = (long)L.nx * L.ny;
+    if (t < 2 * plane) {
+        const bool top = t >= plane;
+        const long r = top ? t - plane : t;
+        k = top ? L.nz - 1 : 0;
+        j = (int)(r / L.nx);
+        i = (int)(r - (long)j * L.nx);
+        return;
+    }
+    t -= 2 * plane;
+    const long ring = 2L * L.nx + 2L * (L.ny - 2);
+    const long kk = t / ring;
+    long r = t - kk * ring;
+    k = 1 + (int)kk;
+    if (r < L.nx) {
+        j = 0;
+        i = (int)r;
+    } else if (r < 2L * L.nx) {
+        j = L.ny - 1;
+        i = (int)(r - L.nx);
+    } else {
+        r -= 2L * L.nx;
+        j = 1 + (int)(r >> 1);
+        i = (r & 1) ? L.nx - 1 : 0;
+    }
+}
+
+// Interior node of interior workgroup ib (blockIdx.x - nbb), XCD-slab order.
+__device__ __forceinline__ bool slab_interior_node(const Lvl& L, long ib, int& i, int& j, int& k, long& id) {
+    const int ni = L.nx - 2, nj = L.ny - 2, nk = L.nz - 2;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return false;
+    const int x = (int)(ib % kXcd);
+    const long lb = ib / kXcd;
+    const int j0 = (int)((long)x * nj / kXcd), j1 = (int)((long)(x + 1) * nj / kXcd);
+    const long per_plane = (long)(j1 - j0) * ni;
     const long t = lb * blockDim.x + threadIdx.x;
-    if (t >= per_plane * L.nz) return false;
-    k = (int)(t / per_plane);
-    const long r = t - (long)k * per_plane;
-    const int jj = (int)(r / L.nx);
-    i = (int)(r - (long)jj * L.nx);
-    j = j0 + jj;
+    if (t >= per_plane * nk) return false;
+    const long kk = t / per_plane;
+    const long r = t - kk * per_plane;
+    const int jj = (int)(r / ni);
+    i = 1 + (int)(r - (long)jj * ni);
+    j = 1 + j0 + jj;
+    k = 1 + (int)kk;
     id = ((long)k * L.ny + j) * L.nx + i;
     return true;
 }
 
-// Workgroups of the XCD-slab launch: kXcd times the blocks of the tallest slab.
+// Boundary workgroups (rounded up to a multiple of kXcd).
+inline unsigned slab_boundary_blocks(const Lvl& L, int block) {
+    const long nb = (boundary_count(L) + block - 1) / block;
+    return (unsigned)((nb + kXcd - 1) / kXcd * kXcd);
+}
+// All workgroups: the boundary ones, then kXcd times those of the tallest slab.
 inline unsigned slab_blocks(const Lvl& L, int block) {
-    const long rows = (L.ny + kXcd - 1) / kXcd;
-    const long per_xcd = (rows * L.nx * L.nz + block - 1) / block;
-    return (unsigned)(per_xcd * kXcd);
+    const long nj = L.ny > 2 ? L.ny - 2 : 0;
+    const long rows = (nj + kXcd - 1) / kXcd;
+    const long per_xcd = (rows * (L.nx > 2 ? L.nx - 2 : 0) * (L.nz > 2 ? L.nz - 2 : 0) + block - 1) / block;
+    return slab_boundary_blocks(L, block) + (unsigned)(per_xcd * kXcd);
 }
 
-// One thread per node, XCD-slab order, any OP.  gmask (optional, one byte per
-// node) holds bit a set where C(+a) != C(-a) in some slot; the grad-C
-// contribution of a clear axis is exactly zero and its 2 x NC neighbour loads
-// are skipped.
+// One thread per node in the layout above, any OP.  gmask (optional, one
+// byte per node) holds bit a set where C(+a) != C(-a) in some slot; the
+// grad-C contribution of a clear axis is exactly zero and its 2 x NC
+// neighbour loads are skipped.
 template <typename T, int SYM, int OP>
 __global__ void __launch_bounds__(kBlock)
     k_stencil_slab(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
                    const T* __restrict__ C, const int* __restrict__ bct, int uniform, Lvl L, T omega,
-                   const unsigned char* __restrict__ gmask, T* __restrict__ out) {
+                   const unsigned char* __restrict__ gmask, unsigned nbb, T* __restrict__ out) {
     int i, j, k;
     long id;
-    if (!slab_node(L, i, j, k, id)) return;
     T o[3];
-    const unsigned gm = gmask ? (unsigned)gmask[id] : 7u;
-    apply_node<T, SYM>(x, C, bct, uniform, L, i, j, k, id, o, gm);
+    if (blockIdx.x < nbb) {
+        const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= boundary_count(L)) return;
+        boundary_node(L, t, i, j, k);
+        id = ((long)k * L.ny + j) * L.nx + i;
+        T nrm[3];
+        const int reg = region_of<T>(i, j, k, L, nrm);
+        apply_boundary<T, SYM>(x, C, bct, L, i, j, k, id, reg, nrm, o);
+    } else {
+        if (!slab_interior_node(L, (long)(blockIdx.x - nbb), i, j, k, id)) return;
+        const unsigned gm = gmask ? (unsigned)gmask[id] : 7u;
+        apply_interior<T, SYM>(x, C, uniform, L, id, o, gm);
+    }
     const unsigned n32 = (unsigned)id;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {

@@ -60,6 +60,7 @@ struct Vecs {
     T* res = nullptr;
     T* rescor = nullptr;
     T* tmp = nullptr;
+    T* D = nullptr;  // compact grad-C matrices, 18 x nflag (MajorMinor)
 };
 
 struct Level {
@@ -67,6 +68,8 @@ struct Level {
     Vecs<double> d;
     Vecs<float> f;
     unsigned char* gmask = nullptr;  // per node: axes along which C varies (k_grad_mask)
+    int* dwbase = nullptr;           // per interior wave of the slab launch: first compact D position
+    int nflag = 0;                   // interior nodes with a nonzero gmask
 };
 
 template <typename T> Vecs<T>& vecs(Level& l);
@@ -133,8 +136,10 @@ constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M do
 // ---------------------------------------------------------------------------
 // Stencil kernel variant: the XCD-slab per-node kernel with the grad-C mask
 // (default), the same without the mask, or the linear per-node kernel.
-// ALAMO_EL_STENCIL=slab|slab-nomask|node selects one; all three give the same
-// result bit for bit.
+// ALAMO_EL_STENCIL=slab|slab-nomask|node selects one.  slab-nomask and node
+// give the same result bit for bit; for MajorMinor the default slab kernel
+// sums the grad-C term through the compact matrices D (one rounding order
+// apart, about 1e-16 relative), for the other symmetries it agrees bit for bit.
 enum { kVariantSlab = 0, kVariantSlabNoMask = 1, kVariantNode = 2 };
 
 int stencil_variant() {
@@ -166,10 +171,19 @@ void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const 
     }
     const unsigned g = ael::slab_blocks(L, kBlock), nbb = ael::slab_boundary_blocks(L, kBlock);
     const unsigned char* gm = var == kVariantSlab ? l.gmask : nullptr;
+    // compact grad-C matrices: MajorMinor, masked variant, built for this level
+    Level& lv = const_cast<Level&>(l);
+    const bool dc = gm && h->sym == 1 && l.dwbase && (l.nflag == 0 || vecs<T>(lv).D);
+    const int* dwb = dc ? l.dwbase : nullptr;
+    const T* Dc = dc ? vecs<T>(lv).D : nullptr;
+    const int nf = dc ? l.nflag : 0;
     switch (h->sym) {
-        case 0: ael::k_stencil_slab<T, 0, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, out); break;
-        case 1: ael::k_stencil_slab<T, 1, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, out); break;
-        default: ael::k_stencil_slab<T, 2, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, out); break;
+        case 0: ael::k_stencil_slab<T, 0, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out); break;
+        case 1:
+            if (dc) ael::k_stencil_slab<T, 1, OP, true><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out);
+            else ael::k_stencil_slab<T, 1, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out);
+            break;
+        default: ael::k_stencil_slab<T, 2, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out); break;
     }
 }
 
@@ -442,12 +456,52 @@ int assemble_dense(alamo_el_t h, hipStream_t s) {
     return 0;
 }
 
-void build_gmask(alamo_el_t h, Level& l, hipStream_t s) {
+// Compact grad-C matrices of one level (MajorMinor): per-wave counts of the
+// flagged interior nodes, their exclusive prefix (on the host, once per
+// model), then D of each flagged node at its position.
+int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
+    if (h->sym != 1) return 0;
+    const unsigned nib = ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
+    const long nw = (long)nib * ael::kWavesPerBlock;
+    if (l.dwbase) (void)hipFree(l.dwbase);
+    if (l.d.D) (void)hipFree(l.d.D);
+    if (l.f.D) (void)hipFree(l.f.D);
+    l.dwbase = nullptr;
+    l.d.D = nullptr;
+    l.f.D = nullptr;
+    l.nflag = 0;
+    if (nw == 0) return 0;
+    HIPCHK(hipMalloc((void**)&l.dwbase, sizeof(int) * nw));
+    ael::k_gradc_count<<<nib, kBlock, 0, s>>>(l.gmask, l.L, l.dwbase);
+    HIPCHK(hipGetLastError());
+    std::vector<int> cnt(nw);
+    HIPCHK(hipMemcpyAsync(cnt.data(), l.dwbase, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    long tot = 0;
+    for (long w = 0; w < nw; ++w) {
+        const int c = cnt[w];
+        cnt[w] = (int)tot;
+        tot += c;
+    }
+    HIPCHK(hipMemcpyAsync(l.dwbase, cnt.data(), sizeof(int) * nw, hipMemcpyHostToDevice, s));
+    l.nflag = (int)tot;
+    if (tot > 0) {
+        HIPCHK(hipMalloc((void**)&l.d.D, sizeof(double) * 18 * tot));
+        ael::k_gradc_fill<<<nib, kBlock, 0, s>>>(l.d.C, l.gmask, l.L, l.dwbase, l.nflag, l.d.D);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));  // cnt is read by the copy above
+    return 0;
+}
+
+int build_gmask(alamo_el_t h, Level& l, hipStream_t s) {
     switch (h->sym) {
         case 0: ael::k_grad_mask<2><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, l.gmask); break;
         case 1: ael::k_grad_mask<21><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, l.gmask); break;
         default: ael::k_grad_mask<9><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, l.gmask); break;
     }
+    HIPCHK(hipGetLastError());
+    return build_gradc(h, l, s);
 }
 
 int prepare(alamo_el_t h, hipStream_t s) {
@@ -457,7 +511,8 @@ int prepare(alamo_el_t h, hipStream_t s) {
         Level& c = h->lv[l];
         Level& f = h->lv[l - 1];
         ael::k_restrict<double><<<nblk8(c.L.N), kBlock, 0, s>>>(f.d.C, f.L, c.d.C, c.L, h->nc);
-        build_gmask(h, c, s);
+        int rc = build_gmask(h, c, s);
+        if (rc) return rc;
     }
     // Diagonal(true) on every MG level (src/Operator/Operator.cpp:16-29)
     for (auto& l : h->lv) launch_diagonal<double>(h, l, l.d.C, l.d.diag, s);
@@ -477,6 +532,11 @@ int prepare32(alamo_el_t h, hipStream_t s) {
         const long N = l.L.N;
         ael::k_cast<double, float><<<nblk((long)h->nc * N), kBlock, 0, s>>>(l.d.C, l.f.C, (long)h->nc * N);
         ael::k_cast<double, float><<<nblk(3 * N), kBlock, 0, s>>>(l.d.diag, l.f.diag, 3 * N);
+        if (l.d.D) {
+            if (l.f.D) (void)hipFree(l.f.D);
+            HIPCHK(hipMalloc((void**)&l.f.D, sizeof(float) * 18 * (long)l.nflag));
+            ael::k_cast<double, float><<<nblk(18L * l.nflag), kBlock, 0, s>>>(l.d.D, l.f.D, 18L * l.nflag);
+        }
     }
     HIPCHK(hipGetLastError());
     h->prepared32 = true;
@@ -707,11 +767,12 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
 int alamo_el_destroy(alamo_el_t h) {
     if (!h) return 0;
     for (auto& l : h->lv) {
-        for (double* p : {l.d.C, l.d.diag, l.d.cor, l.d.cor_hold, l.d.res, l.d.rescor, l.d.tmp})
+        for (double* p : {l.d.C, l.d.diag, l.d.cor, l.d.cor_hold, l.d.res, l.d.rescor, l.d.tmp, l.d.D})
             if (p) (void)hipFree(p);
-        for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp})
+        for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp, l.f.D})
             if (p) (void)hipFree(p);
         if (l.gmask) (void)hipFree(l.gmask);
+        if (l.dwbase) (void)hipFree(l.dwbase);
     }
     for (void* p : {(void*)h->bct_dev, (void*)h->bcv_dev, (void*)h->red_partial, (void*)h->red_out,
                     (void*)h->bottom_work, (void*)h->bottom_status, (void*)h->gu9, (void*)h->dw9, (void*)h->csoa,
@@ -778,8 +839,8 @@ int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t s)
     } else {
         return fail(ALAMO_EL_EINVAL, "bad layout");
     }
-    build_gmask(h, l, s);
-    HIPCHK(hipGetLastError());
+    int rc = build_gmask(h, l, s);
+    if (rc) return rc;
     h->model_set = true;
     h->prepared = false;
     h->prepared32 = false;

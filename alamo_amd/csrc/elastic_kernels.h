@@ -273,8 +273,8 @@ __device__ __forceinline__ void gradc_mm(const T* __restrict__ C, const Lvl& L, 
 // derivative combinations e[j][Q], the 6 symmetric gradients e1[Q] and the
 // result stay live.  Grid spacings enter through reciprocals computed once.
 template <typename T>
-__device__ __forceinline__ void apply_interior_mm(const T* __restrict__ u, const T* __restrict__ C, const Lvl& L,
-                                                  unsigned o, T out[3], unsigned gmask, bool uniform) {
+__device__ __forceinline__ void mm_core(const T* __restrict__ u, const T* __restrict__ C, const Lvl& L, unsigned o,
+                                        T out[3], T e1[6]) {
     const unsigned N = (unsigned)L.N, sy = (unsigned)L.nx, sz = (unsigned)(L.nx * L.ny);
     const T ihx = T(1) / T(L.dx), ihy = T(1) / T(L.dy), ihz = T(1) / T(L.dz);
     const T hx2 = T(0.5) * ihx, hy2 = T(0.5) * ihy, hz2 = T(0.5) * ihz;
@@ -300,7 +300,7 @@ __device__ __forceinline__ void apply_interior_mm(const T* __restrict__ u, const
         B[p][2][0] = B[p][0][2];
         B[p][2][1] = B[p][1][2];
     }
-    T e[3][6], e1[6];
+    T e[3][6];
 #pragma unroll
     for (int Q = 0; Q < 6; ++Q) {
         const int qa = pair_a(Q), qb = pair_b(Q);
@@ -320,8 +320,41 @@ __device__ __forceinline__ void apply_interior_mm(const T* __restrict__ u, const
             if (Q != P && pair_other(Q, j) >= 0) out[pair_other(Q, j)] += c * e[j][P];
         }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_interior_mm(const T* __restrict__ u, const T* __restrict__ C, const Lvl& L,
+                                                  unsigned o, T out[3], unsigned gmask, bool uniform) {
+    T e1[6];
+    mm_core<T>(u, C, L, o, out, e1);
     if (uniform) return;
     gradc_mm<T>(C, L, o, gmask, e1, out);
+}
+
+// The grad-C term of gradc_mm as a matrix on the symmetric gradient pairs:
+// sum_a ((dC/dx_a) : grad u).col(a) = D e1 with D[r][Q] (r = output row,
+// Q = pair) the sum of the central differences of the slots that meet row r
+// and pair Q.  D is zero where C(+a) == C(-a) for every axis a, and it
+// depends on C only, so it is computed once per model (FP64).
+__device__ __forceinline__ void gradc_matrix_mm(const double* __restrict__ C, const Lvl& L, unsigned o,
+                                                double D[18]) {
+    const unsigned N = (unsigned)L.N, sy = (unsigned)L.nx, sz = (unsigned)(L.nx * L.ny);
+    const unsigned st[3] = {1u, sy, sz};
+    const double h2[3] = {0.5 * (1.0 / L.dx), 0.5 * (1.0 / L.dy), 0.5 * (1.0 / L.dz)};
+#pragma unroll
+    for (int q = 0; q < 18; ++q) D[q] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int s = 0; s < 21; ++s) {
+            if (!slot_in_col(s, a)) continue;
+            const int P = slot_p(s), Q = slot_q(s);
+            const double* Cs = C + (size_t)s * N;
+            const double d = (ld32(Cs, o + st[a]) - ld32(Cs, o - st[a])) * h2[a];
+            if (pair_other(P, a) >= 0) D[6 * pair_other(P, a) + Q] += d;
+            if (Q != P && pair_other(Q, a) >= 0) D[6 * pair_other(Q, a) + P] += d;
+        }
+    }
 }
 
 // sigma = C : A for MajorMinor C, streamed over the slots (contract2).

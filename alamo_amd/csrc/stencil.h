@@ -125,15 +125,60 @@ inline unsigned slab_blocks(const Lvl& L, int block) {
     return slab_boundary_blocks(L, block) + (unsigned)(per_xcd * kXcd);
 }
 
+// Compact grad-C matrices (MajorMinor).  D (gradc_matrix_mm) is stored only
+// for the interior nodes with a nonzero gmask, in the order the interior
+// waves of the slab launch meet them: in interior wave w, the flagged lanes
+// take positions dwbase[w] + (number of flagged lanes below them), and entry
+// q of D sits at Dc[q * nflag + p].  A wave reads its D entries with 18
+// coalesced loads instead of the up to 2 x 15 neighbour loads per axis of the
+// central differences of C.
+constexpr int kWavesPerBlock = kBlock / 64;
+
+__device__ __forceinline__ unsigned long long lanes_below() {
+    const unsigned lane = threadIdx.x & 63u;
+    return lane ? (~0ull >> (64u - lane)) : 0ull;
+}
+
+// Flagged lanes per interior wave (grid: the interior workgroups only).
+__global__ void __launch_bounds__(kBlock) k_gradc_count(const unsigned char* __restrict__ gmask, Lvl L,
+                                                        int* __restrict__ wcount) {
+    int i, j, k;
+    long id = 0;
+    const bool v = slab_interior_node(L, (long)blockIdx.x, i, j, k, id);
+    const unsigned long long bal = __ballot(v && gmask[id] != 0);
+    if ((threadIdx.x & 63u) == 0)
+        wcount[(long)blockIdx.x * kWavesPerBlock + threadIdx.x / 64] = (int)__popcll(bal);
+}
+
+// D of every flagged interior node at its compact position (FP64).
+__global__ void __launch_bounds__(kBlock) k_gradc_fill(const double* __restrict__ C,
+                                                       const unsigned char* __restrict__ gmask, Lvl L,
+                                                       const int* __restrict__ dwbase, int nflag,
+                                                       double* __restrict__ Dc) {
+    int i, j, k;
+    long id = 0;
+    const bool v = slab_interior_node(L, (long)blockIdx.x, i, j, k, id);
+    const bool f = v && gmask[id] != 0;
+    const unsigned long long bal = __ballot(f);
+    if (!f) return;
+    const int p = dwbase[(long)blockIdx.x * kWavesPerBlock + threadIdx.x / 64] + (int)__popcll(bal & lanes_below());
+    double D[18];
+    gradc_matrix_mm(C, L, (unsigned)id, D);
+#pragma unroll
+    for (int q = 0; q < 18; ++q) Dc[(size_t)q * nflag + p] = D[q];
+}
+
 // One thread per node in the layout above, any OP.  gmask (optional, one
 // byte per node) holds bit a set where C(+a) != C(-a) in some slot; the
 // grad-C contribution of a clear axis is exactly zero and its 2 x NC
-// neighbour loads are skipped.
-template <typename T, int SYM, int OP>
+// neighbour loads are skipped.  DC (MajorMinor): the grad-C term of the
+// flagged nodes is D e1 with D read from the compact array Dc.
+template <typename T, int SYM, int OP, bool DC = false>
 __global__ void __launch_bounds__(kBlock)
     k_stencil_slab(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
                    const T* __restrict__ C, const int* __restrict__ bct, int uniform, Lvl L, T omega,
-                   const unsigned char* __restrict__ gmask, unsigned nbb, T* __restrict__ out) {
+                   const unsigned char* __restrict__ gmask, unsigned nbb, const int* __restrict__ dwbase,
+                   const T* __restrict__ Dc, int nflag, T* __restrict__ out) {
     int i, j, k;
     long id;
     T o[3];
@@ -146,9 +191,26 @@ __global__ void __launch_bounds__(kBlock)
         const int reg = region_of<T>(i, j, k, L, nrm);
         apply_boundary<T, SYM>(x, C, bct, L, i, j, k, id, reg, nrm, o);
     } else {
-        if (!slab_interior_node(L, (long)(blockIdx.x - nbb), i, j, k, id)) return;
+        const long ib = (long)(blockIdx.x - nbb);
+        if (!slab_interior_node(L, ib, i, j, k, id)) return;
         const unsigned gm = gmask ? (unsigned)gmask[id] : 7u;
-        apply_interior<T, SYM>(x, C, uniform, L, id, o, gm);
+        if constexpr (SYM == 1 && DC) {
+            {
+                T e1[6];
+                mm_core<T>(x, C, L, (unsigned)id, o, e1);
+                const unsigned long long bal = __ballot(gm != 0u);
+                if (!uniform && gm != 0u) {
+                    const unsigned p = (unsigned)dwbase[ib * kWavesPerBlock + threadIdx.x / 64] +
+                                       (unsigned)__popcll(bal & lanes_below());
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int Q = 0; Q < 6; ++Q) o[r] += ld32(Dc + (size_t)(6 * r + Q) * nflag, p) * e1[Q];
+                }
+            }
+        } else {
+            apply_interior<T, SYM>(x, C, uniform, L, id, o, gm);
+        }
     }
     const unsigned n32 = (unsigned)id;
 #pragma unroll

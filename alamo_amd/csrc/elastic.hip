@@ -27,6 +27,8 @@
 #include "amr_kernels.h"
 #include "learned_kernels.h"
 #include "stencil.h"
+#include "stencil_lds.h"
+#include "coarse_cycle.h"
 
 using ael::kBlock;
 using ael::Lvl;
@@ -68,8 +70,10 @@ struct Level {
     Vecs<double> d;
     Vecs<float> f;
     unsigned char* gmask = nullptr;  // per node: axes along which C varies (k_grad_mask)
-    int* dwbase = nullptr;           // per interior wave of the slab launch: first compact D position
+    int* dwbase = nullptr;           // per interior wave of the launch: first compact D position
     int nflag = 0;                   // interior nodes with a nonzero gmask
+    bool tiled = false;              // MajorMinor level served by k_stencil_lds (else k_stencil_slab)
+    ael::TileGrid tg{};              // its tile grid
 };
 
 template <typename T> Vecs<T>& vecs(Level& l);
@@ -88,6 +92,7 @@ struct alamo_el_s {
     double bcv_host[78];
     int uniform = 0;
     double omega = 2.0 / 3.0;
+    long lds_max = 0;  // LDS bytes one workgroup may use on this device
     bool model_set = false, prepared = false, have32 = false, prepared32 = false;
     double* red_partial = nullptr;
     double* red_out = nullptr;
@@ -140,7 +145,13 @@ constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M do
 // give the same result bit for bit; for MajorMinor the default slab kernel
 // sums the grad-C term through the compact matrices D (one rounding order
 // apart, about 1e-16 relative), for the other symmetries it agrees bit for bit.
-enum { kVariantSlab = 0, kVariantSlabNoMask = 1, kVariantNode = 2 };
+enum { kVariantSlab = 0, kVariantSlabNoMask = 1, kVariantNode = 2, kVariantSlabOnly = 3 };
+
+// Tile shape of the LDS-tiled kernel: 32 x 8 nodes, four waves (1-wave
+// 32 x 2 and 16 x 4 tiles and 64 x 1 rows measured no faster).
+using TileT = ael::Tile<32, 8>;
+// about 8 workgroups per CU
+ael::TileGrid tile_grid_of(const Lvl& L) { return ael::tile_grid<TileT>(L, 2048); }
 
 int stencil_variant() {
     static int v = -1;
@@ -149,6 +160,7 @@ int stencil_variant() {
         v = kVariantSlab;
         if (e && std::strcmp(e, "slab-nomask") == 0) v = kVariantSlabNoMask;
         if (e && std::strcmp(e, "node") == 0) v = kVariantNode;
+        if (e && std::strcmp(e, "slab-only") == 0) v = kVariantSlabOnly;
     }
     return v;
 }
@@ -170,10 +182,18 @@ void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const 
         return;
     }
     const unsigned g = ael::slab_blocks(L, kBlock), nbb = ael::slab_boundary_blocks(L, kBlock);
-    const unsigned char* gm = var == kVariantSlab ? l.gmask : nullptr;
-    // compact grad-C matrices: MajorMinor, masked variant, built for this level
+    const bool masked = var == kVariantSlab || var == kVariantSlabOnly;
+    const unsigned char* gm = masked ? l.gmask : nullptr;
     Level& lv = const_cast<Level&>(l);
-    const bool dc = gm && h->sym == 1 && l.dwbase && (l.nflag == 0 || vecs<T>(lv).D);
+    const bool have_d = l.dwbase && (l.nflag == 0 || vecs<T>(lv).D);
+    if (masked && h->sym == 1 && l.tiled && have_d) {
+        // LDS-tiled kernel; its compact D is in tile order
+        ael::k_stencil_lds<TileT, T, OP, true><<<nbb + ael::tile_blocks(l.tg), TileT::NT, 0, s>>>(
+            x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase, vecs<T>(lv).D, l.nflag, out);
+        return;
+    }
+    // compact grad-C matrices in slab order: MajorMinor, masked variant, built for this level
+    const bool dc = gm && h->sym == 1 && !l.tiled && have_d;
     const int* dwb = dc ? l.dwbase : nullptr;
     const T* Dc = dc ? vecs<T>(lv).D : nullptr;
     const int nf = dc ? l.nflag : 0;
@@ -264,6 +284,18 @@ struct Cycle {
             launch_jacobi<T>(h, L(l), v.tmp, b, v.diag, v.C, x, s);
         }
     }
+    // x = 0, then Fsmooth x n; the first sweep from zero is w b / D, one
+    // elementwise kernel instead of the fill and a stencil sweep.
+    void smooth_from_zero(int l, T* x, const T* b, int n) {
+        Vecs<T>& v = V(l);
+        if (n <= 0) {
+            launch_fill<T>(x, T(0), M(l), s);
+            return;
+        }
+        ael::k_jacobi_from_zero<T><<<nblk(M(l)), kBlock, 0, s>>>(b, v.diag, (T)h->omega, v.tmp, M(l));
+        launch_jacobi<T>(h, L(l), v.tmp, b, v.diag, v.C, x, s);
+        smooth(l, x, b, n - 1);
+    }
     void corr_res(int l, T* r, const T* x, const T* b) { launch_residual<T>(h, L(l), x, b, V(l).C, r, s); }
     void restrict_res(int lf, const T* fine, T* crse) {
         ael::k_restrict<T><<<nblk8(L(lf + 1).L.N), kBlock, 0, s>>>(fine, L(lf).L, crse, L(lf + 1).L, 3);
@@ -318,18 +350,100 @@ struct Cycle {
         ael::k_cast<float, double><<<nblk(m), kBlock, 0, s>>>(lv.f.res, lv.d.res, m);
     }
 
-    // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.
+    // LDS bytes of the tail over levels [top, bot]: coefficients, diagonal,
+    // gradient mask and the four work vectors of each smoothing level, the
+    // residual and correction of the bottom.
+    long tail_bytes(int top) {
+        const int bot = nlev() - 1;
+        long elems = 0, masks = 0;
+        for (int l = top; l <= bot; ++l) {
+            const long N = L(l).L.N;
+            elems += l < bot ? (long)h->nc * N + 5 * 3 * N : 2 * 3 * N;
+            masks += l < bot ? N : 0;
+        }
+        return elems * (long)sizeof(T) + masks;
+    }
+    // First MG level of the one-workgroup tail (k_vcycle_tail): the levels of
+    // at most kTailNodes nodes above a dense direct bottom whose operands fit
+    // in LDS; nlev() when the tail is not used.
+    int tail_top() {
+        const int bot = nlev() - 1;
+        if (bottom_solver != ALAMO_EL_BOTTOM_DIRECT || !h->dense_Ainv || bot < 1 || h->lds_max <= 0) return nlev();
+        int f = bot;
+        while (f > 0 && L(f - 1).L.N <= ael::kTailNodes && bot - (f - 1) + 1 <= ael::kTailMaxLevels &&
+               tail_bytes(f - 1) <= h->lds_max)
+            --f;
+        return f < bot ? f : nlev();
+    }
+    // mgVcycle from level top to the bottom and back, one launch.
+    void tail(int top) {
+        const int bot = nlev() - 1;
+        ael::TailArgs<T> a{};
+        a.n = bot - top + 1;
+        a.nc = h->nc;
+        const bool masked = stencil_variant() != kVariantNode && stencil_variant() != kVariantSlabNoMask;
+        long e = 0;  // LDS layout: T data of every level, then the masks
+        for (int l = top; l <= bot; ++l) {
+            Vecs<T>& v = V(l);
+            ael::TailLevel<T>& t = a.lv[l - top];
+            const long N = L(l).L.N;
+            t.L = L(l).L;
+            t.cor = v.cor;
+            t.res = v.res;
+            t.diag = v.diag;
+            t.C = v.C;
+            t.gmask = masked && l < bot ? L(l).gmask : nullptr;
+            if (l < bot) {
+                t.oC = (int)e; e += (long)h->nc * N;
+                t.odiag = (int)e; e += 3 * N;
+                t.otmp = (int)e; e += 3 * N;
+                t.orescor = (int)e; e += 3 * N;
+            }
+            t.ores = (int)e; e += 3 * N;
+            t.ocor = (int)e; e += 3 * N;
+        }
+        long gb = e * (long)sizeof(T);
+        for (int l = top; l < bot; ++l) {
+            a.lv[l - top].ogm = (int)gb;
+            gb += L(l).L.N;
+        }
+        a.nu1 = nu1;
+        a.nu2 = nu2;
+        a.omega = (T)h->omega;
+        a.uniform = h->uniform;
+        a.bct = h->bct_dev;
+        a.Ainv = h->dense_Ainv;
+        const size_t bytes = (size_t)gb;
+        auto go = [&](auto kern) {
+            (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            kern<<<1, ael::kTailBlock, bytes, s>>>(a);
+        };
+        switch (h->sym) {
+            case 0: go(ael::k_vcycle_tail<T, 0>); break;
+            case 1: go(ael::k_vcycle_tail<T, 1>); break;
+            default: go(ael::k_vcycle_tail<T, 2>); break;
+        }
+    }
+
+    // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.  The
+    // levels from tail_top() down run as one k_vcycle_tail launch.
     void vcycle(int top) {
         const int bot = nlev() - 1;
-        for (int l = top; l < bot; ++l) {
+        const int f = tail_top();
+        if (top >= f && top < bot) {
+            tail(top);
+            return;
+        }
+        const int stop = std::min(f, bot);  // levels [top, stop) launch their own kernels
+        for (int l = top; l < stop; ++l) {
             Vecs<T>& v = V(l);
-            launch_fill<T>(v.cor, T(0), M(l), s);
-            smooth(l, v.cor, v.res, nu1);
+            smooth_from_zero(l, v.cor, v.res, nu1);
             corr_res(l, v.rescor, v.cor, v.res);
             restrict_res(l, v.rescor, V(l + 1).res);
         }
-        bottom();
-        for (int l = bot - 1; l >= top; --l) {
+        if (f < bot) tail(f);
+        else bottom();
+        for (int l = stop - 1; l >= top; --l) {
             Vecs<T>& v = V(l);
             interp(l, v.cor, V(l + 1).cor, 0);
             smooth(l, v.cor, v.res, nu2);
@@ -459,10 +573,20 @@ int assemble_dense(alamo_el_t h, hipStream_t s) {
 // Compact grad-C matrices of one level (MajorMinor): per-wave counts of the
 // flagged interior nodes, their exclusive prefix (on the host, once per
 // model), then D of each flagged node at its position.
+// Levels whose interior is at least kTiledMin nodes along every axis use the
+// LDS-tiled MajorMinor kernel (unless ALAMO_EL_STENCIL=slab-only).
+constexpr int kTiledMin = 32;
+
 int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     if (h->sym != 1) return 0;
-    const unsigned nib = ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
-    const long nw = (long)nib * ael::kWavesPerBlock;
+    l.tiled = stencil_variant() == kVariantSlab && l.L.nx - 2 >= kTiledMin && l.L.ny - 2 >= kTiledMin &&
+              l.L.nz - 2 >= kTiledMin;
+    static_assert(TileT::NT == kBlock, "tiled and slab launches share the block size");
+    l.tg = tile_grid_of(l.L);
+    const int wpb = ael::kWavesPerBlock;
+    const unsigned nib = l.tiled ? ael::tile_blocks(l.tg)
+                                 : ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
+    const long nw = (long)nib * wpb * (l.tiled ? l.tg.kc : 1);
     if (l.dwbase) (void)hipFree(l.dwbase);
     if (l.d.D) (void)hipFree(l.d.D);
     if (l.f.D) (void)hipFree(l.f.D);
@@ -472,7 +596,10 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     l.nflag = 0;
     if (nw == 0) return 0;
     HIPCHK(hipMalloc((void**)&l.dwbase, sizeof(int) * nw));
-    ael::k_gradc_count<<<nib, kBlock, 0, s>>>(l.gmask, l.L, l.dwbase);
+    if (l.tiled)
+        ael::k_gradc_count_tiles<TileT><<<nib, kBlock, 0, s>>>(l.gmask, l.L, l.tg, l.dwbase);
+    else
+        ael::k_gradc_count<<<nib, kBlock, 0, s>>>(l.gmask, l.L, l.dwbase);
     HIPCHK(hipGetLastError());
     std::vector<int> cnt(nw);
     HIPCHK(hipMemcpyAsync(cnt.data(), l.dwbase, sizeof(int) * nw, hipMemcpyDeviceToHost, s));
@@ -487,7 +614,10 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     l.nflag = (int)tot;
     if (tot > 0) {
         HIPCHK(hipMalloc((void**)&l.d.D, sizeof(double) * 18 * tot));
-        ael::k_gradc_fill<<<nib, kBlock, 0, s>>>(l.d.C, l.gmask, l.L, l.dwbase, l.nflag, l.d.D);
+        if (l.tiled)
+            ael::k_gradc_fill_tiles<TileT><<<nib, kBlock, 0, s>>>(l.d.C, l.gmask, l.L, l.tg, l.dwbase, l.nflag, l.d.D);
+        else
+            ael::k_gradc_fill<<<nib, kBlock, 0, s>>>(l.d.C, l.gmask, l.L, l.dwbase, l.nflag, l.d.D);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));  // cnt is read by the copy above
@@ -757,6 +887,12 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
         return cleanup(fail(ALAMO_EL_EHIP, "hipHostMalloc failed"));
     for (int q = 0; q < 78; ++q) { h->bct_host[q] = ALAMO_EL_BC_DISPLACEMENT; h->bcv_host[q] = 0.0; }
     for (int a = 0; a < 3; ++a) h->amr_dom[a] = n_cell[a];
+    {
+        int dev = 0, lds = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+            h->lds_max = lds;
+    }
     if (hipMemcpy(h->bct_dev, h->bct_host, sizeof(h->bct_host), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->bcv_dev, h->bcv_host, sizeof(h->bcv_host), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(ALAMO_EL_EHIP, "hipMemcpy of the boundary table failed"));
@@ -1117,10 +1253,10 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         ALLOC(h->kv, (long)(m + 1) * M0);
         ALLOC(h->kz, (long)m * M0);
         ALLOC(h->kw, M0);
-        ALLOC(h->kh, m + 1);
+        ALLOC(h->kh, 2 * (m + 1) + 2);  // two CGS passes of coefficients, |w|^2 after and before
         ALLOC(h->kpart, (long)(m + 1) * kDotBlocks);
         if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
-        HIPCHK(hipHostMalloc((void**)&h->red_host_vec, sizeof(double) * (m + 1)));
+        HIPCHK(hipHostMalloc((void**)&h->red_host_vec, sizeof(double) * (2 * (m + 1) + 2)));
         h->krylov_m = m;
     }
     const bool fp32 = o.precision == ALAMO_EL_PREC_FP32_IR;
@@ -1165,18 +1301,38 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             }
             // w = A Z_j
             launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);
-            // classical Gram-Schmidt, two passes (CGS2): h = V^T w; w -= V h
+            // classical Gram-Schmidt with the DGKS criterion ("twice is
+            // enough"): h = V^T w; w -= V h, and a second pass only when the
+            // first cancelled most of w (|w'| < 0.7 |w|).  Coefficients and
+            // norms stay on the device and come back in one copy per pass.
+            const int nb = (int)std::min<long>(kRedBlocks, std::max<long>(1, nblk(M0)));
+            auto sq_norm = [&](double* dst) {
+                ael::k_reduce_partial<double><<<nb, kBlock, 0, s>>>(h->kw, h->kw, M0, 1, h->red_partial);
+                ael::k_reduce_final<<<1, kBlock, 0, s>>>(h->red_partial, nb, 1, dst);
+            };
+            double* nrm_after = h->kh + 2 * (m + 1);
+            sq_norm(nrm_after + 1);  // |w|^2 before
+            double w2_before = 0.0, hn2 = 0.0;
             for (int i = 0; i <= j; ++i) H[i * m + j] = 0.0;
             for (int pass = 0; pass < 2; ++pass) {
-                ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, j + 1, M0, h->kw, h->kpart);
-                ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h->kh);
-                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, h->kh, -1.0);
-                HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+                double* hp = h->kh + (long)pass * (m + 1);
+                for (int c0 = 0; c0 <= j; c0 += ael::kMultiDotMax) {
+                    const int kc = std::min(ael::kMultiDotMax, j + 1 - c0);
+                    ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv + (long)c0 * M0, kc, M0, h->kw,
+                                                                          h->kpart + (long)c0 * kDotBlocks);
+                }
+                ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, hp);
+                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, hp, -1.0);
+                sq_norm(nrm_after);
+                HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (2 * (m + 1) + 2),
+                                      hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
-                for (int i = 0; i <= j; ++i) H[i * m + j] += h->red_host_vec[i];
+                const double* hv = h->red_host_vec;
+                for (int i = 0; i <= j; ++i) H[i * m + j] += hv[(long)pass * (m + 1) + i];
+                if (pass == 0) w2_before = hv[2 * (m + 1) + 1];
+                hn2 = hv[2 * (m + 1)];
+                if (!(hn2 < 0.49 * w2_before)) break;
             }
-            double hn2;
-            if ((rc = reduce_host<double>(h, h->kw, h->kw, M0, 1, &hn2, s))) return rc;
             const double hn = std::sqrt(hn2);
             H[(j + 1) * m + j] = hn;
             double* Vn = h->kv + (long)(j + 1) * M0;

@@ -530,10 +530,8 @@ __global__ void __launch_bounds__(kBlock) k_diagonal(const T* __restrict__ C, co
 // (src/Operator/Operator.cpp:280-353); the same weights average C between MG
 // levels (src/Operator/Elastic.cpp:786-834).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_restrict(const T* __restrict__ fine, Lvl Lf, T* __restrict__ crse,
-                                                     Lvl Lc, int ncomp) {
-    const long cid = xcd_node();
-    if (cid >= Lc.N) return;
+__device__ __forceinline__ void restrict_node(const T* __restrict__ fine, const Lvl& Lf, T* __restrict__ crse,
+                                              const Lvl& Lc, int ncomp, long cid) {
     int I, J, K;
     node_ijk(cid, Lc, I, J, K);
     const int hx = Lc.nx - 1, hy = Lc.ny - 1, hz = Lc.nz - 1;
@@ -565,6 +563,12 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const T* __restrict__ fine,
         crse[n * Lc.N + cid] = v;
     }
 }
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_restrict(const T* __restrict__ fine, Lvl Lf, T* __restrict__ crse,
+                                                     Lvl Lc, int ncomp) {
+    const long cid = xcd_node();
+    if (cid < Lc.N) restrict_node<T>(fine, Lf, crse, Lc, ncomp, cid);
+}
 
 // Nodal injection crse(I,J,K) = fine(2I,2J,2K): amrex::average_down on nodal
 // data, which MLLinOp::avgDownResMG applies to the residual in the F-cycle.
@@ -582,10 +586,8 @@ __global__ void __launch_bounds__(kBlock) k_inject(const T* __restrict__ fine, L
 // Trilinear prolongation, fine += P crse (src/Operator/Operator.cpp:399-428).
 // With zero_first the fine field is overwritten instead (cor = P cor_crse).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_interp(const T* __restrict__ crse, Lvl Lc, T* __restrict__ fine, Lvl Lf,
-                                                   int ncomp, int zero_first) {
-    const long id = xcd_node();
-    if (id >= Lf.N) return;
+__device__ __forceinline__ void interp_node(const T* __restrict__ crse, const Lvl& Lc, T* __restrict__ fine,
+                                            const Lvl& Lf, int ncomp, int zero_first, long id) {
     int i, j, k;
     node_ijk(id, Lf, i, j, k);
     const int I = i >> 1, J = j >> 1, K = k >> 1;
@@ -608,11 +610,27 @@ __global__ void __launch_bounds__(kBlock) k_interp(const T* __restrict__ crse, L
         *out = zero_first ? v : *out + v;
     }
 }
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_interp(const T* __restrict__ crse, Lvl Lc, T* __restrict__ fine, Lvl Lf,
+                                                   int ncomp, int zero_first) {
+    const long id = xcd_node();
+    if (id < Lf.N) interp_node<T>(crse, Lc, fine, Lf, ncomp, zero_first, id);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fill(T* __restrict__ p, T v, long n) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id < n) p[id] = v;
+}
+
+// The first damped Jacobi sweep from a zero guess, elementwise:
+// (1-w) 0 + w (b - (A 0 - 0 D)) / D = w b / D  (A 0 is zero on every row,
+// boundary rows included), rounded as the full sweep rounds it.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_jacobi_from_zero(const T* __restrict__ b, const T* __restrict__ diag,
+                                                             T omega, T* __restrict__ out, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) out[id] = omega * b[id] / diag[id];
 }
 
 template <typename T>
@@ -933,24 +951,40 @@ __global__ void __launch_bounds__(kBlock)
     if (tid == 0) status[0] = ret;
 }
 
-// h[i] partials of V_i . w for i < k (V: k vectors of length M, stride M).
+// h[i] partials of V_i . w for i < k <= kMultiDotMax (V: k vectors of length
+// M, stride M).  Each thread reads w[q] once and keeps the k running sums in
+// registers, so one pass streams V and w once; the block then reduces each
+// sum through the waves (shuffles) and LDS.
+constexpr int kMultiDotMax = 32;
 __global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __restrict__ V, int k, long M,
                                                              const double* __restrict__ w,
                                                              double* __restrict__ partial) {
-    __shared__ double sh[kBlock];
-    for (int i = 0; i < k; ++i) {
-        const double* Vi = V + (long)i * M;
-        double acc = 0.0;
-        for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < M; q += (long)gridDim.x * blockDim.x)
-            acc += Vi[q] * w[q];
-        sh[threadIdx.x] = acc;
-        __syncthreads();
-        for (int st = kBlock / 2; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-            __syncthreads();
+    __shared__ double sh[kBlock / 64][kMultiDotMax];
+    double acc[kMultiDotMax];
+#pragma unroll
+    for (int i = 0; i < kMultiDotMax; ++i) acc[i] = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < M; q += (long)gridDim.x * blockDim.x) {
+        const double wq = w[q];
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax; ++i)
+            if (i < k) acc[i] += V[(long)i * M + q] * wq;
+    }
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+#pragma unroll
+    for (int i = 0; i < kMultiDotMax; ++i) {
+        if (i < k) {
+            double v = acc[i];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) sh[wave][i] = v;
         }
-        if (threadIdx.x == 0) partial[(long)i * gridDim.x + blockIdx.x] = sh[0];
-        __syncthreads();
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < k) {
+        double t = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < kBlock / 64; ++wv) t += sh[wv][threadIdx.x];
+        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
     }
 }
 

@@ -1,0 +1,319 @@
+// stencil_lds.h -- LDS-staged MajorMinor stencil kernel for the large MG
+// levels.
+//
+// A workgroup owns a TX x TY tile of (i, j) interior columns and marches it
+// over a chunk of kc planes in k, one node per thread per plane.  u lives in
+// an LDS ring of 3 planes (k-1, k, k+1) with a one-node halo; while plane k
+// is computed, each thread holds its share of plane k+2 (PRE values) in
+// registers, so the 19-point neighbourhood costs LDS reads and the global
+// traffic of u is the halo tile once per plane.  C, the compact grad-C
+// matrices D and the OP operands stream from HBM as coalesced rows of TX
+// values.  All global loads of a plane are issued together, ahead of the LDS
+// work (a scheduling barrier keeps the compiler from sinking each load to
+// its use, which would cost one memory round trip per load); the chunk's
+// gradient-mask bytes and compact D bases are fetched once, in the prologue,
+// so that no load of a plane waits on another.  Boundary nodes take the
+// first nbb workgroups, as in k_stencil_slab.
+//
+// Tiles are XCD-slab ordered: XCD x (workgroups b with b % kXcd == x) owns
+// the j-tiles [x*nty/kXcd, (x+1)*nty/kXcd) of every chunk, so the halo rows
+// two tiles share are read through one L2.
+#pragma once
+#include "stencil.h"
+
+namespace ael {
+
+// Tile shape: TX x TY nodes (i x j), one thread per node, whole waves.
+template <int TX_, int TY_>
+struct Tile {
+    static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
+    static constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;  // halo tile of one component
+    static constexpr int PRE = (3 * HP + NT - 1) / NT;            // u values per thread per plane
+    static constexpr int WPB = NT / 64;                           // waves per workgroup
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
+};
+
+struct TileGrid {
+    int ntx, nty, nkc, kc;  // tiles along i and j, k chunks, planes per chunk
+};
+
+// Tile grid of the interior of L: chunks of kc planes, with kc chosen so
+// that the launch has about target workgroups (1 to 16 planes per chunk).
+template <class TS>
+inline TileGrid tile_grid(const Lvl& L, long target) {
+    TileGrid G{0, 0, 0, 1};
+    const int ni = L.nx - 2, nj = L.ny - 2, nk = L.nz - 2;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return G;
+    G.ntx = (ni + TS::TX - 1) / TS::TX;
+    G.nty = (nj + TS::TY - 1) / TS::TY;
+    const long per_plane = (long)G.ntx * G.nty;
+    long kc = (long)nk * per_plane / target;
+    kc = kc < 1 ? 1 : (kc > 16 ? 16 : kc);
+    G.kc = (int)kc;
+    G.nkc = (nk + G.kc - 1) / G.kc;
+    return G;
+}
+// Interior workgroups: kXcd times those of the XCD with the most j-tiles.
+inline unsigned tile_blocks(const TileGrid& G) {
+    if (G.ntx == 0) return 0;
+    const long per = (long)((G.nty + kXcd - 1) / kXcd) * G.ntx * G.nkc;
+    return (unsigned)(per * kXcd);
+}
+
+// Tile (it, jt) and chunk kc of interior workgroup ib; false for the
+// padding workgroups of XCDs with fewer j-tiles (uniform over the workgroup).
+__device__ __forceinline__ bool tile_of(const TileGrid& G, long ib, int& it, int& jt, int& kc) {
+    const int x = (int)(ib % kXcd);
+    const long lb = ib / kXcd;
+    const int t0 = (int)((long)x * G.nty / kXcd), t1 = (int)((long)(x + 1) * G.nty / kXcd);
+    const long per = (long)(t1 - t0) * G.ntx;
+    if (per == 0 || lb >= per * G.nkc) return false;
+    kc = (int)(lb / per);
+    const long r = lb - (long)kc * per;
+    jt = t0 + (int)(r / G.ntx);
+    it = (int)(r - (long)(jt - t0) * G.ntx);
+    return true;
+}
+
+// Flagged lanes per (interior workgroup, plane of its chunk, wave): the
+// compact D order of the tiled launch.  Grid: the interior workgroups.
+template <class TS>
+__global__ void __launch_bounds__(TS::NT) k_gradc_count_tiles(const unsigned char* __restrict__ gmask, Lvl L,
+                                                              TileGrid G, int* __restrict__ wcount) {
+    int it = 0, jt = 0, kc = 0;
+    const long ib = blockIdx.x;
+    const bool t = tile_of(G, ib, it, jt, kc);
+    const int i = 1 + it * TS::TX + (int)threadIdx.x % TS::TX, j = 1 + jt * TS::TY + (int)threadIdx.x / TS::TX;
+    const bool in = t && i <= L.nx - 2 && j <= L.ny - 2;
+    const int k0 = 1 + kc * G.kc;
+    for (int kk = 0; kk < G.kc; ++kk) {
+        const int k = k0 + kk;
+        const bool v = in && k <= L.nz - 2;
+        const long id = ((long)k * L.ny + j) * L.nx + i;
+        const unsigned long long bal = __ballot(v && gmask[id] != 0);
+        if ((threadIdx.x & 63u) == 0)
+            wcount[(ib * G.kc + kk) * TS::WPB + threadIdx.x / 64] = (int)__popcll(bal);
+    }
+}
+
+template <class TS>
+__global__ void __launch_bounds__(TS::NT) k_gradc_fill_tiles(const double* __restrict__ C,
+                                                             const unsigned char* __restrict__ gmask, Lvl L,
+                                                             TileGrid G, const int* __restrict__ dwbase, int nflag,
+                                                             double* __restrict__ Dc) {
+    int it = 0, jt = 0, kc = 0;
+    const long ib = blockIdx.x;
+    const bool t = tile_of(G, ib, it, jt, kc);
+    const int i = 1 + it * TS::TX + (int)threadIdx.x % TS::TX, j = 1 + jt * TS::TY + (int)threadIdx.x / TS::TX;
+    const bool in = t && i <= L.nx - 2 && j <= L.ny - 2;
+    const int k0 = 1 + kc * G.kc;
+    for (int kk = 0; kk < G.kc; ++kk) {
+        const int k = k0 + kk;
+        const bool v = in && k <= L.nz - 2;
+        const long id = ((long)k * L.ny + j) * L.nx + i;
+        const bool f = v && gmask[id] != 0;
+        const unsigned long long bal = __ballot(f);
+        if (f) {
+            const int p = dwbase[(ib * G.kc + kk) * TS::WPB + threadIdx.x / 64] +
+                          (int)__popcll(bal & lanes_below());
+            double D[18];
+            gradc_matrix_mm(C, L, (unsigned)id, D);
+#pragma unroll
+            for (int q = 0; q < 18; ++q) Dc[(size_t)q * nflag + p] = D[q];
+        }
+    }
+}
+
+// One plane of the u halo tile: this thread's PRE values (in range of the
+// box; the others are never read by a node of the tile).
+template <class TS, typename T>
+__device__ __forceinline__ void tile_fetch(const T* __restrict__ u, const Lvl& L, int it, int jt, int k,
+                                           T pre[TS::PRE]) {
+    const unsigned N = (unsigned)L.N;
+    const unsigned plane = (unsigned)k * (unsigned)(L.nx * L.ny);
+#pragma unroll
+    for (int m = 0; m < TS::PRE; ++m) {
+        const int q = (int)threadIdx.x + m * TS::NT;
+        const int c = q / TS::HP, pos = q - c * TS::HP;
+        const int hy = pos / TS::HX, hx = pos - hy * TS::HX;
+        const int gi = it * TS::TX + hx, gj = jt * TS::TY + hy;
+        pre[m] = T(0);
+        if (q < 3 * TS::HP && gi < L.nx && gj < L.ny)
+            pre[m] = ld32(u, (unsigned)c * N + plane + (unsigned)(gj * L.nx + gi));
+    }
+}
+template <class TS, typename T>
+__device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PRE]) {
+#pragma unroll
+    for (int m = 0; m < TS::PRE; ++m) {
+        const int q = (int)threadIdx.x + m * TS::NT;
+        if (q < 3 * TS::HP) s[q] = pre[m];
+    }
+}
+
+template <class TS, typename T, int OP, bool DC>
+__global__ void __launch_bounds__(TS::NT)
+    k_stencil_lds(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
+                  const T* __restrict__ C, const int* __restrict__ bct, int uniform, Lvl L, T omega,
+                  const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G, const int* __restrict__ dwbase,
+                  const T* __restrict__ Dc, int nflag, T* __restrict__ out) {
+    __shared__ T su[3][3 * TS::HP];  // [ring slot][component][hy][hx]
+    if (blockIdx.x < nbb) {
+        const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= boundary_count(L)) return;
+        int i, j, k;
+        boundary_node(L, t, i, j, k);
+        const long id = ((long)k * L.ny + j) * L.nx + i;
+        T nrm[3], o[3];
+        const int reg = region_of<T>(i, j, k, L, nrm);
+        apply_boundary<T, 1>(x, C, bct, L, i, j, k, id, reg, nrm, o);
+        const unsigned n32 = (unsigned)id;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const size_t off = (size_t)p * L.N;
+            const T xv = (OP == 2) ? ld32(x + off, n32) : T(0);
+            const T bv = (OP >= 1) ? ld32(b + off, n32) : T(0);
+            const T dv = (OP == 2) ? ld32(diag + off, n32) : T(1);
+            st32(out + off, n32, finish_op<T, OP>(o[p], xv, bv, dv, omega));
+        }
+        return;
+    }
+    const long ib = (long)(blockIdx.x - nbb);
+    int it, jt, kc;
+    if (!tile_of(G, ib, it, jt, kc)) return;  // whole workgroup, before any barrier
+    constexpr int kHX = TS::HX, kHP = TS::HP;
+    const int tx = (int)threadIdx.x % TS::TX, ty = (int)threadIdx.x / TS::TX;
+    const int i = 1 + it * TS::TX + tx, j = 1 + jt * TS::TY + ty;
+    const bool valid = i <= L.nx - 2 && j <= L.ny - 2;
+    const int k0 = 1 + kc * G.kc, k1 = min(k0 + G.kc, L.nz - 1);  // planes [k0, k1)
+    const unsigned N = (unsigned)L.N, sxy = (unsigned)(L.nx * L.ny);
+    const unsigned col = (unsigned)(j * L.nx + i);
+
+    // prologue: planes k0-1, k0, k0+1 into ring slots 0, 1, 2; the chunk's
+    // gradient-mask bytes (packed, plane kk in byte kk) and, in lane kk of
+    // each wave, the first compact D position of plane kk
+    {
+        T pre[TS::PRE];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            tile_fetch<TS, T>(x, L, it, jt, k0 - 1 + s, pre);
+            tile_store<TS, T>(su[s], pre);
+        }
+    }
+    unsigned long long gpk = 0;
+    int dwb_lane = 0;
+    if constexpr (DC) {
+        if (valid) {
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                if (kk < G.kc && k0 + kk < k1)
+                    gpk |= (unsigned long long)gmask[(unsigned)(k0 + kk) * sxy + col] << (8 * kk);
+        }
+        const int lane = (int)(threadIdx.x & 63u);
+        if (lane < G.kc) dwb_lane = dwbase[(ib * G.kc + lane) * TS::WPB + threadIdx.x / 64];
+    }
+    __syncthreads();
+
+    const T ihx = T(1) / T(L.dx), ihy = T(1) / T(L.dy), ihz = T(1) / T(L.dz);
+    const T hx2 = T(0.5) * ihx, hy2 = T(0.5) * ihy, hz2 = T(0.5) * ihz;
+    const T ixx = ihx * ihx, iyy = ihy * ihy, izz = ihz * ihz;
+    const T ixy = T(0.25) * ihx * ihy, ixz = T(0.25) * ihx * ihz, iyz = T(0.25) * ihy * ihz;
+    const int c0 = (ty + 1) * kHX + (tx + 1);  // centre position in the halo tile
+
+    for (int k = k0; k < k1; ++k) {
+        const int kk = k - k0;
+        const int sm = kk % 3, sc = (kk + 1) % 3, sp = (kk + 2) % 3;  // slots of k-1, k, k+1
+        const bool more = k + 1 < k1;
+        const unsigned o = (unsigned)k * sxy + col;
+
+        // every global load of the plane leaves here, before the LDS work
+        T cv[21], dv[18], pre[TS::PRE], bv[3], gv[3];
+        if (valid) {
+#pragma unroll
+            for (int s = 0; s < 21; ++s) cv[s] = ld32(C + (size_t)s * N, o);
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                bv[p] = (OP >= 1) ? ld32(b + (size_t)p * N, o) : T(0);
+                gv[p] = (OP == 2) ? ld32(diag + (size_t)p * N, o) : T(1);
+            }
+        }
+        bool flagged = false;
+        if constexpr (DC) {
+            const unsigned gm = (unsigned)(gpk & 0xffu);
+            gpk >>= 8;
+            flagged = !uniform && gm != 0u;
+            const unsigned long long bal = __ballot(flagged);
+            const int base = __builtin_amdgcn_readlane(dwb_lane, kk);
+            if (flagged) {
+                const unsigned pidx = (unsigned)base + (unsigned)__popcll(bal & lanes_below());
+#pragma unroll
+                for (int q = 0; q < 18; ++q) dv[q] = ld32(Dc + (size_t)q * nflag, pidx);
+            }
+        }
+        if (more) tile_fetch<TS, T>(x, L, it, jt, k + 2, pre);
+        __builtin_amdgcn_sched_barrier(0);
+
+        if (valid) {
+            // symmetric gradient pairs e1 and second derivative combinations e (LDS)
+            T gu[3][3], B[3][3][3];
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                const T* um = su[sm] + p * kHP;
+                const T* uc = su[sc] + p * kHP;
+                const T* up = su[sp] + p * kHP;
+                const T cc = uc[c0];
+                const T xp = uc[c0 + 1], xm = uc[c0 - 1], yp = uc[c0 + kHX], ym = uc[c0 - kHX];
+                const T zp = up[c0], zm = um[c0];
+                gu[p][0] = (xp - xm) * hx2;
+                gu[p][1] = (yp - ym) * hy2;
+                gu[p][2] = (zp - zm) * hz2;
+                B[p][0][0] = (xp - T(2) * cc + xm) * ixx;
+                B[p][1][1] = (yp - T(2) * cc + ym) * iyy;
+                B[p][2][2] = (zp - T(2) * cc + zm) * izz;
+                B[p][0][1] = (uc[c0 + 1 + kHX] + uc[c0 - 1 - kHX] - uc[c0 + 1 - kHX] - uc[c0 - 1 + kHX]) * ixy;
+                B[p][0][2] = (up[c0 + 1] + um[c0 - 1] - um[c0 + 1] - up[c0 - 1]) * ixz;
+                B[p][1][2] = (up[c0 + kHX] + um[c0 - kHX] - um[c0 + kHX] - up[c0 - kHX]) * iyz;
+                B[p][1][0] = B[p][0][1];
+                B[p][2][0] = B[p][0][2];
+                B[p][2][1] = B[p][1][2];
+            }
+            T e[3][6], e1[6];
+#pragma unroll
+            for (int Q = 0; Q < 6; ++Q) {
+                const int qa = pair_a(Q), qb = pair_b(Q);
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) e[jj][Q] = qa == qb ? B[qa][qa][jj] : B[qa][qb][jj] + B[qb][qa][jj];
+                e1[Q] = qa == qb ? gu[qa][qa] : gu[qa][qb] + gu[qb][qa];
+            }
+            // C : grad grad u  (src/Set/Matrix4_MajorMinor.H:709-767)
+            T outv[3] = {T(0), T(0), T(0)};
+#pragma unroll
+            for (int s = 0; s < 21; ++s) {
+                const int P = slot_p(s), Q = slot_q(s);
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) {
+                    if (pair_other(P, jj) >= 0) outv[pair_other(P, jj)] += cv[s] * e[jj][Q];
+                    if (Q != P && pair_other(Q, jj) >= 0) outv[pair_other(Q, jj)] += cv[s] * e[jj][P];
+                }
+            }
+            if (DC && flagged) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int Q = 0; Q < 6; ++Q) outv[r] += dv[6 * r + Q] * e1[Q];
+            }
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                const T xv = (OP == 2) ? su[sc][p * kHP + c0] : T(0);
+                st32(out + (size_t)p * N, o, finish_op<T, OP>(outv[p], xv, bv[p], gv[p], omega));
+            }
+        }
+        if (more) {
+            __syncthreads();  // every read of slot sm (plane k-1) is done
+            tile_store<TS, T>(su[sm], pre);
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace ael

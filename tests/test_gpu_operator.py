@@ -105,6 +105,53 @@ def test_fsmooth_matches_oracle(sym):
     assert relerr(host(xd), ref) < REL
 
 
+def grain_C(shape, rng, ngrains=12):
+    """Piecewise-constant MajorMinor field: nearest of ngrains random seeds,
+    each grain its own rotated cubic stiffness (gradient mask set only near
+    grain boundaries)."""
+    nz, ny, nx = shape
+    seeds = rng.random((ngrains, 3)) * np.array([nx, ny, nz])
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    d = [(x - sx) ** 2 + (y - sy) ** 2 + (z - sz) ** 2 for sx, sy, sz in seeds]
+    gid = np.argmin(np.stack(d), axis=0)
+    R = M.haar_rotations(ngrains, int(rng.integers(1 << 30)))
+    table = np.stack([M.cubic(1.7, 1.2, 0.75, R[g]) for g in range(ngrains)])
+    return M.field_from_table(gid, table)
+
+
+# Levels with at least 32 interior nodes along every axis run the LDS-tiled
+# MajorMinor kernel (tiles of 32 x 8 columns, chunks of planes in k): a box
+# whose interior is 35 x 33 x 39 has partial tiles along i, j and k.
+@pytest.mark.parametrize("field", ["random", "grains", "uniform"])
+def test_tiled_level_matches_oracle(field):
+    el = _el()
+    rng = np.random.default_rng({"random": 11, "grains": 12, "uniform": 13}[field])
+    n_cell = (36, 34, 40)
+    shape = (41, 35, 37)
+    dx = (1.0 / 36, 0.9 / 34, 1.1 / 40)
+    sym = M.MAJORMINOR
+    C = grain_C(shape, rng) if field != "random" else rand_C(sym, shape, rng)
+    uniform = field == "uniform"
+    if uniform:
+        C = np.ascontiguousarray(np.broadcast_to(C[:, :1, :1, :1], C.shape))
+    bc = rand_bc(rng)
+    u = rng.standard_normal((3,) + shape)
+    b = rng.standard_normal((3,) + shape)
+    op = el.ElasticOperator(n_cell, dx, sym)
+    op.SetBC(bc)
+    op.SetUniform(uniform)
+    op.SetModel(dev(C))
+    f = host(op.Fapply(dev(u)))
+    assert relerr(f, O.fapply(u, C, sym, bc, dx, uniform=uniform)) < REL
+    r = host(op.residual(dev(u), dev(b)))
+    assert relerr(r, O.residual(u, b, C, sym, bc, dx, uniform=uniform)) < REL
+    op.prepare()
+    xd = dev(u)
+    op.Fsmooth(xd, dev(b), nsmooth=2)
+    diag = O.diagonal(C, sym, bc, dx, shape)
+    assert relerr(host(xd), O.fsmooth(u, b, C, sym, bc, dx, diag, nsmooth=2, uniform=uniform)) < REL
+
+
 def test_transfers_and_coefficient_averaging_match_oracle():
     el = _el()
     rng = np.random.default_rng(3)

@@ -73,8 +73,11 @@ def test_inclusion_fp64_matches_oracle(cycle, tol, bar):
     assert rel_l2(host(x), xo) < bar
 
 
-def test_polycrystal_fp64_matches_oracle():
-    p = P.polycrystal(16, ngrains=8, seed=3)
+# 16 cells: every level runs the per-node slab kernel; 40 cells: the finest
+# level (39^3 interior nodes) runs the LDS-tiled kernel.
+@pytest.mark.parametrize("n_cell", [16, 40])
+def test_polycrystal_fp64_matches_oracle(n_cell):
+    p = P.polycrystal(n_cell, ngrains=8, seed=3)
     el, op, rhs = gpu_setup(p)
     rhs_o = oracle_rhs(p)
     x = torch.zeros_like(rhs)
@@ -85,9 +88,28 @@ def test_polycrystal_fp64_matches_oracle():
     assert rel_l2(host(x), xo) < 1e-10
 
 
-@pytest.mark.parametrize("precision,outer", [("fp32", "mlmg"), ("fp64", "fgmres"), ("fp32", "fgmres")])
-def test_variants_reach_the_fp64_residual(precision, outer):
-    p = P.polycrystal(32, ngrains=16, seed=1)
+# With the dense direct bottom, the levels of at most 1024 nodes (9^3, 5^3
+# and the 3^3 bottom here) run as one workgroup (k_vcycle_tail).  Three FP64
+# V-cycles against the oracle's MLMG whose BiCGStab bottom is run to 1e-14 on
+# the same 81 unknowns (a direct solve to rounding): the two agree to 1e-10.
+@pytest.mark.parametrize("n_cell", [16, 32])
+def test_direct_bottom_tail_cycles_match_oracle(n_cell):
+    p = P.polycrystal(n_cell, ngrains=8, seed=4)
+    el, op, rhs = gpu_setup(p)
+    rhs_o = oracle_rhs(p)
+    x = torch.zeros_like(rhs)
+    s = el.Linear(op, cycle="V", bottom_solver="direct", max_iter=3, abort_on_fail=False)
+    s.solve(x, rhs, 1e-14, 0.0)
+    xo, info = O.mlmg_solve(rhs_o, p["C"], p["sym"], p["bc"], p["dx"], tol_rel=1e-14, max_iter=3, cycle="V",
+                            bottom_tol_rel=1e-14, bottom_max_iter=2000)
+    assert s.info.iters == 3 and info["iters"] == 3
+    assert rel_l2(host(x), xo) < 1e-10
+
+
+@pytest.mark.parametrize("precision,outer,n_cell", [("fp32", "mlmg", 32), ("fp64", "fgmres", 32),
+                                                    ("fp32", "fgmres", 32), ("fp32", "fgmres", 40)])
+def test_variants_reach_the_fp64_residual(precision, outer, n_cell):
+    p = P.polycrystal(n_cell, ngrains=16, seed=1)
     el, op, rhs = gpu_setup(p)
     x64 = torch.zeros_like(rhs)
     el.Linear(op, outer="fgmres").solve(x64, rhs, 1e-11, 0.0)

@@ -133,7 +133,7 @@ struct alamo_el_s {
 namespace {
 
 constexpr int kRedBlocks = 1024;
-constexpr int kDotBlocks = 512;
+constexpr int kDotBlocks = 2048;  // multi-dot partials: 8 waves per SIMD over the 256 CUs
 constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M doubles
 
 // ---------------------------------------------------------------------------

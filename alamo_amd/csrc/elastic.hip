@@ -28,7 +28,6 @@
 #include "learned_kernels.h"
 #include "stencil.h"
 #include "stencil_lds.h"
-#include "coarse_cycle.h"
 
 using ael::kBlock;
 using ael::Lvl;
@@ -92,7 +91,6 @@ struct alamo_el_s {
     double bcv_host[78];
     int uniform = 0;
     double omega = 2.0 / 3.0;
-    long lds_max = 0;  // LDS bytes one workgroup may use on this device
     bool model_set = false, prepared = false, have32 = false, prepared32 = false;
     double* red_partial = nullptr;
     double* red_out = nullptr;
@@ -107,7 +105,16 @@ struct alamo_el_s {
     int dense_M = 0;
     double* dense_I = nullptr;     // identity, the unit vectors of the assembly
     double* dense_A = nullptr;     // operator, column-major
-    double* dense_Ainv = nullptr;  // inverse, column-major (null when singular)
+    // direct bottom (ALAMO_EL_BOTTOM_DIRECT): the finest MG level with at
+    // most kDirectMax unknowns and the FP64 inverse of its operator
+    int direct_level = -1;
+    int direct_M = 0;
+    double* direct_I = nullptr;     // unit vectors of the assembly
+    double* direct_A = nullptr;     // operator, destroyed by the inversion
+    double* direct_Ainv = nullptr;  // inverse, column-major (null when singular)
+    double* direct_f = nullptr;     // Gauss-Jordan multipliers
+    double* direct_part = nullptr;  // partial sums of the bottom product
+    ael::GjState* direct_st = nullptr;
     // FGMRES storage, allocated on demand
     double* kh = nullptr;          // Gram-Schmidt coefficients on the device
     double* kpart = nullptr;       // multi-dot partials
@@ -135,6 +142,7 @@ namespace {
 constexpr int kRedBlocks = 1024;
 constexpr int kDotBlocks = 2048;  // multi-dot partials: 8 waves per SIMD over the 256 CUs
 constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M doubles
+constexpr int kDirectMax = 4096; // unknowns of the direct bottom level (an M x M FP64 inverse)
 
 // ---------------------------------------------------------------------------
 // Kernel dispatch
@@ -303,15 +311,23 @@ struct Cycle {
     void interp(int lf, T* fine, const T* crse, int zero_first) {
         ael::k_interp<T><<<nblk8(L(lf).L.N), kBlock, 0, s>>>(crse, L(lf + 1).L, fine, L(lf).L, 3, zero_first);
     }
+    // Bottom level of the cycles: the direct level when its inverse is used,
+    // else the coarsest level.
+    bool direct() const { return bottom_solver == ALAMO_EL_BOTTOM_DIRECT && h->direct_Ainv; }
+    int bot() const { return direct() ? h->direct_level : (int)h->lv.size() - 1; }
     void bottom() {
+        if (direct()) {
+            // exact FP64 solve with the inverse of the direct level's operator
+            const int l = bot();
+            const int Mb = (int)M(l);
+            const dim3 grid(nblk(Mb), (Mb + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk);
+            ael::k_dense_mv_partial<T><<<grid, kBlock, 0, s>>>(h->direct_Ainv, V(l).res, Mb, h->direct_part);
+            ael::k_dense_mv_final<T><<<nblk(Mb), kBlock, 0, s>>>(h->direct_part, (int)grid.y, Mb, V(l).cor);
+            return;
+        }
         const int l = nlev() - 1;
         Level& lv = L(l);
         const int Mb = (int)M(l);
-        if (bottom_solver == ALAMO_EL_BOTTOM_DIRECT && h->dense_Ainv) {
-            // exact FP64 solve with the inverse of the coarsest operator
-            ael::k_bottom_direct<T><<<1, kBlock, sizeof(double) * Mb, s>>>(h->dense_Ainv, V(l).res, V(l).cor, Mb);
-            return;
-        }
         double* x = lv.d.cor;
         const double* b = lv.d.res;
         if (!std::is_same<T, double>::value) launch_copy_cast(lv);
@@ -350,100 +366,17 @@ struct Cycle {
         ael::k_cast<float, double><<<nblk(m), kBlock, 0, s>>>(lv.f.res, lv.d.res, m);
     }
 
-    // LDS bytes of the tail over levels [top, bot]: coefficients, diagonal,
-    // gradient mask and the four work vectors of each smoothing level, the
-    // residual and correction of the bottom.
-    long tail_bytes(int top) {
-        const int bot = nlev() - 1;
-        long elems = 0, masks = 0;
-        for (int l = top; l <= bot; ++l) {
-            const long N = L(l).L.N;
-            elems += l < bot ? (long)h->nc * N + 5 * 3 * N : 2 * 3 * N;
-            masks += l < bot ? N : 0;
-        }
-        return elems * (long)sizeof(T) + masks;
-    }
-    // First MG level of the one-workgroup tail (k_vcycle_tail): the levels of
-    // at most kTailNodes nodes above a dense direct bottom whose operands fit
-    // in LDS; nlev() when the tail is not used.
-    int tail_top() {
-        const int bot = nlev() - 1;
-        if (bottom_solver != ALAMO_EL_BOTTOM_DIRECT || !h->dense_Ainv || bot < 1 || h->lds_max <= 0) return nlev();
-        int f = bot;
-        while (f > 0 && L(f - 1).L.N <= ael::kTailNodes && bot - (f - 1) + 1 <= ael::kTailMaxLevels &&
-               tail_bytes(f - 1) <= h->lds_max)
-            --f;
-        return f < bot ? f : nlev();
-    }
-    // mgVcycle from level top to the bottom and back, one launch.
-    void tail(int top) {
-        const int bot = nlev() - 1;
-        ael::TailArgs<T> a{};
-        a.n = bot - top + 1;
-        a.nc = h->nc;
-        const bool masked = stencil_variant() != kVariantNode && stencil_variant() != kVariantSlabNoMask;
-        long e = 0;  // LDS layout: T data of every level, then the masks
-        for (int l = top; l <= bot; ++l) {
-            Vecs<T>& v = V(l);
-            ael::TailLevel<T>& t = a.lv[l - top];
-            const long N = L(l).L.N;
-            t.L = L(l).L;
-            t.cor = v.cor;
-            t.res = v.res;
-            t.diag = v.diag;
-            t.C = v.C;
-            t.gmask = masked && l < bot ? L(l).gmask : nullptr;
-            if (l < bot) {
-                t.oC = (int)e; e += (long)h->nc * N;
-                t.odiag = (int)e; e += 3 * N;
-                t.otmp = (int)e; e += 3 * N;
-                t.orescor = (int)e; e += 3 * N;
-            }
-            t.ores = (int)e; e += 3 * N;
-            t.ocor = (int)e; e += 3 * N;
-        }
-        long gb = e * (long)sizeof(T);
-        for (int l = top; l < bot; ++l) {
-            a.lv[l - top].ogm = (int)gb;
-            gb += L(l).L.N;
-        }
-        a.nu1 = nu1;
-        a.nu2 = nu2;
-        a.omega = (T)h->omega;
-        a.uniform = h->uniform;
-        a.bct = h->bct_dev;
-        a.Ainv = h->dense_Ainv;
-        const size_t bytes = (size_t)gb;
-        auto go = [&](auto kern) {
-            (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-            kern<<<1, ael::kTailBlock, bytes, s>>>(a);
-        };
-        switch (h->sym) {
-            case 0: go(ael::k_vcycle_tail<T, 0>); break;
-            case 1: go(ael::k_vcycle_tail<T, 1>); break;
-            default: go(ael::k_vcycle_tail<T, 2>); break;
-        }
-    }
-
-    // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.  The
-    // levels from tail_top() down run as one k_vcycle_tail launch.
+    // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.
     void vcycle(int top) {
-        const int bot = nlev() - 1;
-        const int f = tail_top();
-        if (top >= f && top < bot) {
-            tail(top);
-            return;
-        }
-        const int stop = std::min(f, bot);  // levels [top, stop) launch their own kernels
-        for (int l = top; l < stop; ++l) {
+        const int bot = this->bot();
+        for (int l = top; l < bot; ++l) {
             Vecs<T>& v = V(l);
             smooth_from_zero(l, v.cor, v.res, nu1);
             corr_res(l, v.rescor, v.cor, v.res);
             restrict_res(l, v.rescor, V(l + 1).res);
         }
-        if (f < bot) tail(f);
-        else bottom();
-        for (int l = stop - 1; l >= top; --l) {
+        bottom();
+        for (int l = bot - 1; l >= top; --l) {
             Vecs<T>& v = V(l);
             interp(l, v.cor, V(l + 1).cor, 0);
             smooth(l, v.cor, v.res, nu2);
@@ -453,7 +386,7 @@ struct Cycle {
     // amrex::MLMG::mgFcycle.  The residual goes down by nodal injection
     // (MLLinOp::avgDownResMG -> amrex::average_down on nodal data).
     void fcycle() {
-        const int bot = nlev() - 1;
+        const int bot = this->bot();
         for (int l = 1; l <= bot; ++l)
             ael::k_inject<T><<<nblk(L(l).L.N), kBlock, 0, s>>>(V(l - 1).res, L(l - 1).L, V(l).res, L(l).L, 3);
         bottom();
@@ -504,48 +437,18 @@ int ensure32(alamo_el_t h) {
 
 // Gauss-Jordan inverse with partial pivoting of a column-major M x M matrix;
 // false when a pivot vanishes.
-bool invert_dense(const std::vector<double>& A, int M, std::vector<double>& inv) {
-    std::vector<double> a(A);  // column-major: a[c*M + r]
-    inv.assign((size_t)M * M, 0.0);
-    for (int q = 0; q < M; ++q) inv[(size_t)q * M + q] = 1.0;
-    for (int c = 0; c < M; ++c) {
-        int piv = c;
-        double best = std::fabs(a[(size_t)c * M + c]);
-        for (int r = c + 1; r < M; ++r)
-            if (std::fabs(a[(size_t)c * M + r]) > best) { best = std::fabs(a[(size_t)c * M + r]); piv = r; }
-        if (!(best > 0.0)) return false;
-        if (piv != c)
-            for (int k = 0; k < M; ++k) {
-                std::swap(a[(size_t)k * M + c], a[(size_t)k * M + piv]);
-                std::swap(inv[(size_t)k * M + c], inv[(size_t)k * M + piv]);
-            }
-        const double d = a[(size_t)c * M + c];
-        for (int k = 0; k < M; ++k) { a[(size_t)k * M + c] /= d; inv[(size_t)k * M + c] /= d; }
-        for (int r = 0; r < M; ++r) {
-            if (r == c) continue;
-            const double f = a[(size_t)c * M + r];
-            if (f == 0.0) continue;
-            for (int k = 0; k < M; ++k) {
-                a[(size_t)k * M + r] -= f * a[(size_t)k * M + c];
-                inv[(size_t)k * M + r] -= f * inv[(size_t)k * M + c];
-            }
-        }
-    }
-    return true;
-}
-
-// Dense coarsest-level operator (A e_q for every unit vector) and its inverse.
+// Dense coarsest-level operator (A e_q for every unit vector), for the dense
+// BiCGStab bottom.
 int assemble_dense(alamo_el_t h, hipStream_t s) {
     Level& lb = h->lv.back();
     const int M = (int)(3 * lb.L.N);
     if (M > kDenseMax) return 0;
     if (h->dense_M != M) {
-        for (double* p : {h->dense_I, h->dense_A, h->dense_Ainv})
+        for (double* p : {h->dense_I, h->dense_A})
             if (p) (void)hipFree(p);
-        h->dense_I = h->dense_A = h->dense_Ainv = nullptr;
+        h->dense_I = h->dense_A = nullptr;
         ALLOC(h->dense_I, (long)M * M);
         ALLOC(h->dense_A, (long)M * M);
-        ALLOC(h->dense_Ainv, (long)M * M);
         std::vector<double> I((size_t)M * M, 0.0);
         for (int q = 0; q < M; ++q) I[(size_t)q * M + q] = 1.0;
         HIPCHK(hipMemcpy(h->dense_I, I.data(), sizeof(double) * M * M, hipMemcpyHostToDevice));
@@ -558,14 +461,75 @@ int assemble_dense(alamo_el_t h, hipStream_t s) {
         default: ael::k_assemble_dense<2><<<grid, kBlock, 0, s>>>(h->dense_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->dense_A); break;
     }
     HIPCHK(hipGetLastError());
-    std::vector<double> A((size_t)M * M), inv;
-    HIPCHK(hipMemcpyAsync(A.data(), h->dense_A, sizeof(double) * M * M, hipMemcpyDeviceToHost, s));
+    return 0;
+}
+
+// The direct bottom level: the finest MG level with at most kDirectMax
+// unknowns (-1 when even the coarsest has more).
+int direct_level_of(alamo_el_t h) {
+    int l = (int)h->lv.size() - 1;
+    if (3 * h->lv[l].L.N > kDirectMax) return -1;
+    while (l > 0 && 3 * h->lv[l - 1].L.N <= kDirectMax) --l;
+    return l;
+}
+
+void free_direct(alamo_el_t h) {
+    for (void* p : {(void*)h->direct_I, (void*)h->direct_A, (void*)h->direct_Ainv, (void*)h->direct_f,
+                    (void*)h->direct_part, (void*)h->direct_st})
+        if (p) (void)hipFree(p);
+    h->direct_I = h->direct_A = h->direct_Ainv = h->direct_f = h->direct_part = nullptr;
+    h->direct_st = nullptr;
+    h->direct_M = 0;
+}
+
+// Operator of the direct level (A e_q for every unit vector) and its FP64
+// inverse by Gauss-Jordan with partial pivoting on the device.  A singular
+// operator leaves direct_Ainv null (the direct bottom is then unavailable).
+int assemble_direct(alamo_el_t h, hipStream_t s) {
+    const int l = direct_level_of(h);
+    h->direct_level = l;
+    if (l < 0) {
+        free_direct(h);
+        return 0;
+    }
+    Level& lb = h->lv[l];
+    const int M = (int)(3 * lb.L.N);
+    if (h->direct_M != M || !h->direct_Ainv) {
+        free_direct(h);
+        ALLOC(h->direct_I, (long)M * M);
+        ALLOC(h->direct_A, (long)M * M);
+        ALLOC(h->direct_Ainv, (long)M * M);
+        ALLOC(h->direct_f, M);
+        ALLOC(h->direct_part, (long)((M + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk) * M);
+        ALLOC(h->direct_st, 1);
+        std::vector<double> I((size_t)M * M, 0.0);
+        for (int q = 0; q < M; ++q) I[(size_t)q * M + q] = 1.0;
+        HIPCHK(hipMemcpy(h->direct_I, I.data(), sizeof(double) * M * M, hipMemcpyHostToDevice));
+        h->direct_M = M;
+    }
+    dim3 grid(nblk(lb.L.N), M);
+    switch (h->sym) {
+        case 0: ael::k_assemble_dense<0><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
+        case 1: ael::k_assemble_dense<1><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
+        default: ael::k_assemble_dense<2><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->direct_Ainv, h->direct_I, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
+    const ael::GjState zero{0, 0, 0.0};
+    HIPCHK(hipMemcpyAsync(h->direct_st, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    for (int c = 0; c < M; ++c) {
+        ael::k_gj_pivot<<<1, ael::kGjBlock, 0, s>>>(h->direct_A, M, c, h->direct_f, h->direct_st);
+        ael::k_gj_swap_scale<<<nblk(2L * M), kBlock, 0, s>>>(h->direct_A, h->direct_Ainv, M, c, h->direct_st);
+        ael::k_gj_eliminate<<<dim3(nblk(M), 2 * M - c), kBlock, 0, s>>>(h->direct_A, h->direct_Ainv, M, c,
+                                                                        h->direct_f);
+    }
+    HIPCHK(hipGetLastError());
+    ael::GjState st{};
+    HIPCHK(hipMemcpyAsync(&st, h->direct_st, sizeof(st), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (invert_dense(A, M, inv)) {
-        HIPCHK(hipMemcpy(h->dense_Ainv, inv.data(), sizeof(double) * M * M, hipMemcpyHostToDevice));
-    } else {
-        (void)hipFree(h->dense_Ainv);
-        h->dense_Ainv = nullptr;
+    if (st.singular) {
+        (void)hipFree(h->direct_Ainv);
+        h->direct_Ainv = nullptr;
     }
     return 0;
 }
@@ -648,6 +612,8 @@ int prepare(alamo_el_t h, hipStream_t s) {
     for (auto& l : h->lv) launch_diagonal<double>(h, l, l.d.C, l.d.diag, s);
     HIPCHK(hipGetLastError());
     int rc = assemble_dense(h, s);
+    if (rc) return rc;
+    rc = assemble_direct(h, s);
     if (rc) return rc;
     h->prepared = true;
     h->prepared32 = false;
@@ -887,12 +853,6 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
         return cleanup(fail(ALAMO_EL_EHIP, "hipHostMalloc failed"));
     for (int q = 0; q < 78; ++q) { h->bct_host[q] = ALAMO_EL_BC_DISPLACEMENT; h->bcv_host[q] = 0.0; }
     for (int a = 0; a < 3; ++a) h->amr_dom[a] = n_cell[a];
-    {
-        int dev = 0, lds = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
-            h->lds_max = lds;
-    }
     if (hipMemcpy(h->bct_dev, h->bct_host, sizeof(h->bct_host), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->bcv_dev, h->bcv_host, sizeof(h->bcv_host), hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(ALAMO_EL_EHIP, "hipMemcpy of the boundary table failed"));
@@ -910,10 +870,11 @@ int alamo_el_destroy(alamo_el_t h) {
         if (l.gmask) (void)hipFree(l.gmask);
         if (l.dwbase) (void)hipFree(l.dwbase);
     }
+    free_direct(h);
     for (void* p : {(void*)h->bct_dev, (void*)h->bcv_dev, (void*)h->red_partial, (void*)h->red_out,
                     (void*)h->bottom_work, (void*)h->bottom_status, (void*)h->gu9, (void*)h->dw9, (void*)h->csoa,
                     (void*)h->kv, (void*)h->kz, (void*)h->kw, (void*)h->kh, (void*)h->kpart,
-                    (void*)h->dense_I, (void*)h->dense_A, (void*)h->dense_Ainv, (void*)h->nn_in, (void*)h->nn_out,
+                    (void*)h->dense_I, (void*)h->dense_A, (void*)h->nn_in, (void*)h->nn_out,
                     (void*)h->nn_bc, (void*)h->nn_t, (void*)h->nn_fmean})
         if (p) (void)hipFree(p);
     if (h->red_host) (void)hipHostFree(h->red_host);

@@ -812,19 +812,100 @@ __global__ void __launch_bounds__(kBlock) k_assemble_dense(const double* __restr
     for (int c = 0; c < 3; ++c) Acol[q * M + c * L.N + id] = o[c];
 }
 
-// x = Ainv * rhs with the inverse of the coarsest-level operator (column-major),
-// in FP64; rhs and x may be FP32 (the FP32 cycles) or FP64.
+// Direct bottom solve: x = Ainv rhs with the FP64 inverse (column-major,
+// M x M) of the operator of the direct bottom level; rhs and x may be FP32
+// (the FP32 cycles) or FP64.  Column chunks of kDenseMvChunk run in parallel
+// workgroups (partial sums), one more launch folds them.
+constexpr int kDenseMvChunk = 64;
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_bottom_direct(const double* __restrict__ Ainv, const T* __restrict__ rhs,
-                                                          T* __restrict__ x, int M) {
-    extern __shared__ double sb[];
-    for (int q = threadIdx.x; q < M; q += blockDim.x) sb[q] = (double)rhs[q];
-    __syncthreads();
-    for (int row = threadIdx.x; row < M; row += blockDim.x) {
-        double acc = 0.0;
-        for (int c = 0; c < M; ++c) acc += Ainv[(long)c * M + row] * sb[c];
-        x[row] = (T)acc;
+__global__ void __launch_bounds__(kBlock) k_dense_mv_partial(const double* __restrict__ A, const T* __restrict__ x,
+                                                             int M, double* __restrict__ part) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= M) return;
+    const int c0 = (int)blockIdx.y * kDenseMvChunk, c1 = min(M, c0 + kDenseMvChunk);
+    double acc = 0.0;
+    for (int c = c0; c < c1; ++c) acc += A[(long)c * M + r] * (double)x[c];
+    part[(long)blockIdx.y * M + r] = acc;
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_dense_mv_final(const double* __restrict__ part, int nch, int M,
+                                                           T* __restrict__ y) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= M) return;
+    double acc = 0.0;
+    for (int q = 0; q < nch; ++q) acc += part[(long)q * M + r];
+    y[r] = (T)acc;
+}
+
+// Gauss-Jordan inversion with partial pivoting on the device, one pivot
+// column c per step (three launches): the inverse of an M x M column-major
+// matrix a is built in inv (set to the identity first); a is destroyed.
+struct GjState {
+    int piv;      // pivot row of the current step
+    int singular; // set when a pivot is zero
+    double d;     // pivot value
+};
+constexpr int kGjBlock = 1024;
+// Pivot row (largest |a(r, c)|, r >= c, first on ties) and the multipliers
+// f(r) = a(r, c) of the column after the row swap.
+__global__ void __launch_bounds__(kGjBlock) k_gj_pivot(const double* __restrict__ a, int M, int c,
+                                                       double* __restrict__ f, GjState* __restrict__ st) {
+    __shared__ double bv[kGjBlock];
+    __shared__ int bi[kGjBlock];
+    __shared__ int piv_s;
+    const int t = (int)threadIdx.x;
+    double best = -1.0;
+    int idx = c;
+    for (int r = c + t; r < M; r += kGjBlock) {
+        const double v = fabs(a[(long)c * M + r]);
+        if (v > best) { best = v; idx = r; }
     }
+    bv[t] = best;
+    bi[t] = idx;
+    __syncthreads();
+    for (int w = kGjBlock / 2; w > 0; w >>= 1) {
+        if (t < w && (bv[t + w] > bv[t] || (bv[t + w] == bv[t] && bi[t + w] < bi[t]))) {
+            bv[t] = bv[t + w];
+            bi[t] = bi[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        piv_s = bi[0];
+        st->piv = bi[0];
+        st->d = a[(long)c * M + bi[0]];
+        if (!(bv[0] > 0.0)) st->singular = 1;
+    }
+    __syncthreads();
+    const int piv = piv_s;
+    for (int r = t; r < M; r += kGjBlock) {
+        const int src = r == c ? piv : (r == piv ? c : r);
+        f[r] = a[(long)c * M + src];
+    }
+}
+// Rows c and piv exchanged in a and inv, row c divided by the pivot.
+__global__ void __launch_bounds__(kBlock) k_gj_swap_scale(double* __restrict__ a, double* __restrict__ inv, int M,
+                                                          int c, const GjState* __restrict__ st) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2L * M) return;
+    double* m = t < M ? a : inv;
+    const long k = t < M ? t : t - M;
+    const int piv = st->piv;
+    const double d = st->d;
+    double* col = m + k * M;
+    const double vc = col[c], vp = col[piv];
+    col[piv] = vc;
+    col[c] = vp / d;
+}
+// a(r, k) -= f(r) a(c, k) for every row r != c (columns k >= c of a, which
+// left of c are unit columns already, every column of inv).
+__global__ void __launch_bounds__(kBlock) k_gj_eliminate(double* __restrict__ a, double* __restrict__ inv, int M,
+                                                         int c, const double* __restrict__ f) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const long kk = blockIdx.y;  // [0, M - c): column c + kk of a; [M - c, 2M - c): column of inv
+    if (r >= M || r == c) return;
+    double* col = kk < M - c ? a + (long)(c + kk) * M : inv + (kk - (M - c)) * M;
+    col[r] -= f[r] * col[c];
 }
 
 // Block-wide max (op 0) or sum (op 1) for kBlock threads; all threads get it.

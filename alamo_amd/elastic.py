@@ -351,7 +351,8 @@ class Linear:
     reference's solver.* inputs (Linear::Parse, Linear.H:296-382) plus
     ``cycle`` ("F" = setMaxFmgIter(1e8), "V"), ``precision`` ("fp64",
     "fp32"), ``outer`` ("mlmg", "fgmres") and ``bottom_solver`` ("bicgstab",
-    the reference's; "direct", the FP64 inverse of the coarsest operator) and
+    the reference's; "direct": the cycles stop at the finest MG level with at
+    most 4096 unknowns and solve there with the FP64 inverse of its operator) and
     ``learned`` ("", "guess", "guess+precond": the learned coarse-level solver
     attached with ElasticOperator.set_learned)."""
 

@@ -282,7 +282,7 @@ def main():
                    "n_cell": args.n_cell, "nodes": nodes, "tol_rel": args.tol,
                    "path": ("learned judged initial correction (U-Net, BF16 MFMA, on the 33^3 level) + " if learned
                             else "") + f"FP32 F-cycle preconditioned FGMRES({FAST_RESTART}), FP64 residual, Krylov basis "
-                                       "and direct coarsest solve"},
+                                       "and direct FP64 solve on the 9^3 level"},
         "learned_fp32": head if learned else None,
         "fp32_only": fp32_only,
         "plain_fp64": plain,

@@ -69,7 +69,11 @@ extern "C" {
 #define ALAMO_EL_OUTER_FGMRES 1   /* flexible GMRES, one cycle as preconditioner */
 
 #define ALAMO_EL_BOTTOM_BICGSTAB 0 /* the reference bottom solver (Linear.H:258) */
-#define ALAMO_EL_BOTTOM_DIRECT 1   /* FP64 inverse of the coarsest operator */
+/* The cycles stop at the finest MG level with at most 4096 unknowns and solve
+ * there with the FP64 inverse of its operator (computed on the device by
+ * alamo_el_prepare); no counterpart in the reference, whose bottom is always
+ * the coarsest level. */
+#define ALAMO_EL_BOTTOM_DIRECT 1
 
 /* Learned coarse correction (alamo_el_set_learned): the network proposes
  * e = P^L N(R^L r) for the residual r, smoothed by 2 Fsmooth calls; the step

@@ -88,12 +88,14 @@ def test_polycrystal_fp64_matches_oracle(n_cell):
     assert rel_l2(host(x), xo) < 1e-10
 
 
-# With the dense direct bottom, the levels of at most 1024 nodes (9^3, 5^3
-# and the 3^3 bottom here) run as one workgroup (k_vcycle_tail).  Three FP64
-# V-cycles against the oracle's MLMG whose BiCGStab bottom is run to 1e-14 on
-# the same 81 unknowns (a direct solve to rounding): the two agree to 1e-10.
-@pytest.mark.parametrize("n_cell", [16, 32])
-def test_direct_bottom_tail_cycles_match_oracle(n_cell):
+# The direct bottom solves exactly, with the FP64 inverse of the operator of
+# the finest MG level with at most 4096 unknowns (9^3 nodes here), inverted on
+# the device by Gauss-Jordan.  Three FP64 V-cycles against the oracle's MLMG
+# coarsened to the same level (max_coarsening_level) with its BiCGStab bottom
+# run to 1e-14 on those 2187 unknowns (a direct solve to rounding): the two
+# agree to 1e-10.
+@pytest.mark.parametrize("n_cell,coarsenings", [(16, 1), (32, 2)])
+def test_direct_bottom_cycles_match_oracle(n_cell, coarsenings):
     p = P.polycrystal(n_cell, ngrains=8, seed=4)
     el, op, rhs = gpu_setup(p)
     rhs_o = oracle_rhs(p)
@@ -101,7 +103,7 @@ def test_direct_bottom_tail_cycles_match_oracle(n_cell):
     s = el.Linear(op, cycle="V", bottom_solver="direct", max_iter=3, abort_on_fail=False)
     s.solve(x, rhs, 1e-14, 0.0)
     xo, info = O.mlmg_solve(rhs_o, p["C"], p["sym"], p["bc"], p["dx"], tol_rel=1e-14, max_iter=3, cycle="V",
-                            bottom_tol_rel=1e-14, bottom_max_iter=2000)
+                            bottom_tol_rel=1e-14, bottom_max_iter=4000, max_coarsening_level=coarsenings)
     assert s.info.iters == 3 and info["iters"] == 3
     assert rel_l2(host(x), xo) < 1e-10
 

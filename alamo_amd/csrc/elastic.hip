@@ -112,6 +112,7 @@ struct alamo_el_s {
     double* direct_I = nullptr;     // unit vectors of the assembly
     double* direct_A = nullptr;     // operator, destroyed by the inversion
     double* direct_Ainv = nullptr;  // inverse, column-major (null when singular)
+    float* direct_Ainv32 = nullptr; // the inverse rounded to FP32, for the FP32 cycles
     double* direct_f = nullptr;     // Gauss-Jordan multipliers
     double* direct_part = nullptr;  // partial sums of the bottom product
     ael::GjState* direct_st = nullptr;
@@ -321,7 +322,12 @@ struct Cycle {
             const int l = bot();
             const int Mb = (int)M(l);
             const dim3 grid(nblk(Mb), (Mb + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk);
-            ael::k_dense_mv_partial<T><<<grid, kBlock, 0, s>>>(h->direct_Ainv, V(l).res, Mb, h->direct_part);
+            if (std::is_same<T, float>::value && h->direct_Ainv32)
+                ael::k_dense_mv_partial<T, float><<<grid, kBlock, 0, s>>>(h->direct_Ainv32, V(l).res, Mb,
+                                                                          h->direct_part);
+            else
+                ael::k_dense_mv_partial<T, double><<<grid, kBlock, 0, s>>>(h->direct_Ainv, V(l).res, Mb,
+                                                                           h->direct_part);
             ael::k_dense_mv_final<T><<<nblk(Mb), kBlock, 0, s>>>(h->direct_part, (int)grid.y, Mb, V(l).cor);
             return;
         }
@@ -475,9 +481,10 @@ int direct_level_of(alamo_el_t h) {
 
 void free_direct(alamo_el_t h) {
     for (void* p : {(void*)h->direct_I, (void*)h->direct_A, (void*)h->direct_Ainv, (void*)h->direct_f,
-                    (void*)h->direct_part, (void*)h->direct_st})
+                    (void*)h->direct_part, (void*)h->direct_st, (void*)h->direct_Ainv32})
         if (p) (void)hipFree(p);
     h->direct_I = h->direct_A = h->direct_Ainv = h->direct_f = h->direct_part = nullptr;
+    h->direct_Ainv32 = nullptr;
     h->direct_st = nullptr;
     h->direct_M = 0;
 }
@@ -531,6 +538,8 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
         (void)hipFree(h->direct_Ainv);
         h->direct_Ainv = nullptr;
     }
+    if (h->direct_Ainv32) (void)hipFree(h->direct_Ainv32);
+    h->direct_Ainv32 = nullptr;  // rounded again by prepare32
     return 0;
 }
 
@@ -633,6 +642,12 @@ int prepare32(alamo_el_t h, hipStream_t s) {
             HIPCHK(hipMalloc((void**)&l.f.D, sizeof(float) * 18 * (long)l.nflag));
             ael::k_cast<double, float><<<nblk(18L * l.nflag), kBlock, 0, s>>>(l.d.D, l.f.D, 18L * l.nflag);
         }
+    }
+    if (h->direct_Ainv) {
+        const long MM = (long)h->direct_M * h->direct_M;
+        if (h->direct_Ainv32) (void)hipFree(h->direct_Ainv32);
+        HIPCHK(hipMalloc((void**)&h->direct_Ainv32, sizeof(float) * MM));
+        ael::k_cast<double, float><<<nblk(MM), kBlock, 0, s>>>(h->direct_Ainv, h->direct_Ainv32, MM);
     }
     HIPCHK(hipGetLastError());
     h->prepared32 = true;

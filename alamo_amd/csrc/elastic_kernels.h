@@ -812,19 +812,21 @@ __global__ void __launch_bounds__(kBlock) k_assemble_dense(const double* __restr
     for (int c = 0; c < 3; ++c) Acol[q * M + c * L.N + id] = o[c];
 }
 
-// Direct bottom solve: x = Ainv rhs with the FP64 inverse (column-major,
-// M x M) of the operator of the direct bottom level; rhs and x may be FP32
-// (the FP32 cycles) or FP64.  Column chunks of kDenseMvChunk run in parallel
-// workgroups (partial sums), one more launch folds them.
-constexpr int kDenseMvChunk = 64;
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_dense_mv_partial(const double* __restrict__ A, const T* __restrict__ x,
+// Direct bottom solve: x = Ainv rhs with the inverse (column-major, M x M)
+// of the operator of the direct bottom level, FP64 for the FP64 cycles and
+// rounded to FP32 for the FP32 cycles (TA = T); the sums are in FP64.  Column
+// chunks of kDenseMvChunk run in parallel workgroups (partial sums), one more
+// launch folds them.
+constexpr int kDenseMvChunk = 32;
+template <typename T, typename TA>
+__global__ void __launch_bounds__(kBlock) k_dense_mv_partial(const TA* __restrict__ A, const T* __restrict__ x,
                                                              int M, double* __restrict__ part) {
     const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (r >= M) return;
     const int c0 = (int)blockIdx.y * kDenseMvChunk, c1 = min(M, c0 + kDenseMvChunk);
     double acc = 0.0;
-    for (int c = c0; c < c1; ++c) acc += A[(long)c * M + r] * (double)x[c];
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c) acc += (double)A[(long)c * M + r] * (double)x[c];
     part[(long)blockIdx.y * M + r] = acc;
 }
 template <typename T>
@@ -833,6 +835,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_mv_final(const double* __restr
     const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (r >= M) return;
     double acc = 0.0;
+#pragma unroll 8
     for (int q = 0; q < nch; ++q) acc += part[(long)q * M + r];
     y[r] = (T)acc;
 }

@@ -62,6 +62,8 @@ struct Vecs {
     T* rescor = nullptr;
     T* tmp = nullptr;
     T* D = nullptr;  // compact grad-C matrices, 18 x nflag (MajorMinor)
+    T* Ct = nullptr; // tiled levels: padded interior copy of C (ael::TileC)
+    T* Cb = nullptr; // tiled levels: C of the boundary nodes, in boundary-node order
 };
 
 struct Level {
@@ -195,10 +197,11 @@ void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const 
     const unsigned char* gm = masked ? l.gmask : nullptr;
     Level& lv = const_cast<Level&>(l);
     const bool have_d = l.dwbase && (l.nflag == 0 || vecs<T>(lv).D);
-    if (masked && h->sym == 1 && l.tiled && have_d) {
-        // LDS-tiled kernel; its compact D is in tile order
+    if (masked && h->sym == 1 && l.tiled && have_d && vecs<T>(lv).Ct && vecs<T>(lv).Cb && C == vecs<T>(lv).C) {
+        // LDS-tiled kernel on its own copies of this level's C; its compact D is in tile order
         ael::k_stencil_lds<TileT, T, OP, true><<<nbb + ael::tile_blocks(l.tg), TileT::NT, 0, s>>>(
-            x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase, vecs<T>(lv).D, l.nflag, out);
+            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase, vecs<T>(lv).D,
+            l.nflag, out);
         return;
     }
     // compact grad-C matrices in slab order: MajorMinor, masked variant, built for this level
@@ -561,12 +564,22 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
                                  : ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
     const long nw = (long)nib * wpb * (l.tiled ? l.tg.kc : 1);
     if (l.dwbase) (void)hipFree(l.dwbase);
-    if (l.d.D) (void)hipFree(l.d.D);
-    if (l.f.D) (void)hipFree(l.f.D);
+    for (void* p : {(void*)l.d.D, (void*)l.f.D, (void*)l.d.Ct, (void*)l.d.Cb, (void*)l.f.Ct, (void*)l.f.Cb})
+        if (p) (void)hipFree(p);
     l.dwbase = nullptr;
     l.d.D = nullptr;
     l.f.D = nullptr;
+    l.d.Ct = l.d.Cb = nullptr;
+    l.f.Ct = l.f.Cb = nullptr;
     l.nflag = 0;
+    if (l.tiled) {
+        const long nb = ael::boundary_count(l.L);
+        HIPCHK(hipMalloc((void**)&l.d.Ct, sizeof(double) * h->nc * ael::TileC<double>::plane(l.L)));
+        HIPCHK(hipMalloc((void**)&l.d.Cb, sizeof(double) * h->nc * nb));
+        ael::k_pack_tile_c<double><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.d.Ct);
+        ael::k_pack_bnd_c<double><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.d.Cb);
+        HIPCHK(hipGetLastError());
+    }
     if (nw == 0) return 0;
     HIPCHK(hipMalloc((void**)&l.dwbase, sizeof(int) * nw));
     if (l.tiled)
@@ -637,6 +650,16 @@ int prepare32(alamo_el_t h, hipStream_t s) {
         const long N = l.L.N;
         ael::k_cast<double, float><<<nblk((long)h->nc * N), kBlock, 0, s>>>(l.d.C, l.f.C, (long)h->nc * N);
         ael::k_cast<double, float><<<nblk(3 * N), kBlock, 0, s>>>(l.d.diag, l.f.diag, 3 * N);
+        if (l.d.Ct) {
+            const long nb = ael::boundary_count(l.L);
+            if (l.f.Ct) (void)hipFree(l.f.Ct);
+            if (l.f.Cb) (void)hipFree(l.f.Cb);
+            l.f.Ct = l.f.Cb = nullptr;
+            HIPCHK(hipMalloc((void**)&l.f.Ct, sizeof(float) * h->nc * ael::TileC<float>::plane(l.L)));
+            HIPCHK(hipMalloc((void**)&l.f.Cb, sizeof(float) * h->nc * nb));
+            ael::k_pack_tile_c<float><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.f.Ct);
+            ael::k_pack_bnd_c<float><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.f.Cb);
+        }
         if (l.d.D) {
             if (l.f.D) (void)hipFree(l.f.D);
             HIPCHK(hipMalloc((void**)&l.f.D, sizeof(float) * 18 * (long)l.nflag));
@@ -878,9 +901,9 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
 int alamo_el_destroy(alamo_el_t h) {
     if (!h) return 0;
     for (auto& l : h->lv) {
-        for (double* p : {l.d.C, l.d.diag, l.d.cor, l.d.cor_hold, l.d.res, l.d.rescor, l.d.tmp, l.d.D})
+        for (double* p : {l.d.C, l.d.diag, l.d.cor, l.d.cor_hold, l.d.res, l.d.rescor, l.d.tmp, l.d.D, l.d.Ct, l.d.Cb})
             if (p) (void)hipFree(p);
-        for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp, l.f.D})
+        for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp, l.f.D, l.f.Ct, l.f.Cb})
             if (p) (void)hipFree(p);
         if (l.gmask) (void)hipFree(l.gmask);
         if (l.dwbase) (void)hipFree(l.dwbase);

@@ -384,8 +384,15 @@ __device__ __forceinline__ void contract2_mm_stream(const T* __restrict__ C, lon
 template <typename T, int SYM>
 __device__ __forceinline__ void apply_boundary(const T* __restrict__ u, const T* __restrict__ C,
                                                const int* __restrict__ bct, const Lvl& L, int i, int j, int k,
-                                               long id, int reg, const T nrm[3], T out[3]) {
+                                               long id, int reg, const T nrm[3], T out[3], long cN = -1,
+                                               long cid = -1) {
+    // C slot s of the node at C[s * cN + cid] (a separate layout of C), or at
+    // C[s * N + id] by default
     const long N = L.N, sy = L.nx, sz = (long)L.nx * L.ny;
+    if (cN < 0) {
+        cN = N;
+        cid = id;
+    }
     const T hx = T(L.dx), hy = T(L.dy), hz = T(L.dz);
     T uu[3], gu[3][3], sg[3][3];
 #pragma unroll
@@ -397,12 +404,12 @@ __device__ __forceinline__ void apply_boundary(const T* __restrict__ u, const T*
         gu[p][2] = d1(f, id, sz, k, L.nz, hz);
     }
     if constexpr (SYM == 1) {
-        contract2_mm_stream<T>(C, N, id, gu, sg);
+        contract2_mm_stream<T>(C, cN, cid, gu, sg);
     } else {
         constexpr int NC = NComp<SYM>::value;
         T c[NC];
 #pragma unroll
-        for (int s = 0; s < NC; ++s) c[s] = C[s * N + id];
+        for (int s = 0; s < NC; ++s) c[s] = C[s * cN + cid];
         contract2<T, SYM>(c, gu, sg);
     }
     bc_set<T>(bct + 3 * reg, uu, gu, sg, nrm, out);

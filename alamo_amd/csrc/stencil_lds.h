@@ -15,6 +15,12 @@
 // so that no load of a plane waits on another.  Boundary nodes take the
 // first nbb workgroups, as in k_stencil_slab.
 //
+// The kernel reads C from two copies of its own (TileC): the interior nodes
+// from rows padded so that node i = 1 starts a 128-byte line (a tile row is
+// then whole lines: 2 in FP64, 1 in FP32, and the requests to L2 are full
+// lines), and the boundary nodes from a compact array in the order of the
+// boundary workgroups, so that the two never share a cache line.
+//
 // Tiles are XCD-slab ordered: XCD x (workgroups b with b % kXcd == x) owns
 // the j-tiles [x*nty/kXcd, (x+1)*nty/kXcd) of every chunk, so the halo rows
 // two tiles share are read through one L2.
@@ -36,6 +42,40 @@ struct Tile {
 struct TileGrid {
     int ntx, nty, nkc, kc;  // tiles along i and j, k chunks, planes per chunk
 };
+
+// Padded interior copy of C: slot s of node (i, j, k) at
+// s * NP + (k * ny + j) * P + i + A, with A = E - 1 and P a multiple of E,
+// E = 128 / sizeof(T) (values per 128-byte line).
+template <typename T>
+struct TileC {
+    static constexpr int E = 128 / (int)sizeof(T);
+    static constexpr int A = E - 1;
+    __host__ __device__ static int pitch(const Lvl& L) { return (L.nx + A + E - 1) / E * E; }
+    __host__ __device__ static long plane(const Lvl& L) { return (long)L.nz * L.ny * pitch(L); }
+};
+// Copies of C for the tiled kernel, from the FP64 C (rounded to T).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pack_tile_c(const double* __restrict__ C, Lvl L, int nc,
+                                                        T* __restrict__ Ct) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    const long P = TileC<T>::pitch(L), NP = TileC<T>::plane(L);
+    const long o = ((long)k * L.ny + j) * P + i + TileC<T>::A;
+    for (int s = 0; s < nc; ++s) Ct[s * NP + o] = (T)C[s * L.N + id];
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pack_bnd_c(const double* __restrict__ C, Lvl L, int nc,
+                                                       T* __restrict__ Cb) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nb = boundary_count(L);
+    if (t >= nb) return;
+    int i, j, k;
+    boundary_node(L, t, i, j, k);
+    const long id = ((long)k * L.ny + j) * L.nx + i;
+    for (int s = 0; s < nc; ++s) Cb[s * nb + t] = (T)C[s * L.N + id];
+}
 
 // Tile grid of the interior of L: chunks of kc planes, with kc chosen so
 // that the launch has about target workgroups (1 to 16 planes per chunk).
@@ -154,9 +194,9 @@ __device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PR
 template <class TS, typename T, int OP, bool DC>
 __global__ void __launch_bounds__(TS::NT)
     k_stencil_lds(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
-                  const T* __restrict__ C, const int* __restrict__ bct, int uniform, Lvl L, T omega,
-                  const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G, const int* __restrict__ dwbase,
-                  const T* __restrict__ Dc, int nflag, T* __restrict__ out) {
+                  const T* __restrict__ Ct, const T* __restrict__ Cb, const int* __restrict__ bct, int uniform,
+                  Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
+                  const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out) {
     __shared__ T su[3][3 * TS::HP];  // [ring slot][component][hy][hx]
     if (blockIdx.x < nbb) {
         const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -166,7 +206,7 @@ __global__ void __launch_bounds__(TS::NT)
         const long id = ((long)k * L.ny + j) * L.nx + i;
         T nrm[3], o[3];
         const int reg = region_of<T>(i, j, k, L, nrm);
-        apply_boundary<T, 1>(x, C, bct, L, i, j, k, id, reg, nrm, o);
+        apply_boundary<T, 1>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
         const unsigned n32 = (unsigned)id;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
@@ -188,6 +228,9 @@ __global__ void __launch_bounds__(TS::NT)
     const int k0 = 1 + kc * G.kc, k1 = min(k0 + G.kc, L.nz - 1);  // planes [k0, k1)
     const unsigned N = (unsigned)L.N, sxy = (unsigned)(L.nx * L.ny);
     const unsigned col = (unsigned)(j * L.nx + i);
+    const unsigned cP = (unsigned)TileC<T>::pitch(L);
+    const size_t cNP = (size_t)TileC<T>::plane(L);
+    const unsigned ccol = (unsigned)(j * cP + i + TileC<T>::A);
 
     // prologue: planes k0-1, k0, k0+1 into ring slots 0, 1, 2; the chunk's
     // gradient-mask bytes (packed, plane kk in byte kk) and, in lane kk of
@@ -225,12 +268,13 @@ __global__ void __launch_bounds__(TS::NT)
         const int sm = kk % 3, sc = (kk + 1) % 3, sp = (kk + 2) % 3;  // slots of k-1, k, k+1
         const bool more = k + 1 < k1;
         const unsigned o = (unsigned)k * sxy + col;
+        const unsigned oc = (unsigned)k * ((unsigned)L.ny * cP) + ccol;  // in the padded copy of C
 
         // every global load of the plane leaves here, before the LDS work
         T cv[21], dv[18], pre[TS::PRE], bv[3], gv[3];
         if (valid) {
 #pragma unroll
-            for (int s = 0; s < 21; ++s) cv[s] = ld32(C + (size_t)s * N, o);
+            for (int s = 0; s < 21; ++s) cv[s] = ld32(Ct + (size_t)s * cNP, oc);
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
                 bv[p] = (OP >= 1) ? ld32(b + (size_t)p * N, o) : T(0);

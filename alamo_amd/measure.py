@@ -70,9 +70,11 @@ def config5(fields=4, n_cell=256, weights=None):
         el.Linear(op, cycle="F", outer="fgmres", abort_on_fail=False).solve(ref, b, 1e-10, 0.0)
         row = dict(field=seed, grains=256, nodes=int(np.prod(op.level_shape(0))), network_level=level)
         runs = [("plain_fp64_V", dict(cycle="V")), ("plain_fp64_F", dict(cycle="F")),
-                ("fp32", dict(cycle="F", precision="fp32", outer="fgmres", restart=10, bottom_solver="direct"))]
+                ("fp32", dict(cycle="F", precision="fp32", outer="fgmres", restart=10, bottom_solver="direct", pre_smooth=3,
+                             post_smooth=3))]
         if net is not None:
             runs.append(("learned_fp32", dict(cycle="F", precision="fp32", outer="fgmres", restart=10, bottom_solver="direct",
+                                              pre_smooth=3, post_smooth=3,
                                               learned="guess")))
         for name, kw in runs:
             x = torch.zeros_like(b)

@@ -35,6 +35,7 @@ struct Tile {
     static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
     static constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;  // halo tile of one component
     static constexpr int PRE = (3 * HP + NT - 1) / NT;            // u values per thread per plane
+    static constexpr int RING = PRE * NT;                         // LDS values per ring slot (>= 3 HP)
     static constexpr int WPB = NT / 64;                           // waves per workgroup
     static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
 };
@@ -173,21 +174,22 @@ __device__ __forceinline__ void tile_fetch(const T* __restrict__ u, const Lvl& L
     const unsigned plane = (unsigned)k * (unsigned)(L.nx * L.ny);
 #pragma unroll
     for (int m = 0; m < TS::PRE; ++m) {
-        const int q = (int)threadIdx.x + m * TS::NT;
+        // Every lane loads (no register default, so no wait on the previous
+        // plane's loads): halo positions past the grid or past 3 HP read a
+        // clamped in-grid node, and only nodes outside the interior, whose
+        // results are never stored, read those positions.
+        const int q = min((int)threadIdx.x + m * TS::NT, 3 * TS::HP - 1);
         const int c = q / TS::HP, pos = q - c * TS::HP;
         const int hy = pos / TS::HX, hx = pos - hy * TS::HX;
-        const int gi = it * TS::TX + hx, gj = jt * TS::TY + hy;
-        pre[m] = T(0);
-        if (q < 3 * TS::HP && gi < L.nx && gj < L.ny)
-            pre[m] = ld32(u, (unsigned)c * N + plane + (unsigned)(gj * L.nx + gi));
+        const int gi = min(it * TS::TX + hx, L.nx - 1), gj = min(jt * TS::TY + hy, L.ny - 1);
+        pre[m] = ld32(u, (unsigned)c * N + plane + (unsigned)(gj * L.nx + gi));
     }
 }
 template <class TS, typename T>
 __device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PRE]) {
 #pragma unroll
     for (int m = 0; m < TS::PRE; ++m) {
-        const int q = (int)threadIdx.x + m * TS::NT;
-        if (q < 3 * TS::HP) s[q] = pre[m];
+        s[threadIdx.x + m * TS::NT] = pre[m];  // ring slots hold RING >= 3 HP values
     }
 }
 
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(TS::NT)
                   const T* __restrict__ Ct, const T* __restrict__ Cb, const int* __restrict__ bct, int uniform,
                   Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
                   const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out) {
-    __shared__ T su[3][3 * TS::HP];  // [ring slot][component][hy][hx]
+    __shared__ T su[3][TS::RING];  // [ring slot][component][hy][hx], padded to RING
     if (blockIdx.x < nbb) {
         const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
         if (t >= boundary_count(L)) return;
@@ -255,6 +257,10 @@ __global__ void __launch_bounds__(TS::NT)
         const int lane = (int)(threadIdx.x & 63u);
         if (lane < G.kc) dwb_lane = dwbase[(ib * G.kc + lane) * TS::WPB + threadIdx.x / 64];
     }
+    // Every prologue load has landed (dwb_lane included): without this the
+    // compiler cannot prove dwb_lane complete inside the loop and waits for
+    // all loads in flight before its readlane on every plane.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     const T ihx = T(1) / T(L.dx), ihy = T(1) / T(L.dy), ihz = T(1) / T(L.dz);

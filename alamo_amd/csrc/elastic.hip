@@ -159,10 +159,19 @@ constexpr int kDirectMax = 4096; // unknowns of the direct bottom level (an M x 
 enum { kVariantSlab = 0, kVariantSlabNoMask = 1, kVariantNode = 2, kVariantSlabOnly = 3 };
 
 // Tile shape of the LDS-tiled kernel: 32 x 8 nodes, four waves (1-wave
-// 32 x 2 and 16 x 4 tiles and 64 x 1 rows measured no faster).
+// 32 x 2 and 16 x 4 tiles, 64 x 1 rows, and 64 x 4 and 128 x 2 tiles measured
+// no faster).
 using TileT = ael::Tile<32, 8>;
-// about 8 workgroups per CU
-ael::TileGrid tile_grid_of(const Lvl& L) { return ael::tile_grid<TileT>(L, 2048); }
+constexpr long kNtMinBytes = 64l << 20;
+// Chunks of k planes sized for about 2048 tile workgroups (8 per CU), at
+// most 16 planes.  ALAMO_EL_TILE_TARGET (read when a level's C is set)
+// overrides the workgroup target: the tests use it to reach every chunk
+// length on small boxes.
+ael::TileGrid tile_grid_of(const Lvl& L) {
+    const char* e = std::getenv("ALAMO_EL_TILE_TARGET");
+    const long target = e ? std::atol(e) : 2048;
+    return ael::tile_grid<TileT>(L, target > 0 ? target : 2048);
+}
 
 int stencil_variant() {
     static int v = -1;
@@ -199,9 +208,19 @@ void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const 
     const bool have_d = l.dwbase && (l.nflag == 0 || vecs<T>(lv).D);
     if (masked && h->sym == 1 && l.tiled && have_d && vecs<T>(lv).Ct && vecs<T>(lv).Cb && C == vecs<T>(lv).C) {
         // LDS-tiled kernel on its own copies of this level's C; its compact D is in tile order
-        ael::k_stencil_lds<TileT, T, OP, true><<<nbb + ael::tile_blocks(l.tg), TileT::NT, 0, s>>>(
-            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase, vecs<T>(lv).D,
-            l.nflag, out);
+        // C streamed with the non-temporal hint on levels whose C exceeds
+        // kNtMinBytes: read once per launch, it then leaves L2 to the u halos
+        // (129^3: FP64 Fapply 124 -> 108 us, FP32 sweep 65 -> 62 us).  A smaller
+        // level's C stays in the 256 MB last-level cache between sweeps, which
+        // the hint gives up (65^3 FP32 sweep: 11.9 us without, 12.6 us with).
+        if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
+            ael::k_stencil_lds<TileT, T, OP, true, true><<<nbb + ael::tile_blocks(l.tg), TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase,
+                vecs<T>(lv).D, l.nflag, out);
+        else
+            ael::k_stencil_lds<TileT, T, OP, true, false><<<nbb + ael::tile_blocks(l.tg), TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase,
+                vecs<T>(lv).D, l.nflag, out);
         return;
     }
     // compact grad-C matrices in slab order: MajorMinor, masked variant, built for this level

@@ -242,6 +242,18 @@ template <typename T>
 __device__ __forceinline__ void st32(T* __restrict__ base, unsigned idx, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(idx * (unsigned)sizeof(T))) = v;
 }
+// The same with the non-temporal hint (global_load/store ... nt): for data
+// streamed once per launch, so that it does not displace reused lines (the u
+// halos) from L2.
+template <typename T>
+__device__ __forceinline__ T ld32nt(const T* __restrict__ base, unsigned idx) {
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(idx * (unsigned)sizeof(T))));
+}
+template <typename T>
+__device__ __forceinline__ void st32nt(T* __restrict__ base, unsigned idx, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(idx * (unsigned)sizeof(T))));
+}
 
 // + sum_a ((dC/dx_a) : grad u).col(a)  (src/Operator/Elastic.cpp:245-254) for
 // MajorMinor C at interior node o, for the axes whose bit is set in gmask;

@@ -43,6 +43,9 @@ struct Tile {
 struct TileGrid {
     int ntx, nty, nkc, kc;  // tiles along i and j, k chunks, planes per chunk
 };
+// Longest chunk: the chunk's gradient masks (3 bits each) travel packed 4 bits
+// a plane in one 64-bit register.
+constexpr int kMaxChunk = 16;
 
 // Padded interior copy of C: slot s of node (i, j, k) at
 // s * NP + (k * ny + j) * P + i + A, with A = E - 1 and P a multiple of E,
@@ -89,7 +92,7 @@ inline TileGrid tile_grid(const Lvl& L, long target) {
     G.nty = (nj + TS::TY - 1) / TS::TY;
     const long per_plane = (long)G.ntx * G.nty;
     long kc = (long)nk * per_plane / target;
-    kc = kc < 1 ? 1 : (kc > 16 ? 16 : kc);
+    kc = kc < 1 ? 1 : (kc > kMaxChunk ? kMaxChunk : kc);
     G.kc = (int)kc;
     G.nkc = (nk + G.kc - 1) / G.kc;
     return G;
@@ -193,7 +196,7 @@ __device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PR
     }
 }
 
-template <class TS, typename T, int OP, bool DC>
+template <class TS, typename T, int OP, bool DC, bool NTC>
 __global__ void __launch_bounds__(TS::NT)
     k_stencil_lds(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
                   const T* __restrict__ Ct, const T* __restrict__ Cb, const int* __restrict__ bct, int uniform,
@@ -235,7 +238,7 @@ __global__ void __launch_bounds__(TS::NT)
     const unsigned ccol = (unsigned)(j * cP + i + TileC<T>::A);
 
     // prologue: planes k0-1, k0, k0+1 into ring slots 0, 1, 2; the chunk's
-    // gradient-mask bytes (packed, plane kk in byte kk) and, in lane kk of
+    // gradient masks (packed, plane kk in bits 4 kk .. 4 kk + 3) and, in lane kk of
     // each wave, the first compact D position of plane kk
     {
         T pre[TS::PRE];
@@ -250,9 +253,9 @@ __global__ void __launch_bounds__(TS::NT)
     if constexpr (DC) {
         if (valid) {
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
+            for (int kk = 0; kk < kMaxChunk; ++kk)
                 if (kk < G.kc && k0 + kk < k1)
-                    gpk |= (unsigned long long)gmask[(unsigned)(k0 + kk) * sxy + col] << (8 * kk);
+                    gpk |= (unsigned long long)(gmask[(unsigned)(k0 + kk) * sxy + col] & 0xfu) << (4 * kk);
         }
         const int lane = (int)(threadIdx.x & 63u);
         if (lane < G.kc) dwb_lane = dwbase[(ib * G.kc + lane) * TS::WPB + threadIdx.x / 64];
@@ -280,7 +283,7 @@ __global__ void __launch_bounds__(TS::NT)
         T cv[21], dv[18], pre[TS::PRE], bv[3], gv[3];
         if (valid) {
 #pragma unroll
-            for (int s = 0; s < 21; ++s) cv[s] = ld32(Ct + (size_t)s * cNP, oc);
+            for (int s = 0; s < 21; ++s) cv[s] = NTC ? ld32nt(Ct + (size_t)s * cNP, oc) : ld32(Ct + (size_t)s * cNP, oc);
 #pragma unroll
             for (int p = 0; p < 3; ++p) {
                 bv[p] = (OP >= 1) ? ld32(b + (size_t)p * N, o) : T(0);
@@ -289,8 +292,8 @@ __global__ void __launch_bounds__(TS::NT)
         }
         bool flagged = false;
         if constexpr (DC) {
-            const unsigned gm = (unsigned)(gpk & 0xffu);
-            gpk >>= 8;
+            const unsigned gm = (unsigned)(gpk & 0xfu);
+            gpk >>= 4;
             flagged = !uniform && gm != 0u;
             const unsigned long long bal = __ballot(flagged);
             const int base = __builtin_amdgcn_readlane(dwb_lane, kk);

@@ -124,8 +124,23 @@ def grain_C(shape, rng, ngrains=12):
 # whose interior is 35 x 33 x 39 has partial tiles along i, j and k.
 @pytest.mark.parametrize("field", ["random", "grains", "uniform"])
 def test_tiled_level_matches_oracle(field):
+    _tiled_case(field, {"random": 11, "grains": 12, "uniform": 13}[field])
+
+
+# The k chunks are sized for ALAMO_EL_TILE_TARGET tile workgroups (default
+# 2048, read when C is set).  The 35 x 33 x 39 interior has 10 tiles a plane:
+# targets 1, 40 and 130 give chunks of 16 (the longest), 9 and 3 planes, so
+# that every position of a chunk's packed gradient masks is used.
+@pytest.mark.parametrize("target", [1, 40, 130])
+@pytest.mark.parametrize("field", ["random", "grains"])
+def test_tiled_chunk_lengths_match_oracle(field, target, monkeypatch):
+    monkeypatch.setenv("ALAMO_EL_TILE_TARGET", str(target))
+    _tiled_case(field, 20 + target)
+
+
+def _tiled_case(field, seed):
     el = _el()
-    rng = np.random.default_rng({"random": 11, "grains": 12, "uniform": 13}[field])
+    rng = np.random.default_rng(seed)
     n_cell = (36, 34, 40)
     shape = (41, 35, 37)
     dx = (1.0 / 36, 0.9 / 34, 1.1 / 40)

@@ -74,7 +74,7 @@ struct Level {
     int* dwbase = nullptr;           // per interior wave of the launch: first compact D position
     int nflag = 0;                   // interior nodes with a nonzero gmask
     bool tiled = false;              // MajorMinor level served by k_stencil_lds (else k_stencil_slab)
-    ael::TileGrid tg{};              // its tile grid
+    ael::TileGrid tg{}, tg32{};      // its tile grids (chunk lengths) of the FP64 and FP32 launches
 };
 
 template <typename T> Vecs<T>& vecs(Level& l);
@@ -163,14 +163,18 @@ enum { kVariantSlab = 0, kVariantSlabNoMask = 1, kVariantNode = 2, kVariantSlabO
 // no faster).
 using TileT = ael::Tile<32, 8>;
 constexpr long kNtMinBytes = 64l << 20;
-// Chunks of k planes sized for about 2048 tile workgroups (8 per CU), at
-// most 16 planes.  ALAMO_EL_TILE_TARGET (read when a level's C is set)
-// overrides the workgroup target: the tests use it to reach every chunk
-// length on small boxes.
-ael::TileGrid tile_grid_of(const Lvl& L) {
+// Chunks of k planes sized so that the tile workgroups fill about two
+// rounds of the resident ones: 2 workgroups per CU in FP64 (200-220 VGPRs),
+// 4 in FP32 (110-120 VGPRs).  At 129^3: FP64 chunks of 8 planes (1024
+// workgroups; Fapply 104.4 us vs 108.4 with 3 planes, 107.0 with 4, 108.8
+// with 16), FP32 chunks of 4.  ALAMO_EL_TILE_TARGET (read when a level's C
+// is set) replaces both targets: the tests use it to reach every chunk length
+// on small boxes.
+constexpr long kTileTarget64 = 2 * 2 * 256, kTileTarget32 = 2 * 4 * 256;
+ael::TileGrid tile_grid_of(const Lvl& L, long target) {
     const char* e = std::getenv("ALAMO_EL_TILE_TARGET");
-    const long target = e ? std::atol(e) : 2048;
-    return ael::tile_grid<TileT>(L, target > 0 ? target : 2048);
+    const long t = e ? std::atol(e) : 0;
+    return ael::tile_grid<TileT>(L, t > 0 ? t : target);
 }
 
 int stencil_variant() {
@@ -213,13 +217,14 @@ void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const 
         // (129^3: FP64 Fapply 124 -> 108 us, FP32 sweep 65 -> 62 us).  A smaller
         // level's C stays in the 256 MB last-level cache between sweeps, which
         // the hint gives up (65^3 FP32 sweep: 11.9 us without, 12.6 us with).
+        const ael::TileGrid& G = sizeof(T) == 8 ? l.tg : l.tg32;
         if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
-            ael::k_stencil_lds<TileT, T, OP, true, true><<<nbb + ael::tile_blocks(l.tg), TileT::NT, 0, s>>>(
-                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase,
+            ael::k_stencil_lds<TileT, T, OP, true, true><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
                 vecs<T>(lv).D, l.nflag, out);
         else
-            ael::k_stencil_lds<TileT, T, OP, true, false><<<nbb + ael::tile_blocks(l.tg), TileT::NT, 0, s>>>(
-                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, l.tg, l.dwbase,
+            ael::k_stencil_lds<TileT, T, OP, true, false><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
                 vecs<T>(lv).D, l.nflag, out);
         return;
     }
@@ -577,11 +582,14 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     l.tiled = stencil_variant() == kVariantSlab && l.L.nx - 2 >= kTiledMin && l.L.ny - 2 >= kTiledMin &&
               l.L.nz - 2 >= kTiledMin;
     static_assert(TileT::NT == kBlock, "tiled and slab launches share the block size");
-    l.tg = tile_grid_of(l.L);
+    l.tg = tile_grid_of(l.L, kTileTarget64);
+    l.tg32 = tile_grid_of(l.L, kTileTarget32);
     const int wpb = ael::kWavesPerBlock;
-    const unsigned nib = l.tiled ? ael::tile_blocks(l.tg)
+    // count/fill workgroups: one per (tile, plane) on tiled levels (the compact
+    // D order of both tile grids), else the slab launch's interior ones
+    const unsigned nib = l.tiled ? (unsigned)(ael::tile_wave_planes(l.L, l.tg, wpb) / wpb)
                                  : ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
-    const long nw = (long)nib * wpb * (l.tiled ? l.tg.kc : 1);
+    const long nw = (long)nib * wpb;
     if (l.dwbase) (void)hipFree(l.dwbase);
     for (void* p : {(void*)l.d.D, (void*)l.f.D, (void*)l.d.Ct, (void*)l.d.Cb, (void*)l.f.Ct, (void*)l.f.Cb})
         if (p) (void)hipFree(p);

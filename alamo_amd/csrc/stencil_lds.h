@@ -91,7 +91,7 @@ inline TileGrid tile_grid(const Lvl& L, long target) {
     G.ntx = (ni + TS::TX - 1) / TS::TX;
     G.nty = (nj + TS::TY - 1) / TS::TY;
     const long per_plane = (long)G.ntx * G.nty;
-    long kc = (long)nk * per_plane / target;
+    long kc = ((long)nk * per_plane + target / 2) / target;  // nearest
     kc = kc < 1 ? 1 : (kc > kMaxChunk ? kMaxChunk : kc);
     G.kc = (int)kc;
     G.nkc = (nk + G.kc - 1) / G.kc;
@@ -119,25 +119,27 @@ __device__ __forceinline__ bool tile_of(const TileGrid& G, long ib, int& it, int
     return true;
 }
 
-// Flagged lanes per (interior workgroup, plane of its chunk, wave): the
-// compact D order of the tiled launch.  Grid: the interior workgroups.
+// Compact D order of a tiled level: (plane k, j-tile, i-tile, wave), the
+// same for every chunk length, so that launches that group planes into
+// chunks differently (FP64 and FP32) share one D.  Index of the wave-plane:
+__host__ __device__ inline long tile_wave_plane(const TileGrid& G, int k, int it, int jt, int wave, int wpb) {
+    return (((long)(k - 1) * G.nty + jt) * G.ntx + it) * wpb + wave;
+}
+// Wave-planes of a tiled level.
+inline long tile_wave_planes(const Lvl& L, const TileGrid& G, int wpb) { return (long)(L.nz - 2) * G.nty * G.ntx * wpb; }
+
+// Flagged lanes per wave-plane.  Grid: one workgroup per (tile, plane), in
+// wave-plane order.
 template <class TS>
 __global__ void __launch_bounds__(TS::NT) k_gradc_count_tiles(const unsigned char* __restrict__ gmask, Lvl L,
                                                               TileGrid G, int* __restrict__ wcount) {
-    int it = 0, jt = 0, kc = 0;
-    const long ib = blockIdx.x;
-    const bool t = tile_of(G, ib, it, jt, kc);
+    const long tp = blockIdx.x;  // ((k - 1) * nty + jt) * ntx + it
+    const int it = (int)(tp % G.ntx), jt = (int)(tp / G.ntx % G.nty), k = 1 + (int)(tp / ((long)G.ntx * G.nty));
     const int i = 1 + it * TS::TX + (int)threadIdx.x % TS::TX, j = 1 + jt * TS::TY + (int)threadIdx.x / TS::TX;
-    const bool in = t && i <= L.nx - 2 && j <= L.ny - 2;
-    const int k0 = 1 + kc * G.kc;
-    for (int kk = 0; kk < G.kc; ++kk) {
-        const int k = k0 + kk;
-        const bool v = in && k <= L.nz - 2;
-        const long id = ((long)k * L.ny + j) * L.nx + i;
-        const unsigned long long bal = __ballot(v && gmask[id] != 0);
-        if ((threadIdx.x & 63u) == 0)
-            wcount[(ib * G.kc + kk) * TS::WPB + threadIdx.x / 64] = (int)__popcll(bal);
-    }
+    const bool v = i <= L.nx - 2 && j <= L.ny - 2;
+    const long id = ((long)k * L.ny + j) * L.nx + i;
+    const unsigned long long bal = __ballot(v && gmask[id] != 0);
+    if ((threadIdx.x & 63u) == 0) wcount[tp * TS::WPB + threadIdx.x / 64] = (int)__popcll(bal);
 }
 
 template <class TS>
@@ -145,26 +147,19 @@ __global__ void __launch_bounds__(TS::NT) k_gradc_fill_tiles(const double* __res
                                                              const unsigned char* __restrict__ gmask, Lvl L,
                                                              TileGrid G, const int* __restrict__ dwbase, int nflag,
                                                              double* __restrict__ Dc) {
-    int it = 0, jt = 0, kc = 0;
-    const long ib = blockIdx.x;
-    const bool t = tile_of(G, ib, it, jt, kc);
+    const long tp = blockIdx.x;
+    const int it = (int)(tp % G.ntx), jt = (int)(tp / G.ntx % G.nty), k = 1 + (int)(tp / ((long)G.ntx * G.nty));
     const int i = 1 + it * TS::TX + (int)threadIdx.x % TS::TX, j = 1 + jt * TS::TY + (int)threadIdx.x / TS::TX;
-    const bool in = t && i <= L.nx - 2 && j <= L.ny - 2;
-    const int k0 = 1 + kc * G.kc;
-    for (int kk = 0; kk < G.kc; ++kk) {
-        const int k = k0 + kk;
-        const bool v = in && k <= L.nz - 2;
-        const long id = ((long)k * L.ny + j) * L.nx + i;
-        const bool f = v && gmask[id] != 0;
-        const unsigned long long bal = __ballot(f);
-        if (f) {
-            const int p = dwbase[(ib * G.kc + kk) * TS::WPB + threadIdx.x / 64] +
-                          (int)__popcll(bal & lanes_below());
-            double D[18];
-            gradc_matrix_mm(C, L, (unsigned)id, D);
+    const bool v = i <= L.nx - 2 && j <= L.ny - 2;
+    const long id = ((long)k * L.ny + j) * L.nx + i;
+    const bool f = v && gmask[id] != 0;
+    const unsigned long long bal = __ballot(f);
+    if (f) {
+        const int p = dwbase[tp * TS::WPB + threadIdx.x / 64] + (int)__popcll(bal & lanes_below());
+        double D[18];
+        gradc_matrix_mm(C, L, (unsigned)id, D);
 #pragma unroll
-            for (int q = 0; q < 18; ++q) Dc[(size_t)q * nflag + p] = D[q];
-        }
+        for (int q = 0; q < 18; ++q) Dc[(size_t)q * nflag + p] = D[q];
     }
 }
 
@@ -258,7 +253,8 @@ __global__ void __launch_bounds__(TS::NT)
                     gpk |= (unsigned long long)(gmask[(unsigned)(k0 + kk) * sxy + col] & 0xfu) << (4 * kk);
         }
         const int lane = (int)(threadIdx.x & 63u);
-        if (lane < G.kc) dwb_lane = dwbase[(ib * G.kc + lane) * TS::WPB + threadIdx.x / 64];
+        if (lane < G.kc && k0 + lane < k1)
+            dwb_lane = dwbase[tile_wave_plane(G, k0 + lane, it, jt, (int)(threadIdx.x / 64), TS::WPB)];
     }
     // Every prologue load has landed (dwb_lane included): without this the
     // compiler cannot prove dwb_lane complete inside the loop and waits for

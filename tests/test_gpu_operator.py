@@ -127,11 +127,12 @@ def test_tiled_level_matches_oracle(field):
     _tiled_case(field, {"random": 11, "grains": 12, "uniform": 13}[field])
 
 
-# The k chunks are sized for ALAMO_EL_TILE_TARGET tile workgroups (default
-# 2048, read when C is set).  The 35 x 33 x 39 interior has 10 tiles a plane:
-# targets 1, 40 and 130 give chunks of 16 (the longest), 9 and 3 planes, so
-# that every position of a chunk's packed gradient masks is used.
-@pytest.mark.parametrize("target", [1, 40, 130])
+# The k chunks are sized for ALAMO_EL_TILE_TARGET tile workgroups (read when
+# C is set; by default about two rounds of resident workgroups, a different
+# chunk length in FP64 and FP32).  The 35 x 33 x 39 interior has 10 tiles a
+# plane: targets 1, 43 and 130 give chunks of 16 (the longest), 9 and 3
+# planes, so that every position of a chunk's packed gradient masks is used.
+@pytest.mark.parametrize("target", [1, 43, 130])
 @pytest.mark.parametrize("field", ["random", "grains"])
 def test_tiled_chunk_lengths_match_oracle(field, target, monkeypatch):
     monkeypatch.setenv("ALAMO_EL_TILE_TARGET", str(target))

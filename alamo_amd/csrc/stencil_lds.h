@@ -12,10 +12,10 @@
 // work (a scheduling barrier keeps the compiler from sinking each load to
 // its use, which would cost one memory round trip per load); the chunk's
 // gradient masks and compact D bases are fetched once, in the prologue,
-// so that no load of a plane waits on another.  Boundary nodes take the last
-// nbb workgroups: dispatched after the tiles, their latency-bound serial path
-// overlaps the tail of the tile workgroups instead of holding most of the
-// resident slots during the first round (Fapply at 129^3: 107.5 -> 101.1 us).
+// so that no load of a plane waits on another.  Boundary nodes take nbb
+// workgroups of their own, after the tiles in FP64 and before them in FP32
+// (see the kernel).  Jacobi sweeps compute an interior node's diagonal from
+// its C instead of reading diag.
 //
 // The kernel reads C from two copies of its own (TileC): the interior nodes
 // from rows padded so that node i = 1 starts a 128-byte line (a tile row is
@@ -200,9 +200,18 @@ __global__ void __launch_bounds__(TS::NT)
                   Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
                   const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out) {
     __shared__ T su[3][TS::RING];  // [ring slot][component][hy][hx], padded to RING
-    const unsigned ntb = gridDim.x - nbb;  // tile workgroups, then nbb boundary ones
-    if (blockIdx.x >= ntb) {
-        const long t = (long)(blockIdx.x - ntb) * blockDim.x + threadIdx.x;
+    // The nbb boundary workgroups (a multiple of kXcd, so that tile ib keeps
+    // ib % kXcd == XCD) come after the tile ones in FP64 and before them in
+    // FP32.  FP64 runs 2 workgroups per CU and 2 rounds of tiles: the latency-
+    // bound boundary workgroups fill the slots the second round frees (Fapply
+    // at 129^3: 101.5 us, vs 106.3 first and 110.4 spread among the tiles).
+    // FP32 runs 4 per CU: placed last they form a trailing round of their own
+    // (65^3 sweep 12.9 us vs 11.9 first; 129^3 residual 63.5 vs 61.8 us).
+    constexpr bool kBoundaryLast = sizeof(T) == 8;
+    const unsigned ntb = gridDim.x - nbb;  // tile workgroups
+    const bool bnd = kBoundaryLast ? blockIdx.x >= ntb : blockIdx.x < nbb;
+    if (bnd) {
+        const long t = (long)(kBoundaryLast ? blockIdx.x - ntb : blockIdx.x) * blockDim.x + threadIdx.x;
         if (t >= boundary_count(L)) return;
         int i, j, k;
         boundary_node(L, t, i, j, k);
@@ -221,7 +230,7 @@ __global__ void __launch_bounds__(TS::NT)
         }
         return;
     }
-    const long ib = (long)blockIdx.x;
+    const long ib = (long)(kBoundaryLast ? blockIdx.x : blockIdx.x - nbb);
     int it, jt, kc;
     if (!tile_of(G, ib, it, jt, kc)) return;  // whole workgroup, before any barrier
     constexpr int kHX = TS::HX, kHP = TS::HP;
@@ -284,10 +293,7 @@ __global__ void __launch_bounds__(TS::NT)
 #pragma unroll
             for (int s = 0; s < 21; ++s) cv[s] = NTC ? ld32nt(Ct + (size_t)s * cNP, oc) : ld32(Ct + (size_t)s * cNP, oc);
 #pragma unroll
-            for (int p = 0; p < 3; ++p) {
-                bv[p] = (OP >= 1) ? ld32(b + (size_t)p * N, o) : T(0);
-                gv[p] = (OP == 2) ? ld32(diag + (size_t)p * N, o) : T(1);
-            }
+            for (int p = 0; p < 3; ++p) bv[p] = (OP >= 1) ? ld32(b + (size_t)p * N, o) : T(0);
         }
         bool flagged = false;
         if constexpr (DC) {
@@ -339,6 +345,22 @@ __global__ void __launch_bounds__(TS::NT)
             }
             // C : grad grad u  (src/Set/Matrix4_MajorMinor.H:709-767)
             T outv[3] = {T(0), T(0), T(0)};
+            if constexpr (OP == 2) {
+                // Diagonal of an interior row, from the C already in registers
+                // instead of a load: (C : Gradient_Diagonal)_p = -2 sum_l C_plpl / h_l^2
+                // (src/Operator/Elastic.cpp:321-374, src/Numeric/Stencil.H:900-963).
+                const T m[3] = {T(-2) * ixx, T(-2) * iyy, T(-2) * izz};
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    T d = T(0);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) d += cv[tri6(vpair(p, l), vpair(p, l))] * m[l];
+                    gv[p] = d;
+                }
+            } else {
+#pragma unroll
+                for (int p = 0; p < 3; ++p) gv[p] = T(1);
+            }
 #pragma unroll
             for (int s = 0; s < 21; ++s) {
                 const int P = slot_p(s), Q = slot_q(s);

@@ -1090,6 +1090,19 @@ int alamo_el_bc_init(alamo_el_t h, double* rhs, int homogeneous, hipStream_t s) 
     return 0;
 }
 
+int alamo_el_mix_models(alamo_el_t h, const double* eta, int nmodels, const double* table_host, int ncomp,
+                        double* out, hipStream_t s) {
+    if (!h || !eta || !table_host || !out || nmodels < 1 || ncomp < 1)
+        return fail(ALAMO_EL_EINVAL, "bad argument");
+    if ((long)nmodels * ncomp > ael::kMixMax) return fail(ALAMO_EL_EINVAL, "nmodels * ncomp exceeds 256");
+    const Lvl L = h->lv[0].L;
+    ael::MixTable tab{};
+    for (int q = 0; q < nmodels * ncomp; ++q) tab.v[q] = table_host[q];
+    ael::k_mix_models<<<nblk(L.N), kBlock, 0, s>>>(eta, nmodels, tab, ncomp, L.N, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double* F0, const double* u,
                         const double* b, double* rhs, hipStream_t s) {
     if (!h || !C || !u || !b || !rhs) return fail(ALAMO_EL_EINVAL, "bad argument");

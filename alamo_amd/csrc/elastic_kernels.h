@@ -493,6 +493,26 @@ __device__ __forceinline__ void apply_node(const T* __restrict__ u, const T* __r
     else apply_interior<T, SYM>(u, C, uniform, L, id, out, gmask);
 }
 
+// Integrator::Mechanics::UpdateModel (src/Integrator/Mechanics.H:249-253):
+// model(node) = sum_n eta_n(node) * model_n, component by component, from zero
+// and in model order, each product rounded before it is added (the
+// reference's `model += eta * models[n]`), so no fused multiply-add.
+constexpr int kMixMax = 256;  // nmodels * ncomp, passed by value
+struct MixTable {
+    double v[kMixMax];
+};
+__global__ void __launch_bounds__(kBlock) k_mix_models(const double* __restrict__ eta, int nmodels, MixTable tab,
+                                                       int ncomp, long N, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= N) return;
+    for (int c = 0; c < ncomp; ++c) {
+        double acc = 0.0;
+        for (int n = 0; n < nmodels; ++n) acc += eta[n * N + id] * tab.v[n * ncomp + c];
+        out[c * N + id] = acc;
+    }
+}
+
 // Elastic<SYM>::Diagonal (src/Operator/Elastic.cpp:321-374) with
 // Gradient_Diagonal (src/Numeric/Stencil.H:900-963).
 template <typename T, int SYM>

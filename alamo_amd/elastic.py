@@ -60,7 +60,7 @@ EXPORTS = [
     "alamo_el_fsmooth", "alamo_el_residual", "alamo_el_restrict", "alamo_el_interp_add", "alamo_el_bc_init",
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
-    "alamo_amr_patch_info", "alamo_amr_solve",
+    "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -95,6 +95,8 @@ def lib():
         L.alamo_el_interp_add.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_bc_init.argtypes = [vp, vp, ctypes.c_int, vp]
         L.alamo_el_newton_rhs.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.alamo_el_mix_models.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int, vp,
+                                          vp]
         L.alamo_el_solve.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Opts),
                                      ctypes.POINTER(Info), vp]
         L.alamo_el_norm_inf.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double), vp]
@@ -292,6 +294,28 @@ class ElasticOperator:
         _check(lib().alamo_el_newton_rhs(self._h, _ptr(C), int(layout), f0, self._v(u, 0), self._v(b, 0),
                                          self._v(r, 0), _stream()),
                "alamo_el_newton_rhs")
+        return r
+
+    def mix_models(self, eta, table, out=None):
+        """Integrator::Mechanics::UpdateModel on the finest level
+        (src/Integrator/Mechanics.H:249-253): sum_n eta_n * table[n], from zero
+        in model order.  eta: (nmodels,) + level shape, float64 on the GPU;
+        table: (nmodels, ncomp) on the host (a model's C slots or its F0)."""
+        import numpy as np
+        tab = np.ascontiguousarray(np.asarray(table, dtype=np.float64))
+        if tab.ndim != 2:
+            raise AlamoError("mix_models: table must be (nmodels, ncomp)")
+        nmodels, ncomp = tab.shape
+        shape = self.level_shape(0)
+        if tuple(eta.shape) != (nmodels,) + shape:
+            raise AlamoError(f"mix_models: expected eta of shape {(nmodels,) + shape}, got {tuple(eta.shape)}")
+        torch = _torch()
+        r = torch.empty((ncomp,) + shape, dtype=torch.float64, device="cuda") if out is None else out
+        if tuple(r.shape) != (ncomp,) + shape:
+            raise AlamoError(f"mix_models: expected out of shape {(ncomp,) + shape}, got {tuple(r.shape)}")
+        _check(lib().alamo_el_mix_models(self._h, _ptr(eta), int(nmodels),
+                                         tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(ncomp), _ptr(r),
+                                         _stream()), "alamo_el_mix_models")
         return r
 
     def norm_inf(self, v, mglev: int = 0) -> float:

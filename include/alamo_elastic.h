@@ -168,6 +168,16 @@ int alamo_el_interp_add(alamo_el_t h, int fmglev, double* fine, const double* cr
  * of rhs get the boundary values, or 0 for displacement components when
  * homogeneous (src/BC/Operator/Elastic/Constant.H:197-286). */
 int alamo_el_bc_init(alamo_el_t h, double* rhs, int homogeneous, hipStream_t stream);
+/* Integrator::Mechanics::UpdateModel on the finest level
+ * (src/Integrator/Mechanics.H:249-253): out[c*N + node] =
+ * sum_n eta[n*N + node] * table_host[n*ncomp + c], from zero in model order,
+ * each product rounded before it is added, like the reference's
+ * `model += eta(i,j,k,n) * models[n]`.  eta and out are device fields
+ * (nmodels and ncomp planes); table_host holds each model's components on the
+ * host: its C in the handle's slot order (2 Isotropic, 21 MajorMinor, 9
+ * Diagonal; ALAMO_EL_SOA), or its F0 (9 values, 3a+b).  nmodels * ncomp <= 256. */
+int alamo_el_mix_models(alamo_el_t h, const double* eta, int nmodels, const double* table_host, int ncomp,
+                        double* out, hipStream_t stream);
 /* Newton<T>::prepareForSolve right-hand side, psi unset
  * (src/Solver/Nonlocal/Newton.H:151-231): rhs = b - BC(u, grad u, DW) on the
  * boundary and b - div DW in the interior, with DW = C:(grad u - F0).  F0 is

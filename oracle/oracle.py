@@ -172,3 +172,16 @@ def mlmg_solve(b, C, sym, bc, dx, x0=None, tol_rel=1e-10, tol_abs=0.0, max_iter=
 
 def num_threads():
     return lib().ora_num_threads()
+
+
+def mix_models(eta, table):
+    """Integrator::Mechanics::UpdateModel (src/Integrator/Mechanics.H:249-253):
+    model = Zero, then model += eta_n * models[n] for n in order.  Each product
+    is rounded before it is added (numpy multiplies, then adds).
+    eta: (nmodels, nz, ny, nx); table: (nmodels, ncomp) -> (ncomp, nz, ny, nx)."""
+    eta = np.asarray(eta, dtype=np.float64)
+    table = np.asarray(table, dtype=np.float64)
+    out = np.zeros((table.shape[1],) + eta.shape[1:])
+    for n in range(table.shape[0]):
+        out += eta[n][None] * table[n].reshape((-1,) + (1,) * (eta.ndim - 1))
+    return out

@@ -1127,6 +1127,44 @@ int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double*
     return 0;
 }
 
+int alamo_el_stress(alamo_el_t h, const double* C, int layout, const double* F0, const double* u, double* sigma,
+                    double* strain, hipStream_t s) {
+    if (!h || !C || !u || !sigma) return fail(ALAMO_EL_EINVAL, "bad argument");
+    const Lvl L = h->lv[0].L;
+    const double* Cs = C;
+    if (layout == ALAMO_EL_AOS) {
+        if (!h->csoa) ALLOC(h->csoa, (long)h->nc * L.N);
+        ael::k_aos_to_soa<<<nblk(L.N), kBlock, 0, s>>>(C, h->csoa, L.N, h->nc);
+        Cs = h->csoa;
+    } else if (layout != ALAMO_EL_SOA) {
+        return fail(ALAMO_EL_EINVAL, "bad layout");
+    }
+    switch (h->sym) {
+        case 0: ael::k_stress<0><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, sigma, strain); break;
+        case 1: ael::k_stress<1><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, sigma, strain); break;
+        default: ael::k_stress<2><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, sigma, strain); break;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_face_traction(alamo_el_t h, const double* sigma, const double* u, int axis, int side,
+                           double* trac_host, double* disp_host, hipStream_t s) {
+    if (!h || !sigma || !u || !trac_host || axis < 0 || axis > 2 || (side != 0 && side != 1))
+        return fail(ALAMO_EL_EINVAL, "bad argument");
+    const Lvl L = h->lv[0].L;
+    ael::k_face_traction<<<1, ael::kFaceBlock, 0, s>>>(sigma, u, L, axis, side, h->red_partial);
+    HIPCHK(hipGetLastError());
+    double out[6];
+    HIPCHK(hipMemcpyAsync(out, h->red_partial, sizeof(out), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int r = 0; r < 3; ++r) {
+        trac_host[r] = out[r];
+        if (disp_host) disp_host[r] = out[3 + r];
+    }
+    return 0;
+}
+
 int alamo_el_norm_inf(alamo_el_t h, int mglev, const double* v, double* out_host, hipStream_t s) {
     if (bad_level(h, mglev) || !v || !out_host) return fail(ALAMO_EL_EINVAL, "bad argument");
     return reduce_host<double>(h, v, v, 3 * h->lv[mglev].L.N, 0, out_host, s);

@@ -797,6 +797,86 @@ __global__ void __launch_bounds__(kBlock) k_newton_dw(const double* __restrict__
         }
 }
 
+// Base::Mechanics post-processing (src/Integrator/Base/Mechanics.H:210-245),
+// small-strain models: stress = DW(grad u) = C:(grad u - F0) and strain =
+// (grad u + grad u^T) / 2 at every node, grad u central inside and one-sided
+// on the domain faces (Numeric::GetStencil on the domain-clipped box).
+// sigma and strain: 9 planes, (3p+q).
+template <int SYM>
+__global__ void __launch_bounds__(kBlock) k_stress(const double* __restrict__ u, const double* __restrict__ C,
+                                                   const double* __restrict__ F0, Lvl L, double* __restrict__ sigma,
+                                                   double* __restrict__ strain) {
+    constexpr int NC = NComp<SYM>::value;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    const long sy = L.nx, sz = (long)L.nx * L.ny;
+    double gu[3][3], A[3][3], s[3][3], c[NC];
+    for (int p = 0; p < 3; ++p) {
+        const double* f = u + p * L.N;
+        gu[p][0] = d1(f, id, 1, i, L.nx, L.dx);
+        gu[p][1] = d1(f, id, sy, j, L.ny, L.dy);
+        gu[p][2] = d1(f, id, sz, k, L.nz, L.dz);
+    }
+    for (int p = 0; p < 3; ++p)
+        for (int a = 0; a < 3; ++a) A[p][a] = gu[p][a] - (F0 ? F0[(3 * p + a) * L.N + id] : 0.0);
+    for (int q = 0; q < NC; ++q) c[q] = C[q * L.N + id];
+    contract2<double, SYM>(c, F0 ? A : gu, s);
+    for (int p = 0; p < 3; ++p)
+        for (int a = 0; a < 3; ++a) {
+            sigma[(3 * p + a) * L.N + id] = s[p][a];
+            if (strain) strain[(3 * p + a) * L.N + id] = 0.5 * (gu[p][a] + gu[a][p]);
+        }
+}
+
+// Base::Mechanics::Integrate, 3D (src/Integrator/Base/Mechanics.H:391-448):
+// the traction on a domain face, sum over the face's cells of
+// 0.25 (sigma at the 4 corner nodes) . da with da = e_axis h_b h_c, and the
+// displacement at the face node whose two in-face indices are n - 2 (the
+// reference's disp_hi, last written in its loop order).  The reference
+// integrates the x-hi face; any face is allowed here.  One workgroup: out[0..2]
+// traction, out[3..5] displacement.
+constexpr int kFaceBlock = 1024;
+__global__ void __launch_bounds__(kFaceBlock) k_face_traction(const double* __restrict__ sigma,
+                                                              const double* __restrict__ u, Lvl L, int axis, int side,
+                                                              double* __restrict__ out) {
+    __shared__ double red[3][kFaceBlock];
+    const int n[3] = {L.nx, L.ny, L.nz};
+    const double hh[3] = {L.dx, L.dy, L.dz};
+    const int b = axis == 0 ? 1 : 0, c = axis == 2 ? 1 : 2;  // the in-face axes, b < c
+    const int ia = side ? n[axis] - 1 : 0;
+    const long st[3] = {1, (long)L.nx, (long)L.nx * L.ny};
+    const double da = hh[b] * hh[c];
+    const long nb = n[b] - 1, ncell = nb * (n[c] - 1);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (long q = threadIdx.x; q < ncell; q += kFaceBlock) {
+        const long jb = q % nb, jc = q / nb;
+        const long id = ia * st[axis] + jb * st[b] + jc * st[c];
+        const long corner[4] = {id, id + st[b], id + st[c], id + st[b] + st[c]};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const double* sr = sigma + (3 * r + axis) * L.N;
+            acc[r] += 0.25 * (sr[corner[0]] + sr[corner[1]] + sr[corner[2]] + sr[corner[3]]) * da;
+        }
+    }
+    for (int r = 0; r < 3; ++r) red[r][threadIdx.x] = acc[r];
+    __syncthreads();
+    for (int w = kFaceBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int r = 0; r < 3; ++r) red[r][threadIdx.x] += red[r][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const long jb = n[b] >= 2 ? n[b] - 2 : 0, jc = n[c] >= 2 ? n[c] - 2 : 0;
+        const long id = ia * st[axis] + jb * st[b] + jc * st[c];
+        for (int r = 0; r < 3; ++r) {
+            out[r] = red[r][0];
+            out[3 + r] = u[r * L.N + id];
+        }
+    }
+}
+
 // Newton set-up, pass 2 (src/Solver/Nonlocal/Newton.H:216-231).
 __global__ void __launch_bounds__(kBlock) k_newton_rhs(const double* __restrict__ u, const double* __restrict__ b,
                                                        const double* __restrict__ gu9, const double* __restrict__ dw9,

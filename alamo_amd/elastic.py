@@ -60,7 +60,7 @@ EXPORTS = [
     "alamo_el_fsmooth", "alamo_el_residual", "alamo_el_restrict", "alamo_el_interp_add", "alamo_el_bc_init",
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
-    "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models",
+    "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -95,6 +95,9 @@ def lib():
         L.alamo_el_interp_add.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_bc_init.argtypes = [vp, vp, ctypes.c_int, vp]
         L.alamo_el_newton_rhs.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.alamo_el_stress.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.alamo_el_face_traction.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double), vp]
         L.alamo_el_mix_models.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int, vp,
                                           vp]
         L.alamo_el_solve.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Opts),
@@ -317,6 +320,33 @@ class ElasticOperator:
                                          tab.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(ncomp), _ptr(r),
                                          _stream()), "alamo_el_mix_models")
         return r
+
+    def stress(self, C, u, F0=None, layout: int = SOA, strain: bool = False):
+        """Base::Mechanics stress (and strain) on the finest level
+        (src/Integrator/Base/Mechanics.H:210-245): sigma = C:(grad u - F0),
+        9 planes (3p+q).  Returns sigma, or (sigma, strain)."""
+        torch = _torch()
+        shape = self.level_shape(0)
+        want = (self.ncomp,) + shape if layout == SOA else shape + (self.ncomp,)
+        if tuple(C.shape) != want:
+            raise AlamoError(f"stress: expected C of shape {want}, got {tuple(C.shape)}")
+        f0 = self._v(F0, 0, 9) if F0 is not None else ctypes.c_void_p()
+        sig = torch.empty((9,) + shape, dtype=torch.float64, device="cuda")
+        eps = torch.empty((9,) + shape, dtype=torch.float64, device="cuda") if strain else None
+        _check(lib().alamo_el_stress(self._h, _ptr(C), int(layout), f0, self._v(u, 0), _ptr(sig),
+                                     _ptr(eps) if strain else ctypes.c_void_p(), _stream()), "alamo_el_stress")
+        return (sig, eps) if strain else sig
+
+    def face_traction(self, sigma, u, axis: int = 0, side: int = 1):
+        """Base::Mechanics::Integrate, 3D (src/Integrator/Base/Mechanics.H:391-448):
+        (traction, displacement) on a domain face; axis 0, side 1 is the
+        reference's trac_hi / disp_hi of thermo.dat."""
+        import numpy as np
+        trac = (ctypes.c_double * 3)()
+        disp = (ctypes.c_double * 3)()
+        _check(lib().alamo_el_face_traction(self._h, self._v(sigma, 0, 9), self._v(u, 0), int(axis), int(side), trac,
+                                            disp, _stream()), "alamo_el_face_traction")
+        return np.array(trac[:]), np.array(disp[:])
 
     def norm_inf(self, v, mglev: int = 0) -> float:
         out = ctypes.c_double()

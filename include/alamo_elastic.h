@@ -168,6 +168,23 @@ int alamo_el_interp_add(alamo_el_t h, int fmglev, double* fine, const double* cr
  * of rhs get the boundary values, or 0 for displacement components when
  * homogeneous (src/BC/Operator/Elastic/Constant.H:197-286). */
 int alamo_el_bc_init(alamo_el_t h, double* rhs, int homogeneous, hipStream_t stream);
+/* Base::Mechanics post-processing on the finest level, small-strain models
+ * (src/Integrator/Base/Mechanics.H:210-245): sigma = DW(grad u) =
+ * C:(grad u - F0) and, when strain is not NULL, strain = (grad u + grad u^T)/2,
+ * at every node, grad u central inside and one-sided on the domain faces.
+ * sigma and strain: 9 planes (3p+q) of N; C in the layout passed; F0 NULL or
+ * 9 planes. */
+int alamo_el_stress(alamo_el_t h, const double* C, int layout, const double* F0, const double* u, double* sigma,
+                    double* strain, hipStream_t stream);
+/* Base::Mechanics::Integrate, 3D (src/Integrator/Base/Mechanics.H:391-448):
+ * the traction on the domain face normal to axis (0..2) at side (0 low,
+ * 1 high), sum over the face's cells of 0.25 (sigma at the 4 corner nodes) . da
+ * with da = e_axis * (product of the other two cell sizes), and the
+ * displacement at the face node whose two in-face indices are n - 2 (the
+ * reference's trac_hi / disp_hi of thermo.dat for axis 0, side 1).  Synchronizes
+ * the stream; trac_host and disp_host (may be NULL) receive 3 values. */
+int alamo_el_face_traction(alamo_el_t h, const double* sigma, const double* u, int axis, int side,
+                           double* trac_host, double* disp_host, hipStream_t stream);
 /* Integrator::Mechanics::UpdateModel on the finest level
  * (src/Integrator/Mechanics.H:249-253): out[c*N + node] =
  * sum_n eta[n*N + node] * table_host[n*ncomp + c], from zero in model order,

@@ -185,3 +185,42 @@ def mix_models(eta, table):
     for n in range(table.shape[0]):
         out += eta[n][None] * table[n].reshape((-1,) + (1,) * (eta.ndim - 1))
     return out
+
+
+def face_traction(sigma, u, dx, axis=0, side=1):
+    """Base::Mechanics::Integrate, 3D (src/Integrator/Base/Mechanics.H:391-448):
+    over the cells of a domain face, in the reference's loop order, add
+    0.25 (sigma at the 4 corner nodes, summed in the order (0,0), (+b), (+c),
+    (+b+c)) . da with da = e_axis * h_b * h_c; and the displacement at the face
+    node whose in-face indices are n - 2 (disp_hi).  sigma: (9, nz, ny, nx),
+    component 3p+q; u: (3, nz, ny, nx).  Returns (traction[3], disp[3])."""
+    sigma = np.asarray(sigma, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    h = _dx(dx)
+    n = (sigma.shape[3], sigma.shape[2], sigma.shape[1])  # nx, ny, nz
+    b, c = (1, 2) if axis == 0 else ((0, 2) if axis == 1 else (0, 1))
+    ia = n[axis] - 1 if side else 0
+    da = h[b] * h[c]
+
+    def at(comp, idx):
+        return sigma[comp, idx[2], idx[1], idx[0]]
+
+    trac = np.zeros(3)
+    for jc in range(n[c] - 1):
+        for jb in range(n[b] - 1):
+            base = [0, 0, 0]
+            base[axis], base[b], base[c] = ia, jb, jc
+            corners = []
+            for db, dcc in ((0, 0), (1, 0), (0, 1), (1, 1)):
+                q = list(base)
+                q[b] += db
+                q[c] += dcc
+                corners.append(q)
+            for r in range(3):
+                comp = 3 * r + axis
+                trac[r] += 0.25 * (at(comp, corners[0]) + at(comp, corners[1]) + at(comp, corners[2])
+                                   + at(comp, corners[3])) * da
+    node = [0, 0, 0]
+    node[axis], node[b], node[c] = ia, max(n[b] - 2, 0), max(n[c] - 2, 0)
+    disp = np.array([u[r, node[2], node[1], node[0]] for r in range(3)])
+    return trac, disp

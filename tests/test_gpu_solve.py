@@ -147,3 +147,42 @@ def test_uniaxial_tension_closed_form():
     ue = np.stack([np.broadcast_to(eps * xx, shape), np.broadcast_to(-0.3 * eps * y, shape),
                    np.broadcast_to(-0.3 * eps * z, shape)])
     assert np.abs(host(x) - ue).max() < 1e-8 * eps
+
+
+@pytest.mark.parametrize("name,F0", [("linear", None), ("affine", 1e-3)])
+def test_solid_cubic_regression_end_to_end(name, F0):
+    """tests/Solid (input:7-27, test:24-51) on the GPU end to end: for each
+    time of the committed reference series, the device Newton right-hand side
+    (src/Solver/Nonlocal/Newton.H:151-231), the FP64 solve, the device stress
+    (src/Integrator/Base/Mechanics.H:210-245) and the x-hi face traction of
+    thermo.dat (Mechanics.H:391-448); the checker's metric against the
+    reference's trac_xhi_x must stay below its 1e-4 tolerance."""
+    import os
+    from alamo_amd import elastic as el
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", f"solid_{name}_cubic.dat"), skiprows=1)
+    n = 4
+    N = n + 1
+    slots = M.cubic(168.3, 1.221, 0.757)
+    C = dev(np.ascontiguousarray(np.broadcast_to(slots[:, None, None, None], (21, N, N, N))))
+    F0f = None
+    if F0 is not None:
+        F0f = dev(np.ascontiguousarray(np.broadcast_to((F0 * np.eye(3)).ravel()[:, None, None, None], (9, N, N, N))))
+    op = el.ElasticOperator(n, 1.0 / n, M.MAJORMINOR)
+    op.SetModel(C)
+    zero = torch.zeros((3, N, N, N), dtype=torch.float64, device="cuda")
+    trac = []
+    for t in ref[:, 0]:
+        op.SetBC(B.tension_test("uniaxial_stress", disp=0.01 * t))
+        b = op.bc_init(zero.clone())
+        rhs = op.newton_rhs(C, zero, b, F0=F0f)
+        du = torch.zeros_like(b)
+        el.Linear(op, cycle="V").solve(du, rhs, 1e-10, 0.0)
+        sig = op.stress(C, du, F0=F0f)
+        trac.append(op.face_traction(sig, du, axis=0, side=1)[0][0])
+    trac = np.array(trac)
+
+    def integrate(xv, yv):
+        return np.sum(0.5 * (yv[1:] + yv[:-1]) * (xv[1:] - xv[:-1]))
+    times = ref[:, 0]
+    err = np.sqrt(integrate(times, (ref[:, 2] - trac) ** 2)) / np.sqrt(integrate(times, ref[:, 2] ** 2))
+    assert err < 1e-4, err

@@ -327,12 +327,7 @@ def test_solid_cubic_regression_fixture(name, F0):
         rhs = O.newton_rhs(np.zeros((3, N, N, N)), b, C, M.MAJORMINOR, bc, h, F0=F0f)
         du, info = O.mlmg_solve(rhs, C, M.MAJORMINOR, bc, h, tol_rel=1e-10, cycle="V")
         sig = O.stress(du, C, M.MAJORMINOR, h, F0=F0f)
-        sxx = sig[0]
-        face = sxx[:, :, n]
-        # 0.25 (4 nodes) * dy*dz over the face cells
-        tr = sum(0.25 * (face[k, j] + face[k, j + 1] + face[k + 1, j] + face[k + 1, j + 1]) * h * h
-                 for k in range(n) for j in range(n))
-        trac.append(tr)
+        trac.append(O.face_traction(sig, du, h, axis=0, side=1)[0][0])
     trac = np.array(trac)
 
     def integrate(xv, yv):
@@ -430,3 +425,30 @@ def test_model_mixing_pins():
     w /= w.sum(axis=0)
     same = np.tile(table[0], (4, 1))
     assert np.allclose(O.mix_models(w, same), table[0][:, None, None, None] * np.ones(shape), rtol=1e-14, atol=1e-14)
+
+
+def test_face_traction_closed_forms():
+    """The oracle's thermo.dat traction (src/Integrator/Base/Mechanics.H:391-448):
+    a uniform stress integrates to sigma . e_axis times the face area on every
+    face; a stress linear along the face integrates exactly (the 4-corner
+    average is the cell-centre value); disp_hi is u at in-face indices n - 2."""
+    rng = np.random.default_rng(8)
+    shape = (5, 6, 7)  # nz, ny, nx
+    dx = (0.5, 0.25, 0.2)
+    S = rng.standard_normal((3, 3))
+    sig = np.ascontiguousarray(np.broadcast_to(S.reshape(9, 1, 1, 1), (9,) + shape))
+    u = rng.standard_normal((3,) + shape)
+    L = [(shape[2] - 1) * dx[0], (shape[1] - 1) * dx[1], (shape[0] - 1) * dx[2]]
+    for axis in range(3):
+        area = np.prod([L[a] for a in range(3) if a != axis])
+        for side in (0, 1):
+            tr, dp = O.face_traction(sig, u, dx, axis, side)
+            assert np.allclose(tr, S[:, axis] * area, rtol=1e-13, atol=1e-13)
+    z, y, x = np.meshgrid(np.arange(shape[0]) * dx[2], np.arange(shape[1]) * dx[1], np.arange(shape[2]) * dx[0],
+                          indexing="ij")
+    lin = np.zeros((9,) + shape)
+    lin[0] = 1.0 + 2.0 * y + 3.0 * z  # sigma_xx on the x faces, linear in y and z
+    tr, dp = O.face_traction(lin, u, dx, 0, 1)
+    exact = L[1] * L[2] + 2.0 * L[1] ** 2 / 2 * L[2] + 3.0 * L[2] ** 2 / 2 * L[1]
+    assert abs(tr[0] - exact) < 1e-12 * abs(exact) and tr[1] == 0.0 and tr[2] == 0.0
+    assert np.array_equal(dp, u[:, shape[0] - 2, shape[1] - 2, shape[2] - 1])

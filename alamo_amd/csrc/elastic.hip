@@ -62,7 +62,7 @@ struct Vecs {
     T* rescor = nullptr;
     T* tmp = nullptr;
     T* D = nullptr;  // compact grad-C matrices, 18 x nflag (MajorMinor)
-    T* Ct = nullptr; // tiled levels: padded interior copy of C (ael::TileC)
+    T* Ct = nullptr; // tiled levels: tile-blocked interior copy of C (ael::TileC)
     T* Cb = nullptr; // tiled levels: C of the boundary nodes, in boundary-node order
 };
 
@@ -587,7 +587,7 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     const int wpb = ael::kWavesPerBlock;
     // count/fill workgroups: one per (tile, plane) on tiled levels (the compact
     // D order of both tile grids), else the slab launch's interior ones
-    const unsigned nib = l.tiled ? (unsigned)(ael::tile_wave_planes(l.L, l.tg, wpb) / wpb)
+    const unsigned nib = l.tiled ? (unsigned)ael::tile_planes(l.L, l.tg)
                                  : ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
     const long nw = (long)nib * wpb;
     if (l.dwbase) (void)hipFree(l.dwbase);
@@ -601,9 +601,10 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     l.nflag = 0;
     if (l.tiled) {
         const long nb = ael::boundary_count(l.L);
-        HIPCHK(hipMalloc((void**)&l.d.Ct, sizeof(double) * h->nc * ael::TileC<double>::plane(l.L)));
+        HIPCHK(hipMalloc((void**)&l.d.Ct, sizeof(double) * ael::TileC<TileT>::elems(l.L, l.tg)));
         HIPCHK(hipMalloc((void**)&l.d.Cb, sizeof(double) * h->nc * nb));
-        ael::k_pack_tile_c<double><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.d.Ct);
+        ael::k_pack_tile_c<TileT, double><<<(unsigned)ael::tile_planes(l.L, l.tg), TileT::NT, 0, s>>>(l.d.C, l.L, l.tg,
+                                                                                                   l.d.Ct);
         ael::k_pack_bnd_c<double><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.d.Cb);
         HIPCHK(hipGetLastError());
     }
@@ -682,9 +683,10 @@ int prepare32(alamo_el_t h, hipStream_t s) {
             if (l.f.Ct) (void)hipFree(l.f.Ct);
             if (l.f.Cb) (void)hipFree(l.f.Cb);
             l.f.Ct = l.f.Cb = nullptr;
-            HIPCHK(hipMalloc((void**)&l.f.Ct, sizeof(float) * h->nc * ael::TileC<float>::plane(l.L)));
+            HIPCHK(hipMalloc((void**)&l.f.Ct, sizeof(float) * ael::TileC<TileT>::elems(l.L, l.tg)));
             HIPCHK(hipMalloc((void**)&l.f.Cb, sizeof(float) * h->nc * nb));
-            ael::k_pack_tile_c<float><<<nblk(l.L.N), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.f.Ct);
+            ael::k_pack_tile_c<TileT, float><<<(unsigned)ael::tile_planes(l.L, l.tg), TileT::NT, 0, s>>>(l.d.C, l.L,
+                                                                                                     l.tg, l.f.Ct);
             ael::k_pack_bnd_c<float><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.f.Cb);
         }
         if (l.d.D) {

@@ -17,11 +17,11 @@
 // (see the kernel).  Jacobi sweeps compute an interior node's diagonal from
 // its C instead of reading diag.
 //
-// The kernel reads C from two copies of its own (TileC): the interior nodes
-// from rows padded so that node i = 1 starts a 128-byte line (a tile row is
-// then whole lines: 2 in FP64, 1 in FP32, and the requests to L2 are full
-// lines), and the boundary nodes from a compact array in the order of the
-// boundary workgroups, so that the two never share a cache line.
+// The kernel reads C from two copies of its own: the interior nodes from a
+// tile-blocked copy (TileC: the 21 slots of a tile's plane in one contiguous
+// block, whole 128-byte lines per wave and slot), and the boundary nodes from
+// a compact array in the order of the boundary workgroups, so that the two
+// never share a cache line.
 //
 // Tiles are XCD-slab ordered: XCD x (workgroups b with b % kXcd == x) owns
 // the j-tiles [x*nty/kXcd, (x+1)*nty/kXcd) of every chunk, so the halo rows
@@ -49,27 +49,32 @@ struct TileGrid {
 // a plane in one 64-bit register.
 constexpr int kMaxChunk = 16;
 
-// Padded interior copy of C: slot s of node (i, j, k) at
-// s * NP + (k * ny + j) * P + i + A, with A = E - 1 and P a multiple of E,
-// E = 128 / sizeof(T) (values per 128-byte line).
-template <typename T>
+// Interior copy of C in tile blocks: the 21 slots of the TX x TY nodes of
+// one (tile, plane) are contiguous, slot-major, so that the workgroup's 21
+// loads of a plane read one block (21 NT values) and each wave's load of a
+// slot is 64 consecutive values.  Slot s of the node (tx, ty) of tile
+// (it, jt) in plane k is at block(G, k, it, jt) + s NT + ty TX + tx; nodes of
+// partial tiles outside the interior hold zeros.
+template <class TS>
 struct TileC {
-    static constexpr int E = 128 / (int)sizeof(T);
-    static constexpr int A = E - 1;
-    __host__ __device__ static int pitch(const Lvl& L) { return (L.nx + A + E - 1) / E * E; }
-    __host__ __device__ static long plane(const Lvl& L) { return (long)L.nz * L.ny * pitch(L); }
+    static constexpr int NC = 21;
+    __host__ __device__ static long block(const TileGrid& G, int k, int it, int jt) {
+        return (((long)(k - 1) * G.nty + jt) * G.ntx + it) * NC * TS::NT;
+    }
+    static long elems(const Lvl& L, const TileGrid& G) { return (long)(L.nz - 2) * G.nty * G.ntx * NC * TS::NT; }
 };
-// Copies of C for the tiled kernel, from the FP64 C (rounded to T).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_pack_tile_c(const double* __restrict__ C, Lvl L, int nc,
+// The tile-blocked copy from the FP64 C (rounded to T).  Grid: one workgroup
+// of NT threads per (tile, plane), in block order.
+template <class TS, typename T>
+__global__ void __launch_bounds__(TS::NT) k_pack_tile_c(const double* __restrict__ C, Lvl L, TileGrid G,
                                                         T* __restrict__ Ct) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= L.N) return;
-    int i, j, k;
-    node_ijk(id, L, i, j, k);
-    const long P = TileC<T>::pitch(L), NP = TileC<T>::plane(L);
-    const long o = ((long)k * L.ny + j) * P + i + TileC<T>::A;
-    for (int s = 0; s < nc; ++s) Ct[s * NP + o] = (T)C[s * L.N + id];
+    const long tp = blockIdx.x;  // ((k - 1) * nty + jt) * ntx + it
+    const int it = (int)(tp % G.ntx), jt = (int)(tp / G.ntx % G.nty), k = 1 + (int)(tp / ((long)G.ntx * G.nty));
+    const int i = 1 + it * TS::TX + (int)threadIdx.x % TS::TX, j = 1 + jt * TS::TY + (int)threadIdx.x / TS::TX;
+    const bool v = i <= L.nx - 2 && j <= L.ny - 2;
+    const long id = ((long)k * L.ny + j) * L.nx + i;
+    T* blk = Ct + TileC<TS>::block(G, k, it, jt) + threadIdx.x;
+    for (int s = 0; s < TileC<TS>::NC; ++s) blk[(long)s * TS::NT] = v ? (T)C[s * L.N + id] : T(0);
 }
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pack_bnd_c(const double* __restrict__ C, Lvl L, int nc,
@@ -127,8 +132,9 @@ __device__ __forceinline__ bool tile_of(const TileGrid& G, long ib, int& it, int
 __host__ __device__ inline long tile_wave_plane(const TileGrid& G, int k, int it, int jt, int wave, int wpb) {
     return (((long)(k - 1) * G.nty + jt) * G.ntx + it) * wpb + wave;
 }
-// Wave-planes of a tiled level.
-inline long tile_wave_planes(const Lvl& L, const TileGrid& G, int wpb) { return (long)(L.nz - 2) * G.nty * G.ntx * wpb; }
+// (tile, plane) pairs and wave-planes of a tiled level.
+inline long tile_planes(const Lvl& L, const TileGrid& G) { return (long)(L.nz - 2) * G.nty * G.ntx; }
+inline long tile_wave_planes(const Lvl& L, const TileGrid& G, int wpb) { return tile_planes(L, G) * wpb; }
 
 // Flagged lanes per wave-plane.  Grid: one workgroup per (tile, plane), in
 // wave-plane order.
@@ -240,9 +246,6 @@ __global__ void __launch_bounds__(TS::NT)
     const int k0 = 1 + kc * G.kc, k1 = min(k0 + G.kc, L.nz - 1);  // planes [k0, k1)
     const unsigned N = (unsigned)L.N, sxy = (unsigned)(L.nx * L.ny);
     const unsigned col = (unsigned)(j * L.nx + i);
-    const unsigned cP = (unsigned)TileC<T>::pitch(L);
-    const size_t cNP = (size_t)TileC<T>::plane(L);
-    const unsigned ccol = (unsigned)(j * cP + i + TileC<T>::A);
 
     // prologue: planes k0-1, k0, k0+1 into ring slots 0, 1, 2; the chunk's
     // gradient masks (packed, plane kk in bits 4 kk .. 4 kk + 3) and, in lane kk of
@@ -285,13 +288,14 @@ __global__ void __launch_bounds__(TS::NT)
         const int sm = kk % 3, sc = (kk + 1) % 3, sp = (kk + 2) % 3;  // slots of k-1, k, k+1
         const bool more = k + 1 < k1;
         const unsigned o = (unsigned)k * sxy + col;
-        const unsigned oc = (unsigned)k * ((unsigned)L.ny * cP) + ccol;  // in the padded copy of C
+        const T* cblk = Ct + TileC<TS>::block(G, k, it, jt);  // this (tile, plane) of the C copy
 
         // every global load of the plane leaves here, before the LDS work
         T cv[21], dv[18], pre[TS::PRE], bv[3], gv[3];
         if (valid) {
 #pragma unroll
-            for (int s = 0; s < 21; ++s) cv[s] = NTC ? ld32nt(Ct + (size_t)s * cNP, oc) : ld32(Ct + (size_t)s * cNP, oc);
+            for (int s = 0; s < 21; ++s)
+                cv[s] = NTC ? ld32nt(cblk + s * TS::NT, threadIdx.x) : ld32(cblk + s * TS::NT, threadIdx.x);
 #pragma unroll
             for (int p = 0; p < 3; ++p) bv[p] = (OP >= 1) ? ld32(b + (size_t)p * N, o) : T(0);
         }

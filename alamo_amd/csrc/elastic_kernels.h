@@ -284,10 +284,25 @@ __device__ __forceinline__ void gradc_mm(const T* __restrict__ C, const Lvl& L, 
 // used for every term it enters, and dropped, so only the 18 second
 // derivative combinations e[j][Q], the 6 symmetric gradients e1[Q] and the
 // result stay live.  Grid spacings enter through reciprocals computed once.
+// The operands of mm_core at interior node o: the 19-point neighbourhood of
+// each component, uv[p][n] with n = centre, x+, x-, y+, y-, z+, z-, then the
+// xy, xz and yz corners (++, --, +-, -+), and the 21 slots of C.
 template <typename T>
-__device__ __forceinline__ void mm_core(const T* __restrict__ u, const T* __restrict__ C, const Lvl& L, unsigned o,
-                                        T out[3], T e1[6]) {
+__device__ __forceinline__ void mm_load(const T* __restrict__ u, const T* __restrict__ C, const Lvl& L, unsigned o,
+                                        T uv[3][19], T cv[21]) {
     const unsigned N = (unsigned)L.N, sy = (unsigned)L.nx, sz = (unsigned)(L.nx * L.ny);
+    const unsigned off[19] = {o,           o + 1,       o - 1,       o + sy,      o - sy,      o + sz,      o - sz,
+                              o + 1 + sy,  o - 1 - sy,  o + 1 - sy,  o - 1 + sy,  o + 1 + sz,  o - 1 - sz,  o + 1 - sz,
+                              o - 1 + sz,  o + sy + sz, o - sy - sz, o + sy - sz, o - sy + sz};
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int n = 0; n < 19; ++n) uv[p][n] = ld32(u + (size_t)p * N, off[n]);
+#pragma unroll
+    for (int s = 0; s < 21; ++s) cv[s] = ld32(C + (size_t)s * N, o);
+}
+template <typename T>
+__device__ __forceinline__ void mm_compute(const T uv[3][19], const T cv[21], const Lvl& L, T out[3], T e1[6]) {
     const T ihx = T(1) / T(L.dx), ihy = T(1) / T(L.dy), ihz = T(1) / T(L.dz);
     const T hx2 = T(0.5) * ihx, hy2 = T(0.5) * ihy, hz2 = T(0.5) * ihz;
     const T ixx = ihx * ihx, iyy = ihy * ihy, izz = ihz * ihz;
@@ -295,19 +310,17 @@ __device__ __forceinline__ void mm_core(const T* __restrict__ u, const T* __rest
     T gu[3][3], B[3][3][3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-        const T* f = u + (size_t)p * N;
-        const T c0 = ld32(f, o);
-        const T xp = ld32(f, o + 1), xm = ld32(f, o - 1), yp = ld32(f, o + sy), ym = ld32(f, o - sy);
-        const T zp = ld32(f, o + sz), zm = ld32(f, o - sz);
+        const T* f = uv[p];
+        const T c0 = f[0], xp = f[1], xm = f[2], yp = f[3], ym = f[4], zp = f[5], zm = f[6];
         gu[p][0] = (xp - xm) * hx2;
         gu[p][1] = (yp - ym) * hy2;
         gu[p][2] = (zp - zm) * hz2;
         B[p][0][0] = (xp - T(2) * c0 + xm) * ixx;
         B[p][1][1] = (yp - T(2) * c0 + ym) * iyy;
         B[p][2][2] = (zp - T(2) * c0 + zm) * izz;
-        B[p][0][1] = (ld32(f, o + 1 + sy) + ld32(f, o - 1 - sy) - ld32(f, o + 1 - sy) - ld32(f, o - 1 + sy)) * ixy;
-        B[p][0][2] = (ld32(f, o + 1 + sz) + ld32(f, o - 1 - sz) - ld32(f, o + 1 - sz) - ld32(f, o - 1 + sz)) * ixz;
-        B[p][1][2] = (ld32(f, o + sy + sz) + ld32(f, o - sy - sz) - ld32(f, o + sy - sz) - ld32(f, o - sy + sz)) * iyz;
+        B[p][0][1] = (f[7] + f[8] - f[9] - f[10]) * ixy;
+        B[p][0][2] = (f[11] + f[12] - f[13] - f[14]) * ixz;
+        B[p][1][2] = (f[15] + f[16] - f[17] - f[18]) * iyz;
         B[p][1][0] = B[p][0][1];
         B[p][2][0] = B[p][0][2];
         B[p][2][1] = B[p][1][2];
@@ -325,13 +338,20 @@ __device__ __forceinline__ void mm_core(const T* __restrict__ u, const T* __rest
 #pragma unroll
     for (int s = 0; s < 21; ++s) {
         const int P = slot_p(s), Q = slot_q(s);
-        const T c = ld32(C + (size_t)s * N, o);
+        const T c = cv[s];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             if (pair_other(P, j) >= 0) out[pair_other(P, j)] += c * e[j][Q];
             if (Q != P && pair_other(Q, j) >= 0) out[pair_other(Q, j)] += c * e[j][P];
         }
     }
+}
+template <typename T>
+__device__ __forceinline__ void mm_core(const T* __restrict__ u, const T* __restrict__ C, const Lvl& L, unsigned o,
+                                        T out[3], T e1[6]) {
+    T uv[3][19], cv[21];
+    mm_load<T>(u, C, L, o, uv, cv);
+    mm_compute<T>(uv, cv, L, out, e1);
 }
 
 template <typename T>

@@ -193,23 +193,41 @@ __global__ void __launch_bounds__(kBlock)
     } else {
         const long ib = (long)(blockIdx.x - nbb);
         if (!slab_interior_node(L, ib, i, j, k, id)) return;
-        const unsigned gm = gmask ? (unsigned)gmask[id] : 7u;
         if constexpr (SYM == 1 && DC) {
-            {
-                T e1[6];
-                mm_core<T>(x, C, L, (unsigned)id, o, e1);
-                const unsigned long long bal = __ballot(gm != 0u);
-                if (!uniform && gm != 0u) {
-                    const unsigned p = (unsigned)dwbase[ib * kWavesPerBlock + threadIdx.x / 64] +
-                                       (unsigned)__popcll(bal & lanes_below());
+            // Every load of the node leaves first (the 19-point u neighbourhood,
+            // C, the mask, the wave's first D position and the OP operands),
+            // ahead of the arithmetic: on the latency-bound small levels the
+            // kernel then waits on memory twice (these, then D), not once per
+            // group of loads the compiler would otherwise interleave.
+            const unsigned o32 = (unsigned)id;
+            const int dwb = dwbase[ib * kWavesPerBlock + threadIdx.x / 64];
+            const unsigned gm = gmask[id];  // the compact-D launch always passes the mask
+            T uv[3][19], cv[21], xv[3], bv[3], dv[3];
+            mm_load<T>(x, C, L, o32, uv, cv);
 #pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                        for (int Q = 0; Q < 6; ++Q) o[r] += ld32(Dc + (size_t)(6 * r + Q) * nflag, p) * e1[Q];
-                }
+            for (int p = 0; p < 3; ++p) {
+                const size_t off = (size_t)p * L.N;
+                xv[p] = (OP == 2) ? uv[p][0] : T(0);
+                bv[p] = (OP >= 1) ? ld32(b + off, o32) : T(0);
+                dv[p] = (OP == 2) ? ld32(diag + off, o32) : T(1);
             }
+            __builtin_amdgcn_sched_barrier(0);
+            T e1[6];
+            mm_compute<T>(uv, cv, L, o, e1);
+            const unsigned long long bal = __ballot(gm != 0u);
+            if (!uniform && gm != 0u) {
+                const unsigned p = (unsigned)dwb + (unsigned)__popcll(bal & lanes_below());
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int Q = 0; Q < 6; ++Q) o[r] += ld32(Dc + (size_t)(6 * r + Q) * nflag, p) * e1[Q];
+            }
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+                st32(out + (size_t)p * L.N, o32, finish_op<T, OP>(o[p], xv[p], bv[p], dv[p], omega));
+            return;
         } else {
-            apply_interior<T, SYM>(x, C, uniform, L, id, o, gm);
+            apply_interior<T, SYM>(x, C, uniform, L, id, o, gmask ? (unsigned)gmask[id] : 7u);
         }
     }
     const unsigned n32 = (unsigned)id;

@@ -127,6 +127,30 @@ def test_tiled_level_matches_oracle(field):
     _tiled_case(field, {"random": 11, "grains": 12, "uniform": 13}[field])
 
 
+# Levels too thin for tiles run the slab kernel; those of at most 2^18 nodes
+# hold the grad-C matrices dense (every test box above), larger ones compact,
+# per wave: a 129 x 257 x 9-node box (298,809 nodes) takes the compact form.
+def test_large_slab_level_compact_gradc_matches_oracle():
+    el = _el()
+    rng = np.random.default_rng(17)
+    n_cell = (128, 256, 8)
+    shape = (9, 257, 129)
+    dx = (1.0 / 128, 2.0 / 256, 0.0625 / 8)
+    C = grain_C(shape, rng, ngrains=20)
+    bc = rand_bc(rng)
+    u = rng.standard_normal((3,) + shape)
+    b = rng.standard_normal((3,) + shape)
+    op = el.ElasticOperator(n_cell, dx, M.MAJORMINOR)
+    op.SetBC(bc)
+    op.SetModel(dev(C))
+    assert relerr(host(op.Fapply(dev(u))), O.fapply(u, C, M.MAJORMINOR, bc, dx)) < REL
+    op.prepare()
+    xd = dev(u)
+    op.Fsmooth(xd, dev(b), nsmooth=1)
+    diag = O.diagonal(C, M.MAJORMINOR, bc, dx, shape)
+    assert relerr(host(xd), O.fsmooth(u, b, C, M.MAJORMINOR, bc, dx, diag, nsmooth=1)) < REL
+
+
 # The k chunks are sized for ALAMO_EL_TILE_TARGET tile workgroups (read when
 # C is set; by default about two rounds of resident workgroups, a different
 # chunk length in FP64 and FP32).  The 35 x 33 x 39 interior has 10 tiles a

@@ -200,7 +200,9 @@ __global__ void __launch_bounds__(kBlock)
             // kernel then waits on memory twice (these, then D), not once per
             // group of loads the compiler would otherwise interleave.
             const unsigned o32 = (unsigned)id;
-            const int dwb = dwbase[ib * kWavesPerBlock + threadIdx.x / 64];
+            // wave-uniform; readfirstlane (convergent) keeps the load ahead of
+            // the flagged-lanes branch, where the compiler would otherwise sink it
+            const int dwb = __builtin_amdgcn_readfirstlane(dwbase[ib * kWavesPerBlock + threadIdx.x / 64]);
             const unsigned gm = gmask[id];  // the compact-D launch always passes the mask
             T uv[3][19], cv[21], xv[3], bv[3], dv[3];
             mm_load<T>(x, C, L, o32, uv, cv);

@@ -1318,19 +1318,22 @@ namespace {
 
 // Flexible GMRES(m), right preconditioned by one multigrid cycle in the
 // chosen precision (the cycle is a nonlinear, changing preconditioner, so
-// the preconditioned directions Z are stored).  Modified Gram-Schmidt with
-// dot products reduced on the host.  The Krylov basis is FP64.
+// the preconditioned directions Z are stored).  Classical Gram-Schmidt, one
+// multi-vector dot kernel per pass, with the DGKS test deciding a second pass.
+// The Krylov basis is FP64, its vectors MP apart (M0 rounded up to whole
+// 128-byte lines, so that every vector starts a line).
 int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const alamo_el_opts& o, int& iters,
                  double& resnorm, int& learned_taken, hipStream_t s) {
     Level& l0 = h->lv[0];
     const long M0 = 3 * l0.L.N;
+    const long MP = (M0 + 15) / 16 * 16;
     const int m = std::max(2, std::min(o.restart, 100));
     if (h->krylov_m < m) {
         for (double* p : {h->kv, h->kz, h->kw, h->kh, h->kpart})
             if (p) (void)hipFree(p);
         h->kv = h->kz = h->kw = h->kh = h->kpart = nullptr;
-        ALLOC(h->kv, (long)(m + 1) * M0);
-        ALLOC(h->kz, (long)m * M0);
+        ALLOC(h->kv, (long)(m + 1) * MP);
+        ALLOC(h->kz, (long)m * MP);
         ALLOC(h->kw, M0);
         ALLOC(h->kh, 2 * (m + 1) + 2);  // two CGS passes of coefficients, |w|^2 after and before
         ALLOC(h->kpart, (long)(m + 1) * kDotBlocks);
@@ -1366,8 +1369,8 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         g[0] = beta;
         int j = 0;
         for (; j < m && iters < o.max_iter; ++j) {
-            double* Vj = h->kv + (long)j * M0;
-            double* Zj = h->kz + (long)j * M0;
+            double* Vj = h->kv + (long)j * MP;
+            double* Zj = h->kz + (long)j * MP;
             // Z_j = cycle(V_j)
             if (fp32) {
                 ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(Vj, l0.f.res, M0);
@@ -1397,11 +1400,11 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                 double* hp = h->kh + (long)pass * (m + 1);
                 for (int c0 = 0; c0 <= j; c0 += ael::kMultiDotMax) {
                     const int kc = std::min(ael::kMultiDotMax, j + 1 - c0);
-                    ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv + (long)c0 * M0, kc, M0, h->kw,
+                    ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv + (long)c0 * MP, kc, M0, MP, h->kw,
                                                                           h->kpart + (long)c0 * kDotBlocks);
                 }
                 ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, hp);
-                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, hp, -1.0);
+                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, MP, hp, -1.0);
                 sq_norm(nrm_after);
                 HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (2 * (m + 1) + 2),
                                       hipMemcpyDeviceToHost, s));
@@ -1414,7 +1417,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             }
             const double hn = std::sqrt(hn2);
             H[(j + 1) * m + j] = hn;
-            double* Vn = h->kv + (long)(j + 1) * M0;
+            double* Vn = h->kv + (long)(j + 1) * MP;
             ael::k_scale_copy<double><<<nblk(M0), kBlock, 0, s>>>(h->kw, hn > 0 ? 1.0 / hn : 0.0, Vn, M0);
             for (int i = 0; i < j; ++i) {
                 const double a = H[i * m + j], c = H[(i + 1) * m + j];
@@ -1443,7 +1446,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             for (int q = i + 1; q < k; ++q) acc -= H[i * m + q] * y[q];
             y[i] = acc / H[i * m + i];
         }
-        for (int i = 0; i < k; ++i) launch_axpy<double>(x, y[i], h->kz + (long)i * M0, M0, s);
+        for (int i = 0; i < k; ++i) launch_axpy<double>(x, y[i], h->kz + (long)i * MP, M0, s);
         launch_residual<double>(h, l0, x, b, l0.d.C, l0.d.res, s);
         if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 0, &resnorm, s))) return rc;
         if (std::isnan(resnorm)) return fail(ALAMO_EL_ENAN, "NaN in the residual");

@@ -1178,19 +1178,37 @@ __global__ void __launch_bounds__(kBlock)
 // M, stride M).  Each thread reads w[q] once and keeps the k running sums in
 // registers, so one pass streams V and w once; the block then reduces each
 // sum through the waves (shuffles) and LDS.
-constexpr int kMultiDotMax = 32;
-__global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __restrict__ V, int k, long M,
+constexpr int kMultiDotMax = 12;
+__global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __restrict__ V, int k, long M, long ld,
                                                              const double* __restrict__ w,
                                                              double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64][kMultiDotMax];
     double acc[kMultiDotMax];
 #pragma unroll
     for (int i = 0; i < kMultiDotMax; ++i) acc[i] = 0.0;
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < M; q += (long)gridDim.x * blockDim.x) {
+    // four grid strides per step with independent loads (the accumulation
+    // order is the plain loop's), then the remainder
+    const long st = (long)gridDim.x * blockDim.x;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q + 3 * st < M; q += 4 * st) {
+        double wq[4], vq[kMultiDotMax][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) wq[u] = w[q + u * st];
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax; ++i)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax; ++i)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i < k) acc[i] += vq[i][u] * wq[u];
+    }
+    for (; q < M; q += st) {
         const double wq = w[q];
 #pragma unroll
         for (int i = 0; i < kMultiDotMax; ++i)
-            if (i < k) acc[i] += V[(long)i * M + q] * wq;
+            if (i < k) acc[i] += V[(long)i * ld + q] * wq;
     }
     const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
 #pragma unroll
@@ -1228,11 +1246,11 @@ __global__ void __launch_bounds__(kBlock) k_multidot_final(const double* __restr
 
 // w -= sum_i h[i] V_i  (h on the device), then optionally y += sum_i g[i] Z_i.
 __global__ void __launch_bounds__(kBlock) k_multiaxpy(double* __restrict__ w, const double* __restrict__ V, int k,
-                                                      long M, const double* __restrict__ h, double sign) {
+                                                      long M, long ld, const double* __restrict__ h, double sign) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= M) return;
     double acc = w[q];
-    for (int i = 0; i < k; ++i) acc += sign * h[i] * V[(long)i * M + q];
+    for (int i = 0; i < k; ++i) acc += sign * h[i] * V[(long)i * ld + q];
     w[q] = acc;
 }
 

@@ -127,9 +127,9 @@ def test_tiled_level_matches_oracle(field):
     _tiled_case(field, {"random": 11, "grains": 12, "uniform": 13}[field])
 
 
-# Levels too thin for tiles run the slab kernel; those of at most 2^18 nodes
-# hold the grad-C matrices dense (every test box above), larger ones compact,
-# per wave: a 129 x 257 x 9-node box (298,809 nodes) takes the compact form.
+# Levels too thin for tiles run the slab kernel with the compact grad-C
+# matrices in slab order: a 129 x 257 x 9-node box (298,809 nodes, many
+# waves per slab) checks that path on a large level.
 def test_large_slab_level_compact_gradc_matches_oracle():
     el = _el()
     rng = np.random.default_rng(17)

@@ -246,15 +246,23 @@ __global__ void __launch_bounds__(TS::NT)
     const int k0 = 1 + kc * G.kc, k1 = min(k0 + G.kc, L.nz - 1);  // planes [k0, k1)
     const unsigned N = (unsigned)L.N, sxy = (unsigned)(L.nx * L.ny);
     const unsigned col = (unsigned)(j * L.nx + i);
+    // Even chunks march down from k1 - 1, odd chunks up from k0: two adjacent
+    // chunks then read the planes around their common boundary at the same
+    // time (both at the start of their march, or both at its end), and the
+    // second read of those planes hits L2.  Step kk centres on plane
+    // kfirst + d kk.
+    const int d = (kc & 1) ? 1 : -1;
+    const int kfirst = d > 0 ? k0 : k1 - 1, nstep = k1 - k0;
 
-    // prologue: planes k0-1, k0, k0+1 into ring slots 0, 1, 2; the chunk's
-    // gradient masks (packed, plane kk in bits 4 kk .. 4 kk + 3) and, in lane kk of
-    // each wave, the first compact D position of plane kk
+    // prologue: planes kfirst - d, kfirst, kfirst + d (behind, centre, ahead)
+    // into ring slots 0, 1, 2; the chunk's gradient masks (packed, step kk in
+    // bits 4 kk .. 4 kk + 3) and, in lane kk of each wave, the first compact D
+    // position of the plane of step kk
     {
         T pre[TS::PRE];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            tile_fetch<TS, T>(x, L, it, jt, k0 - 1 + s, pre);
+            tile_fetch<TS, T>(x, L, it, jt, kfirst + d * (s - 1), pre);
             tile_store<TS, T>(su[s], pre);
         }
     }
@@ -264,12 +272,12 @@ __global__ void __launch_bounds__(TS::NT)
         if (valid) {
 #pragma unroll
             for (int kk = 0; kk < kMaxChunk; ++kk)
-                if (kk < G.kc && k0 + kk < k1)
-                    gpk |= (unsigned long long)(gmask[(unsigned)(k0 + kk) * sxy + col] & 0xfu) << (4 * kk);
+                if (kk < nstep)
+                    gpk |= (unsigned long long)(gmask[(unsigned)(kfirst + d * kk) * sxy + col] & 0xfu) << (4 * kk);
         }
         const int lane = (int)(threadIdx.x & 63u);
-        if (lane < G.kc && k0 + lane < k1)
-            dwb_lane = dwbase[tile_wave_plane(G, k0 + lane, it, jt, (int)(threadIdx.x / 64), TS::WPB)];
+        if (lane < nstep)
+            dwb_lane = dwbase[tile_wave_plane(G, kfirst + d * lane, it, jt, (int)(threadIdx.x / 64), TS::WPB)];
     }
     // Every prologue load has landed (dwb_lane included): without this the
     // compiler cannot prove dwb_lane complete inside the loop and waits for
@@ -283,10 +291,11 @@ __global__ void __launch_bounds__(TS::NT)
     const T ixy = T(0.25) * ihx * ihy, ixz = T(0.25) * ihx * ihz, iyz = T(0.25) * ihy * ihz;
     const int c0 = (ty + 1) * kHX + (tx + 1);  // centre position in the halo tile
 
-    for (int k = k0; k < k1; ++k) {
-        const int kk = k - k0;
-        const int sm = kk % 3, sc = (kk + 1) % 3, sp = (kk + 2) % 3;  // slots of k-1, k, k+1
-        const bool more = k + 1 < k1;
+    for (int kk = 0; kk < nstep; ++kk) {
+        const int k = kfirst + d * kk;
+        const int sb = kk % 3, sc = (kk + 1) % 3, sa = (kk + 2) % 3;  // slots behind, centre, ahead
+        const int sm = d > 0 ? sb : sa, sp = d > 0 ? sa : sb;         // slots of planes k-1 and k+1
+        const bool more = kk + 1 < nstep;
         const unsigned o = (unsigned)k * sxy + col;
         const T* cblk = Ct + TileC<TS>::block(G, k, it, jt);  // this (tile, plane) of the C copy
 
@@ -312,7 +321,7 @@ __global__ void __launch_bounds__(TS::NT)
                 for (int q = 0; q < 18; ++q) dv[q] = ld32(Dc + (size_t)q * nflag, pidx);
             }
         }
-        if (more) tile_fetch<TS, T>(x, L, it, jt, k + 2, pre);
+        if (more) tile_fetch<TS, T>(x, L, it, jt, k + 2 * d, pre);
         __builtin_amdgcn_sched_barrier(0);
 
         if (valid) {
@@ -387,8 +396,8 @@ __global__ void __launch_bounds__(TS::NT)
             }
         }
         if (more) {
-            __syncthreads();  // every read of slot sm (plane k-1) is done
-            tile_store<TS, T>(su[sm], pre);
+            __syncthreads();  // every read of slot sb (the plane behind) is done
+            tile_store<TS, T>(su[sb], pre);
             __syncthreads();
         }
     }

@@ -78,21 +78,36 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const bf16_t* __restrict__ X, 
     f32x4 acc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        const int k = k0 + grp * 8;
-        uint4 a = make_uint4(0, 0, 0, 0);
-        if (vin && k < K) {
-            const int t = k / Cin, c = k - t * Cin;
-            int nbr;
-            if (tap_node(g, x, y, z, t, nbr)) a = *reinterpret_cast<const uint4*>(X + (size_t)nbr * Cin + c);
-        }
-        const bf16x8 A = __builtin_bit_cast(bf16x8, a);
+    // KU steps of 32 along K per round: every A and B load of the round is
+    // issued before its MFMAs (one wait per round instead of one per step;
+    // the small levels have few waves to hide it), the MFMAs in step order.
+    constexpr int KU = 4;
+    for (int k0 = 0; k0 < K; k0 += 32 * KU) {
+        uint4 a[KU], b[KU][NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int n = nbase + nb * 16 + row;
-            uint4 b = make_uint4(0, 0, 0, 0);
-            if (k < K) b = *reinterpret_cast<const uint4*>(Wt + (size_t)n * K + k);
-            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, __builtin_bit_cast(bf16x8, b), acc[nb], 0, 0, 0);
+        for (int u = 0; u < KU; ++u) {
+            const int k = k0 + 32 * u + grp * 8;
+            a[u] = make_uint4(0, 0, 0, 0);
+            if (vin && k < K) {
+                const int t = k / Cin, c = k - t * Cin;
+                int nbr;
+                if (tap_node(g, x, y, z, t, nbr)) a[u] = *reinterpret_cast<const uint4*>(X + (size_t)nbr * Cin + c);
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int n = nbase + nb * 16 + row;
+                b[u][nb] = make_uint4(0, 0, 0, 0);
+                if (k < K) b[u][nb] = *reinterpret_cast<const uint4*>(Wt + (size_t)n * K + k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            if (k0 + 32 * u >= K) break;
+            const bf16x8 A = __builtin_bit_cast(bf16x8, a[u]);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, __builtin_bit_cast(bf16x8, b[u][nb]), acc[nb], 0,
+                                                                 0, 0);
         }
     }
 #pragma unroll

@@ -1446,7 +1446,11 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             for (int q = i + 1; q < k; ++q) acc -= H[i * m + q] * y[q];
             y[i] = acc / H[i * m + i];
         }
-        for (int i = 0; i < k; ++i) launch_axpy<double>(x, y[i], h->kz + (long)i * MP, M0, s);
+        // one pass over x and Z (the accumulation order of k axpys: x + y_0 z_0 + ...)
+        if (k > 0) {
+            HIPCHK(hipMemcpyAsync(h->kh, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+            ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(x, h->kz, k, M0, MP, h->kh, 1.0);
+        }
         launch_residual<double>(h, l0, x, b, l0.d.C, l0.d.res, s);
         if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 0, &resnorm, s))) return rc;
         if (std::isnan(resnorm)) return fail(ALAMO_EL_ENAN, "NaN in the residual");

@@ -23,6 +23,44 @@ __device__ __forceinline__ bool ghost_node(const Lvl& L, const Patch& P, int i, 
            (P.gl[2] && k == 0) || (P.gh[2] && k == L.nz - 1);
 }
 
+// Integrator::Mechanics::TagCellsForRefinement (src/Integrator/Mechanics.H:298-313):
+// node (i, j, k) tags cell (i, j, k) when |grad eta_n| |dx| > thr for some
+// component n, grad central inside the box and one-sided on its faces
+// (Numeric::Gradient with GetStencil on the box, as d1), |dx| the 2-norm of
+// the cell size.  Only the box's cells are kept (the top nodes' tags fall
+// outside it).  The tagged cells' bounding box and count accumulate with
+// atomics into bb[0..2] (min i, j, k), bb[3..5] (max) and bb[6] (count),
+// initialised by k_tag_init.
+__global__ void __launch_bounds__(kBlock) k_tag_init(int* __restrict__ bb) {
+    if (threadIdx.x < 7) bb[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : (threadIdx.x < 6 ? -1 : 0);
+}
+__global__ void __launch_bounds__(kBlock) k_tag_eta(const double* __restrict__ eta, int ncomp, Lvl L, double thr,
+                                                    int* __restrict__ bb) {
+#pragma clang fp contract(off)
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    if (i >= L.nx - 1 || j >= L.ny - 1 || k >= L.nz - 1) return;
+    const double dxn = sqrt(L.dx * L.dx + L.dy * L.dy + L.dz * L.dz);
+    const long sy = L.nx, sz = (long)L.nx * L.ny;
+    bool tag = false;
+    for (int n = 0; n < ncomp; ++n) {
+        const double* f = eta + (long)n * L.N;
+        const double g0 = d1(f, id, 1, i, L.nx, L.dx), g1 = d1(f, id, sy, j, L.ny, L.dy),
+                     g2 = d1(f, id, sz, k, L.nz, L.dz);
+        tag |= sqrt(g0 * g0 + g1 * g1 + g2 * g2) * dxn > thr;
+    }
+    if (!tag) return;
+    atomicMin(bb + 0, i);
+    atomicMin(bb + 1, j);
+    atomicMin(bb + 2, k);
+    atomicMax(bb + 3, i);
+    atomicMax(bb + 4, j);
+    atomicMax(bb + 5, k);
+    atomicAdd(bb + 6, 1);
+}
+
 // v = 0 on ghost nodes (mode 0) or v = src on ghost nodes (mode 1).
 __global__ void __launch_bounds__(kBlock) k_patch_ghost(double* __restrict__ v, const double* __restrict__ src, Lvl L,
                                                         Patch P, int mode) {

@@ -1615,6 +1615,27 @@ int alamo_amr_patch_create(alamo_el_t parent, const int clo[3], const int chi[3]
     return 0;
 }
 
+int alamo_el_tag_bbox(alamo_el_t h, const double* eta, int ncomp, double threshold, int buffer, int clo[3],
+                      int chi[3], long long* ntag, hipStream_t s) {
+    if (!h || !eta || ncomp < 1 || !clo || !chi || !ntag || buffer < 0) return fail(ALAMO_EL_EINVAL, "bad argument");
+    const Lvl L = h->lv[0].L;
+    int* bb = reinterpret_cast<int*>(h->red_partial);
+    ael::k_tag_init<<<1, kBlock, 0, s>>>(bb);
+    ael::k_tag_eta<<<nblk(L.N), kBlock, 0, s>>>(eta, ncomp, L, threshold, bb);
+    HIPCHK(hipGetLastError());
+    int out[7];
+    HIPCHK(hipMemcpyAsync(out, bb, sizeof(out), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *ntag = out[6];
+    if (out[6] == 0) return 0;
+    const int nc[3] = {L.nx - 1, L.ny - 1, L.nz - 1};
+    for (int a = 0; a < 3; ++a) {
+        clo[a] = std::max(out[a] - buffer, 0);
+        chi[a] = std::min(out[3 + a] + 1 + buffer, nc[a]);
+    }
+    return 0;
+}
+
 int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost_hi[3]) {
     if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
     const int o[3] = {h->amr.ox, h->amr.oy, h->amr.oz};

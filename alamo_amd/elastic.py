@@ -61,6 +61,7 @@ EXPORTS = [
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
+    "alamo_el_tag_bbox",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -96,6 +97,9 @@ def lib():
         L.alamo_el_bc_init.argtypes = [vp, vp, ctypes.c_int, vp]
         L.alamo_el_newton_rhs.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
         L.alamo_el_stress.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.alamo_el_tag_bbox.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_longlong), vp]
         L.alamo_el_face_traction.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double), vp]
         L.alamo_el_mix_models.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int, vp,
@@ -347,6 +351,24 @@ class ElasticOperator:
         _check(lib().alamo_el_face_traction(self._h, self._v(sigma, 0, 9), self._v(u, 0), int(axis), int(side), trac,
                                             disp, _stream()), "alamo_el_face_traction")
         return np.array(trac[:]), np.array(disp[:])
+
+    def tag_box(self, eta, threshold: float = 0.1, buffer: int = 2):
+        """Integrator::Mechanics::TagCellsForRefinement on the finest level
+        (src/Integrator/Mechanics.H:298-313): ((clo, chi), ntag) with [clo, chi]
+        the tagged cells' bounding box grown by `buffer` cells (level cell
+        indices), or (None, 0) when no cell is tagged.  eta: (ncomp,) + level
+        shape, float64 on the GPU."""
+        shape = self.level_shape(0)
+        if eta.dim() != 4 or tuple(eta.shape[1:]) != shape:
+            raise AlamoError(f"tag_box: expected eta of shape (ncomp,) + {shape}, got {tuple(eta.shape)}")
+        clo = (ctypes.c_int * 3)()
+        chi = (ctypes.c_int * 3)()
+        n = ctypes.c_longlong()
+        _check(lib().alamo_el_tag_bbox(self._h, _ptr(eta), int(eta.shape[0]), float(threshold), int(buffer), clo,
+                                       chi, ctypes.byref(n), _stream()), "alamo_el_tag_bbox")
+        if n.value == 0:
+            return None, 0
+        return (tuple(clo[:]), tuple(chi[:])), int(n.value)
 
     def norm_inf(self, v, mglev: int = 0) -> float:
         out = ctypes.c_double()

@@ -239,6 +239,17 @@ int alamo_el_learned_guess(alamo_el_t h, const double* b, double* x, hipStream_t
  * Operator::reflux, fixUpResidualMask and averageDownSolutionRHS
  * (src/Operator/Operator.cpp:230-251, 436-448, 526-633) with MLMG's sweep over
  * AMR levels. --- */
+/* Integrator::Mechanics::TagCellsForRefinement on the finest level of h
+ * (src/Integrator/Mechanics.H:298-313): cell (i, j, k) is tagged when
+ * |grad eta_n| |dx| > threshold at node (i, j, k) for some component n of eta
+ * (ncomp device planes of the level's nodes; the gradient central inside and
+ * one-sided on the box faces).  *ntag receives the number of tagged cells and,
+ * when it is not 0, [clo, chi] the tagged cells' bounding box grown by buffer
+ * cells and clipped to the level, in the level's cell indices (the one box per
+ * level that alamo_amr_patch_create takes; the clustering of tags into several
+ * boxes by AmrCore is not restated).  Synchronizes the stream. */
+int alamo_el_tag_bbox(alamo_el_t h, const double* eta, int ncomp, double threshold, int buffer, int clo[3],
+                      int chi[3], long long* ntag, hipStream_t stream);
 /* Patch of the next AMR level covering the parent's cells [clo, chi] (global
  * parent-level cell indices, clo < chi <= domain cells).  The new handle holds
  * the fine nodes 2 clo .. 2 chi plus one ghost node layer on each face inside

@@ -224,3 +224,29 @@ def face_traction(sigma, u, dx, axis=0, side=1):
     node[axis], node[b], node[c] = ia, max(n[b] - 2, 0), max(n[c] - 2, 0)
     disp = np.array([u[r, node[2], node[1], node[0]] for r in range(3)])
     return trac, disp
+
+
+def tag_box(eta, dx, threshold=0.1, buffer=2):
+    """Integrator::Mechanics::TagCellsForRefinement (src/Integrator/Mechanics.H:298-313):
+    node (i, j, k) tags cell (i, j, k) when |grad eta_n| |dx| > threshold for
+    some component n, the gradient central inside the box and one-sided on its
+    faces (numpy.gradient, edge_order=1), |dx| the 2-norm of the cell size;
+    the box's cells only.  Returns ((clo, chi), ntag) with the tagged cells'
+    bounding box grown by `buffer` cells and clipped, or (None, 0).
+    eta: (ncomp, nz, ny, nx)."""
+    eta = np.asarray(eta, dtype=np.float64)
+    h = _dx(dx)
+    dxn = np.sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2])
+    tags = np.zeros(eta.shape[1:], dtype=bool)
+    for n in range(eta.shape[0]):
+        g = [np.gradient(eta[n], h[a], axis=2 - a, edge_order=1) for a in range(3)]
+        tags |= np.sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) * dxn > threshold
+    tags = tags[:-1, :-1, :-1]
+    ntag = int(tags.sum())
+    if ntag == 0:
+        return None, 0
+    k, j, i = np.nonzero(tags)
+    nc = (eta.shape[3] - 1, eta.shape[2] - 1, eta.shape[1] - 1)
+    clo = tuple(max(int(v.min()) - buffer, 0) for v in (i, j, k))
+    chi = tuple(min(int(v.max()) + 1 + buffer, nc[a]) for a, v in enumerate((i, j, k)))
+    return (clo, chi), ntag

@@ -452,3 +452,39 @@ def test_face_traction_closed_forms():
     exact = L[1] * L[2] + 2.0 * L[1] ** 2 / 2 * L[2] + 3.0 * L[2] ** 2 / 2 * L[1]
     assert abs(tr[0] - exact) < 1e-12 * abs(exact) and tr[1] == 0.0 and tr[2] == 0.0
     assert np.array_equal(dp, u[:, shape[0] - 2, shape[1] - 2, shape[2] - 1])
+
+
+def test_refinement_tagging_pins():
+    """The oracle's TagCellsForRefinement (src/Integrator/Mechanics.H:298-313):
+    a linear eta has the exact gradient everywhere (central and one-sided), so
+    every cell is tagged just above the threshold and none just below; the
+    nodal gradient agrees with a direct loop over the reference's stencils
+    (src/Numeric/Stencil.H:70-191) on a random field; one tagged node gives
+    its cell grown by the buffer and clipped."""
+    shape = (5, 6, 7)  # nz, ny, nx
+    dx = (0.5, 0.25, 0.2)
+    dxn = np.sqrt(sum(d * d for d in dx))
+    z, y, x = np.meshgrid(np.arange(shape[0]) * dx[2], np.arange(shape[1]) * dx[1], np.arange(shape[2]) * dx[0],
+                          indexing="ij")
+    slope = np.array([0.3, -0.4, 1.2])
+    eta = (slope[0] * x + slope[1] * y + slope[2] * z)[None]
+    thr = np.linalg.norm(slope) * dxn
+    box, n = O.tag_box(eta, dx, threshold=thr * (1 - 1e-9), buffer=0)
+    assert n == 4 * 5 * 6 and box == ((0, 0, 0), (6, 5, 4))
+    assert O.tag_box(eta, dx, threshold=thr * (1 + 1e-9))[1] == 0
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal(shape)
+    n3 = (shape[2], shape[1], shape[0])
+    for a, ax in enumerate((2, 1, 0)):
+        g = np.gradient(f, dx[a], axis=ax, edge_order=1)
+        for idx in np.ndindex(*shape):
+            p = idx[ax]
+            up = list(idx); dn = list(idx)
+            up[ax] = min(p + 1, n3[a] - 1); dn[ax] = max(p - 1, 0)
+            h = dx[a] if p in (0, n3[a] - 1) else 2 * dx[a]
+            assert abs(g[idx] - (f[tuple(up)] - f[tuple(dn)]) / h) <= 1e-14 * (1 + abs(g[idx]))
+    spike = np.zeros((1,) + shape)
+    spike[0, 2, 3, 4] = 1.0  # the cells around node (4, 3, 2) see its gradient
+    box, n = O.tag_box(spike, dx, threshold=1e-6, buffer=1)
+    assert n > 0 and all(box[0][a] >= 0 and box[1][a] <= (6, 5, 4)[a] for a in range(3))
+    assert box[0] == (2, 1, 0) and box[1] == (6, 5, 4)

@@ -73,10 +73,26 @@ __global__ void __launch_bounds__(kBlock) k_patch_ghost(double* __restrict__ v, 
     for (int p = 0; p < 3; ++p) v[p * L.N + id] = mode ? src[p * L.N + id] : 0.0;
 }
 
-// Trilinear interpolation of the parent field c (level Lc, whole parent
-// handle, parent origin po) onto the patch handle field f:
+// Trilinear value of component p of the parent field c (level Lc, whole
+// parent handle, parent origin po) at the global fine node (gx, gy, gz)
+// (src/Operator/Operator.cpp:365-434 weights; node_bilinear_interp).
+__device__ __forceinline__ double parent_trilinear(const double* __restrict__ c, const Lvl& Lc, int pox, int poy,
+                                                   int poz, int gx, int gy, int gz, int p) {
+    const int xs[2] = {(gx >> 1) - pox, ((gx + 1) >> 1) - pox};
+    const int ys[2] = {(gy >> 1) - poy, ((gy + 1) >> 1) - poy};
+    const int zs[2] = {(gz >> 1) - poz, ((gz + 1) >> 1) - poz};
+    const int nxs = (gx & 1) ? 2 : 1, nys = (gy & 1) ? 2 : 1, nzs = (gz & 1) ? 2 : 1;
+    const double w = 1.0 / (double)(nxs * nys * nzs);
+    double acc = 0.0;
+    for (int a = 0; a < nzs; ++a)
+        for (int b = 0; b < nys; ++b)
+            for (int d = 0; d < nxs; ++d) acc += c[p * Lc.N + ((long)zs[a] * Lc.ny + ys[b]) * Lc.nx + xs[d]];
+    return acc * w;
+}
+
+// Trilinear interpolation of the parent field c onto the patch handle field f:
 // mode 0: f = P c everywhere, mode 1: f += P c everywhere, mode 2: f = P c on
-// ghost nodes only (src/Operator/Operator.cpp:365-434 weights).
+// ghost nodes only.
 __global__ void __launch_bounds__(kBlock) k_patch_interp(const double* __restrict__ c, Lvl Lc, int pox, int poy,
                                                          int poz, double* __restrict__ f, Lvl Lf, Patch P, int mode) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,22 +100,34 @@ __global__ void __launch_bounds__(kBlock) k_patch_interp(const double* __restric
     int i, j, k;
     node_ijk(id, Lf, i, j, k);
     if (mode == 2 && !ghost_node(Lf, P, i, j, k)) return;
-    const int gx = P.ox + i, gy = P.oy + j, gz = P.oz + k;  // global fine node
-    const int xs[2] = {(gx >> 1) - pox, ((gx + 1) >> 1) - pox};
-    const int ys[2] = {(gy >> 1) - poy, ((gy + 1) >> 1) - poy};
-    const int zs[2] = {(gz >> 1) - poz, ((gz + 1) >> 1) - poz};
-    const int nxs = (gx & 1) ? 2 : 1, nys = (gy & 1) ? 2 : 1, nzs = (gz & 1) ? 2 : 1;
-    const double w = 1.0 / (double)(nxs * nys * nzs);
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-        double acc = 0.0;
-        for (int a = 0; a < nzs; ++a)
-            for (int b = 0; b < nys; ++b)
-                for (int d = 0; d < nxs; ++d) acc += c[p * Lc.N + ((long)zs[a] * Lc.ny + ys[b]) * Lc.nx + xs[d]];
-        acc *= w;
+        const double acc = parent_trilinear(c, Lc, pox, poy, poz, P.ox + i, P.oy + j, P.oz + k, p);
         double* q = f + p * Lf.N + id;
         *q = mode == 1 ? *q + acc : acc;
     }
+}
+
+// FillPatchTwoLevels with node_bilinear_interp (src/Integrator/Integrator.cpp:
+// 394-446) onto a new patch P of handle shape Lf: a node takes the old patch's
+// value where the old patch's unknowns (global nodes 2 Ps.clo .. 2 Ps.chi of
+// the same AMR level) hold it, else the trilinear interpolation of the parent
+// field c.  Without an old patch (so == nullptr) every node is interpolated:
+// InterpFromCoarseLevel (FillCoarsePatch, :452-478).
+__global__ void __launch_bounds__(kBlock) k_patch_fill(const double* __restrict__ c, Lvl Lc, int pox, int poy,
+                                                       int poz, const double* __restrict__ so, Lvl Ls, Patch Ps,
+                                                       double* __restrict__ f, Lvl Lf, Patch P) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= Lf.N) return;
+    int i, j, k;
+    node_ijk(id, Lf, i, j, k);
+    const int gx = P.ox + i, gy = P.oy + j, gz = P.oz + k;
+    const bool old = so && 2 * Ps.clo[0] <= gx && gx <= 2 * Ps.chi[0] && 2 * Ps.clo[1] <= gy &&
+                     gy <= 2 * Ps.chi[1] && 2 * Ps.clo[2] <= gz && gz <= 2 * Ps.chi[2];
+    const long is = old ? ((long)(gz - Ps.oz) * Ls.ny + (gy - Ps.oy)) * Ls.nx + (gx - Ps.ox) : 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+        f[p * Lf.N + id] = old ? so[p * Ls.N + is] : parent_trilinear(c, Lc, pox, poy, poz, gx, gy, gz, p);
 }
 
 // averageDownSolutionRHS (src/Operator/Operator.cpp:436-448, nodal

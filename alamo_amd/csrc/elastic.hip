@@ -1647,6 +1647,27 @@ int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost
     return 0;
 }
 
+int alamo_amr_fillpatch(alamo_el_t dst, const double* parent_u, alamo_el_t src, const double* src_u, double* dst_u,
+                        hipStream_t s) {
+    if (!dst || !parent_u || !dst_u) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (!dst->amr_parent) return fail(ALAMO_EL_EINVAL, "the destination must be a patch (alamo_amr_patch_create)");
+    if (src) {
+        if (!src_u) return fail(ALAMO_EL_EINVAL, "an old patch needs its field");
+        if (!src->amr_parent) return fail(ALAMO_EL_EINVAL, "the old patch must be a patch (alamo_amr_patch_create)");
+        for (int a = 0; a < 3; ++a)
+            if (src->amr_dom[a] != dst->amr_dom[a])
+                return fail(ALAMO_EL_EINVAL, "the old and new patches must belong to the same AMR level");
+    }
+    const alamo_el_t par = dst->amr_parent;
+    const Lvl& Lf = dst->lv[0].L;
+    const Lvl& Ls = src ? src->lv[0].L : Lf;
+    ael::k_patch_fill<<<nblk(Lf.N), kBlock, 0, s>>>(parent_u, par->lv[0].L, par->amr.ox, par->amr.oy, par->amr.oz,
+                                                     src ? src_u : nullptr, Ls, src ? src->amr : dst->amr, dst_u, Lf,
+                                                     dst->amr);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const double* const* b, double tol_rel,
                     double tol_abs, const alamo_el_opts* opts_in, alamo_el_info* info, hipStream_t s) {
     if (nlev < 1 || !hs || !x || !b) return fail(ALAMO_EL_EINVAL, "bad argument");

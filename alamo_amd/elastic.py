@@ -61,7 +61,7 @@ EXPORTS = [
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
-    "alamo_el_tag_bbox",
+    "alamo_el_tag_bbox", "alamo_amr_fillpatch",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -118,6 +118,7 @@ def lib():
         L.alamo_amr_patch_create.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                              ctypes.POINTER(vp)]
         L.alamo_amr_patch_info.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
+        L.alamo_amr_fillpatch.argtypes = [vp, vp, vp, vp, vp, vp]
         L.alamo_amr_solve.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(Opts), ctypes.POINTER(Info), vp]
         _lib = L
@@ -177,6 +178,7 @@ class ElasticOperator:
         self._bc = _bc.Constant()
         self.parent = None
         self.origin = (0, 0, 0)       # global node index (x, y, z) of node 0
+        self.dom = self.n_cell        # domain cells at this level
         self.ghost_lo = (0, 0, 0)
         self.ghost_hi = (0, 0, 0)
 
@@ -198,6 +200,7 @@ class ElasticOperator:
         p._init_from(h, self.sym, n_cell, np.asarray(self.dx) / 2.0)
         p.parent = self
         p.origin, p.ghost_lo, p.ghost_hi = tuple(o), tuple(gl), tuple(gh)
+        p.dom = tuple(2 * d for d in self.dom)
         p.clo, p.chi = tuple(int(v) for v in lo), tuple(int(v) for v in hi)
         p._bc = self._bc
         return p
@@ -370,6 +373,54 @@ class ElasticOperator:
             return None, 0
         return (tuple(clo[:]), tuple(chi[:])), int(n.value)
 
+    def child_box(self, eta, threshold: float = 0.1, buffer: int = 2, min_cells: int = 2):
+        """Cells [clo, chi] (global, this level) of the next level's patch from
+        this level's tags: tag_box's bounding box clipped so that the patch and
+        its ghost layer lie on this handle's unknowns (the nesting
+        alamo_amr_patch_create requires); None when nothing is tagged or fewer
+        than min_cells cells remain along an axis."""
+        box, _ = self.tag_box(eta, threshold, buffer)
+        if box is None:
+            return None
+        nodes = self.level_shape(0)[::-1]
+        clo, chi = [], []
+        for a in range(3):
+            ulo = self.origin[a] + self.ghost_lo[a]
+            uhi = self.origin[a] + nodes[a] - 1 - self.ghost_hi[a]
+            lo = max(box[0][a] + self.origin[a], ulo + (1 if ulo > 0 else 0))
+            hi = min(box[1][a] + self.origin[a], uhi - (1 if uhi < self.dom[a] else 0))
+            if hi - lo < min_cells:
+                return None
+            clo.append(lo)
+            chi.append(hi)
+        return tuple(clo), tuple(chi)
+
+    def fill_patch(self, parent_u, old=None, old_u=None):
+        """This patch's field after a regrid (alamo_amr_fillpatch;
+        Integrator::FillPatch / FillCoarsePatch, src/Integrator/Integrator.cpp:
+        394-478): the old patch's values where its unknowns hold a node, else
+        the parent field parent_u interpolated trilinearly.  Returns a new
+        (3,) + level shape float64 tensor on the GPU."""
+        if self.parent is None:
+            raise AlamoError("fill_patch: this operator is not a patch")
+        if (old is None) != (old_u is None):
+            raise AlamoError("fill_patch: give both the old patch and its field, or neither")
+        shape = self.level_shape(0)
+        self._check_field(parent_u, self.parent.level_shape(0), "parent_u")
+        if old is not None:
+            self._check_field(old_u, old.level_shape(0), "old_u")
+        out = _torch().empty((3,) + shape, dtype=_torch().float64, device=parent_u.device)
+        _check(lib().alamo_amr_fillpatch(self._h, _ptr(parent_u), old._h if old is not None else None,
+                                         _ptr(old_u) if old is not None else None, _ptr(out), _stream()),
+               "alamo_amr_fillpatch")
+        return out
+
+    @staticmethod
+    def _check_field(v, shape, name):
+        if tuple(v.shape) != (3,) + tuple(shape) or v.dtype != _torch().float64 or not v.is_contiguous():
+            raise AlamoError(f"{name}: expected a contiguous float64 tensor of shape {(3,) + tuple(shape)}, "
+                             f"got {tuple(v.shape)} {v.dtype}")
+
     def norm_inf(self, v, mglev: int = 0) -> float:
         out = ctypes.c_double()
         _check(lib().alamo_el_norm_inf(self._h, mglev, self._v(v, mglev), ctypes.byref(out), _stream()), "alamo_el_norm_inf")
@@ -460,6 +511,36 @@ class Linear:
         if rc == NOT_CONVERGED and self.abort_on_fail:
             raise SolverError(f"MLMG failed to converge: |r| = {self.info.res:.3e} after {self.info.iters} iterations")
         return self.info.res
+
+
+def amr_levels(base, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2):
+    """AMR levels from the device tags (Integrator::Mechanics::
+    TagCellsForRefinement, src/Integrator/Mechanics.H:298-313, one box per
+    level): starting from the base operator, each level tags its cells from
+    eta_of(op), the (ncomp,) + level shape indicator field on op's nodes
+    (float64, GPU), and refines the clipped bounding box (child_box), up to
+    `levels` levels.  Returns [base, patch 1, ...]; set each patch's model."""
+    ops = [base]
+    while len(ops) < levels:
+        box = ops[-1].child_box(eta_of(ops[-1]), threshold, buffer)
+        if box is None:
+            break
+        ops.append(ops[-1].refine(*box))
+    return ops
+
+
+def amr_regrid(old_ops, old_xs, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2):
+    """Regrid (AmrCore::regrid with Integrator::RemakeLevel / MakeNewLevelFromCoarse,
+    src/Integrator/Integrator.cpp:394-478): new patches over the same base from
+    the tags (amr_levels), and each new level's field filled coarse to fine from
+    the old level of the same depth and the already regridded parent
+    (fill_patch).  Returns (ops, xs) with xs[0] = old_xs[0]."""
+    ops = amr_levels(old_ops[0], eta_of, levels, threshold, buffer)
+    xs = [old_xs[0]]
+    for l in range(1, len(ops)):
+        old = old_ops[l] if l < len(old_ops) else None
+        xs.append(ops[l].fill_patch(xs[l - 1], old, old_xs[l] if old is not None else None))
+    return ops, xs
 
 
 def amr_solve(levels, xs, bs, tol_rel: float, tol_abs: float, **kw):

@@ -90,20 +90,11 @@ def config5(fields=4, n_cell=256, weights=None):
     return dict(config=5, workload=f"polycrystal-{n_cell + 1}^3-256g uniaxial 1e-3 z, to 1e-8", fields=rows)
 
 
-def config3(n0=64, seed=0):
+def _amr_models(ops, spheres, width):
+    """Set each level's model and Newton right-hand side for the inclusions."""
     import torch
 
-    from . import bc as B
-    from . import elastic as el
-    from . import model as M
     from . import problems as P
-    spheres = P.random_spheres(3, seed=seed)
-    width = 2.0 / n0
-    boxes = P.amr_hierarchy(n0, spheres, width=width)
-    ops = [el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC)]
-    ops[0].SetBC(B.Constant())
-    for clo, chi in boxes:
-        ops.append(ops[-1].refine(clo, chi))
     bs = []
     for op in ops:
         shape = op.level_shape(0)
@@ -112,10 +103,35 @@ def config3(n0=64, seed=0):
         op.SetModel(Cd)
         b0 = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
         bs.append(op.newton_rhs(Cd, torch.zeros_like(b0), b0, F0=torch.from_numpy(F0).cuda()))
+    return bs
+
+
+def config3(n0=64, seed=0, shift=0.03):
+    """3-level AMR with patches from the device refinement tags, solved in FP64
+    and with FP32 base cycles; then the inclusions move by `shift` along x and
+    the hierarchy is regridded (new tags, FillPatch of the FP64 solution) and
+    solved again from the filled fields and from zero."""
+    import torch
+
+    from . import bc as B
+    from . import elastic as el
+    from . import model as M
+    from . import problems as P
+    spheres = P.random_spheres(3, seed=seed)
+    width = 2.0 / n0
+
+    def eta_of(sph):
+        return lambda op: torch.from_numpy(P.sphere_indicators(op.origin, op.level_shape(0), op.dx[0], sph,
+                                                               width)).cuda()
+
+    base = el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC)
+    base.SetBC(B.Constant())
+    ops, dt_tag = _timed(lambda: el.amr_levels(base, eta_of(spheres), levels=3))
+    bs = _amr_models(ops, spheres, width)
     out = dict(config=3, workload=f"3-level AMR, base {n0 + 1}^3, 3 inclusions (seed {seed})",
                spheres=[dict(center=c, radius=r, E=E) for c, r, E in spheres],
-               patches=[dict(clo=list(lo), chi=list(hi)) for lo, hi in boxes],
-               level_nodes=[int(np.prod(op.level_shape(0))) for op in ops])
+               patches=[dict(clo=list(op.clo), chi=list(op.chi)) for op in ops[1:]],
+               level_nodes=[int(np.prod(op.level_shape(0))) for op in ops], tag_and_refine_seconds=dt_tag)
     sols = {}
     for prec in ("fp64", "fp32"):
         xs = [torch.zeros_like(b) for b in bs]
@@ -126,6 +142,21 @@ def config3(n0=64, seed=0):
         _log(f"config 3 {prec}: {dt:.3f}s, {info.iters} composite V-cycles, converged={info.status == 0}")
     out["fp32_vs_fp64_rel_l2"] = [float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
                                   for a, b in zip(sols["fp32"], sols["fp64"])]
+    # the inclusions move: regrid from new tags, fill from the old solution
+    moved = [((c[0] + shift, c[1], c[2]), r, E) for c, r, E in spheres]
+    (ops2, xs2), dt_regrid = _timed(lambda: el.amr_regrid(ops, sols["fp64"], eta_of(moved), levels=3))
+    bs2 = _amr_models(ops2, moved, width)
+    reg = dict(shift=shift, patches=[dict(clo=list(op.clo), chi=list(op.chi)) for op in ops2[1:]],
+               regrid_seconds=dt_regrid)
+    for name, x0 in (("filled", xs2), ("zero", [torch.zeros_like(b) for b in bs2])):
+        xs = [v.clone() for v in x0]
+        info, dt = _timed(lambda: el.amr_solve(ops2, xs, bs2, 1e-8, 0.0, max_iter=400, cycle="V", precision="fp32",
+                                               abort_on_fail=False))
+        reg[name] = dict(seconds=dt, iters=info.iters, converged=info.status == 0, res=info.res, res0=info.res0,
+                         rhsnorm=info.rhsnorm)
+        _log(f"config 3 regrid, from {name}: {dt:.3f}s, {info.iters} composite V-cycles, "
+             f"|r0| {info.res0:.3e}, |b| {info.rhsnorm:.3e}")
+    out["regrid"] = reg
     return out
 
 

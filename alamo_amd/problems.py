@@ -101,58 +101,13 @@ def random_spheres(n=3, seed=0, r_range=(0.05, 0.15), E_range=(2.0, 20.0)):
     return out
 
 
-def tag_box(n_cell, level_dx, origin_cells, box_cells, spheres, threshold=0.1, buffer=2, width=0.0):
-    """Cells of one level tagged by |grad eta| * |dx| > threshold
-    (src/Integrator/Mechanics.H:298-313) for the sharp sphere indicator,
-    searched inside box_cells = (lo, hi) (global cells), returned as the
-    bounding box (clo, chi) grown by `buffer` cells and clipped to the box,
-    or None when nothing is tagged.  One box per level: the clustering of
-    tags into several boxes (AmrCore) is not restated."""
-    lo, hi = box_cells
-    shape = tuple(hi[a] - lo[a] + 1 for a in (2, 1, 0))  # nodes of the searched cells
-    eta = np.zeros(shape)
-    x = (lo[0] + np.arange(shape[2])) * level_dx
-    y = (lo[1] + np.arange(shape[1])) * level_dx
-    z = (lo[2] + np.arange(shape[0])) * level_dx
-    for center, radius, _ in spheres:
-        eta = np.maximum(eta, sphere_eta(x[None, None, :], y[None, :, None], z[:, None, None], center, radius,
-                                         width))
-    # cell gradient from the 8 corner nodes, times |dx| = sqrt(3) dx
-    gx = (eta[:, :, 1:] - eta[:, :, :-1])[:-1, :-1, :] + (eta[:, :, 1:] - eta[:, :, :-1])[1:, 1:, :]
-    gy = (eta[:, 1:, :] - eta[:, :-1, :])[:-1, :, :-1] + (eta[:, 1:, :] - eta[:, :-1, :])[1:, :, 1:]
-    gz = (eta[1:, :, :] - eta[:-1, :, :])[:, :-1, :-1] + (eta[1:, :, :] - eta[:-1, :, :])[:, 1:, 1:]
-    grad = np.sqrt((gx / 2) ** 2 + (gy / 2) ** 2 + (gz / 2) ** 2) / level_dx
-    tags = grad * np.sqrt(3.0) * level_dx > threshold
-    if not tags.any():
-        return None
-    k, j, i = np.nonzero(tags)
-    clo = [int(i.min()) + lo[0] - buffer, int(j.min()) + lo[1] - buffer, int(k.min()) + lo[2] - buffer]
-    chi = [int(i.max()) + 1 + lo[0] + buffer, int(j.max()) + 1 + lo[1] + buffer, int(k.max()) + 1 + lo[2] + buffer]
-    clo = [max(clo[a], lo[a]) for a in range(3)]
-    chi = [min(chi[a], hi[a]) for a in range(3)]
-    return clo, chi
-
-
-def amr_hierarchy(n0: int, spheres, levels: int = 3, buffer: int = 2, width: float = 0.0):
-    """Patch boxes [(clo, chi), ...] of AMR levels 1..levels-1 over a base grid
-    of n0 cells: each level tags its cells by the sphere-indicator gradient
-    (tag_box) inside the region where a child patch and its ghost layer stay
-    on the parent's unknowns, and takes the bounding box."""
-    boxes = []
-    lo, hi = [0, 0, 0], [n0, n0, n0]      # searchable cells of the current level
-    dom = [n0, n0, n0]
-    for l in range(levels - 1):
-        dx = 1.0 / dom[0]
-        tb = tag_box(dom[0], dx, lo, (lo, hi), spheres, buffer=buffer, width=width)
-        if tb is None:
-            break
-        clo, chi = tb
-        # ghost layer (inner faces) must sit on the parent's unknowns
-        clo = [max(clo[a], lo[a] + (1 if lo[a] > 0 else 0)) for a in range(3)]
-        chi = [min(chi[a], hi[a] - (1 if hi[a] < dom[a] else 0)) for a in range(3)]
-        if any(chi[a] - clo[a] < 2 for a in range(3)):
-            break
-        boxes.append((tuple(clo), tuple(chi)))
-        dom = [2 * d for d in dom]
-        lo, hi = [2 * c for c in clo], [2 * c for c in chi]
-    return boxes
+def sphere_indicators(origin, shape, dx, spheres, width=0.0):
+    """The sphere indicators eta (nspheres, nz, ny, nx) at the nodes
+    (origin + index) * dx of a box: the fields the refinement tags of
+    src/Integrator/Mechanics.H:298-313 are computed from."""
+    nz, ny, nx = shape
+    x = ((origin[0] + np.arange(nx)) * dx)[None, None, :]
+    y = ((origin[1] + np.arange(ny)) * dx)[None, :, None]
+    z = ((origin[2] + np.arange(nz)) * dx)[:, None, None]
+    return np.ascontiguousarray(np.stack([np.broadcast_to(sphere_eta(x, y, z, c, r, width), shape)
+                                          for c, r, _ in spheres]))

@@ -260,6 +260,17 @@ int alamo_el_tag_bbox(alamo_el_t h, const double* eta, int ncomp, double thresho
  * (alamo_amr_patch_info gives the global node index of node 0). */
 int alamo_amr_patch_create(alamo_el_t parent, const int clo[3], const int chi[3], alamo_el_t* out);
 int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost_hi[3]);
+/* Regrid: fill the field dst_u of the new patch dst (every handle node, ghost
+ * layer included) as Integrator::FillPatch does for a nodal field
+ * (src/Integrator/Integrator.cpp:394-446, FillPatchTwoLevels with
+ * node_bilinear_interp): nodes that the old patch src of the same AMR level
+ * holds as unknowns (its global nodes 2 clo .. 2 chi) take src_u's value, the
+ * others the trilinear interpolation of parent_u, the field of dst's parent
+ * handle.  With src NULL every node is interpolated, as
+ * Integrator::FillCoarsePatch does for a new level (:452-478).  The old patch
+ * may have another parent (the regridded one). */
+int alamo_amr_fillpatch(alamo_el_t dst, const double* parent_u, alamo_el_t src, const double* src_u, double* dst_u,
+                        hipStream_t stream);
 /* Solve the composite system over levels[0] (the base handle) and its nested
  * patches levels[1..nlev-1]: x[l], b[l] are fields of each handle.  One
  * iteration pre-smooths every patch (nu1 Fsmooth calls), refluxes the residual

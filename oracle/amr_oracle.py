@@ -7,6 +7,8 @@ Restates, in numpy over the C++ oracle's Fapply/Jacobi/residual/MG cycle:
   - the patch boundary table: the parent's on domain faces, displacement on
     every region touching a face inside the domain (ghost layer);
   - trilinear interpolation onto the patch (Operator.cpp:365-434 weights);
+  - FillPatch of a regridded patch (src/Integrator/Integrator.cpp:394-478):
+    the old patch's unknowns where they reach, interpolation elsewhere;
   - averageDownSolutionRHS, nodal injection (src/Operator/Operator.cpp:436-448);
   - reflux (Operator.cpp:526-633): the parent residual at the covered and
     coarse-fine nodes is replaced by the full-weighting restriction of the
@@ -97,6 +99,28 @@ def interp(c, parent, lev):
             for xi in (xl, xh):
                 out += c[:, zi[:, None, None], yi[None, :, None], xi[None, None, :]]
     return out / 8.0
+
+
+def fill_patch(c, parent, lev, old=None, old_u=None):
+    """Integrator::FillPatch for a nodal field after a regrid
+    (src/Integrator/Integrator.cpp:394-478; FillPatchTwoLevels with
+    node_bilinear_interp): the new patch lev takes the old patch's values at
+    the nodes its unknowns (global nodes 2 clo .. 2 chi) hold, the parent
+    field interpolated trilinearly elsewhere (everywhere without an old
+    patch).  parent, lev, old: objects with origin, shape (and clo, chi)."""
+    out = interp(c, parent, lev)
+    if old is None:
+        return out
+    nz, ny, nx = lev.shape
+    g = [lev.origin[a] + np.arange(n) for a, n in enumerate((nx, ny, nz))]
+    inside = [(2 * old.clo[a] <= g[a]) & (g[a] <= 2 * old.chi[a]) for a in range(3)]
+    if not all(m.any() for m in inside):
+        return out
+    sel = [np.nonzero(m)[0] for m in inside]
+    src = [g[a][sel[a]] - old.origin[a] for a in range(3)]
+    out[:, sel[2][:, None, None], sel[1][None, :, None], sel[0][None, None, :]] = \
+        old_u[:, src[2][:, None, None], src[1][None, :, None], src[0][None, None, :]]
+    return out
 
 
 def inject(f, lev, c, parent):

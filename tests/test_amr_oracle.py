@@ -74,3 +74,37 @@ def test_whole_domain_patch_gives_the_fine_grid_solution():
     assert np.linalg.norm(xs[1] - xf) / np.linalg.norm(xf) < 1e-8
     # the base level holds the injected fine solution
     assert np.allclose(xs[0], xs[1][:, ::2, ::2, ::2])
+
+
+def test_fill_patch_copies_the_old_unknowns_and_interpolates_the_rest():
+    """Regrid FillPatch (src/Integrator/Integrator.cpp:394-478): without an old
+    patch it is the interpolation; an old field that is itself the parent's
+    interpolation changes nothing; a marked old field lands exactly on the
+    overlap of the old unknowns (global nodes 2 clo .. 2 chi per axis, each
+    value at its own global node), and every other node keeps the
+    interpolated value."""
+    from types import SimpleNamespace as NS
+    par = NS(origin=(0, 0, 0), shape=(17, 13, 9))                     # 8 x 12 x 16 cells
+    dom = (8, 12, 16)
+
+    def patch(clo, chi):
+        origin, gl, gh, n_cell = AO.patch_geometry(dom, clo, chi)
+        return NS(origin=origin, shape=(n_cell[2] + 1, n_cell[1] + 1, n_cell[0] + 1), clo=clo, chi=chi)
+
+    new, old = patch((1, 2, 3), (6, 9, 12)), patch((3, 0, 6), (8, 5, 15))
+    c = np.random.default_rng(5).standard_normal((3,) + par.shape)
+    base = AO.interp(c, par, new)
+    assert np.array_equal(AO.fill_patch(c, par, new), base)
+    assert np.allclose(AO.fill_patch(c, par, new, old, AO.interp(c, par, old)), base, rtol=0, atol=1e-15)
+    # mark each old node by its global index: value = gx + 1000 gy + 1e6 gz (+ 1e9 comp)
+    oz, oy, ox = np.meshgrid(*[old.origin[a] + np.arange(old.shape[2 - a]) for a in (2, 1, 0)], indexing="ij")
+    mark = np.stack([ox + 1e3 * oy + 1e6 * oz + 1e9 * p for p in range(3)]).astype(np.float64)
+    f = AO.fill_patch(c, par, new, old, mark)
+    nz, ny, nx = np.meshgrid(*[new.origin[a] + np.arange(new.shape[2 - a]) for a in (2, 1, 0)], indexing="ij")
+    hit = ((2 * 3 <= nx) & (nx <= 2 * 8) & (2 * 0 <= ny) & (ny <= 2 * 5) & (2 * 6 <= nz) & (nz <= 2 * 15))
+    span = [min(new.origin[a] + new.shape[2 - a] - 1, 2 * old.chi[a]) - max(new.origin[a], 2 * old.clo[a]) + 1
+            for a in range(3)]
+    assert span == [8, 8, 14] and hit.sum() == 8 * 8 * 14
+    for p in range(3):
+        assert np.array_equal(f[p][hit], (nx + 1e3 * ny + 1e6 * nz + 1e9 * p)[hit])
+        assert np.array_equal(f[p][~hit], base[p][~hit])

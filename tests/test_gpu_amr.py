@@ -117,6 +117,42 @@ def test_bad_patches_are_rejected():
         p.refine((8, 8, 8), (20, 20, 20))          # ghost layer outside the parent's unknowns
 
 
+def _eta_of(spheres, width):
+    return lambda op: torch.from_numpy(P.sphere_indicators(op.origin, op.level_shape(0), op.dx[0], spheres,
+                                                           width)).cuda()
+
+
+def _device_boxes(n0, spheres, width, levels=3):
+    base = el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC)
+    return [(op.clo, op.chi) for op in el.amr_levels(base, _eta_of(spheres, width), levels=levels)[1:]]
+
+
+def _oracle_boxes(n0, spheres, width, levels=3, threshold=0.1, buffer=2):
+    """The same hierarchy from the oracle's tags: each level's indicator at
+    its handle's nodes, the tags' box clipped to the level's unknowns less one
+    node where they meet an inner face (the nesting a patch needs)."""
+    boxes, origin, gl, gh, dom = [], (0, 0, 0), (0, 0, 0), (0, 0, 0), (n0,) * 3
+    n_cell = (n0,) * 3
+    while len(boxes) < levels - 1:
+        dx = 1.0 / dom[0]
+        shape = tuple(n_cell[a] + 1 for a in (2, 1, 0))
+        eta = P.sphere_indicators(origin, shape, dx, spheres, width)
+        box, n = O.tag_box(eta, (dx,) * 3, threshold, buffer)
+        if box is None:
+            break
+        clo, chi = [], []
+        for a in range(3):
+            ulo, uhi = origin[a] + gl[a], origin[a] + n_cell[a] - gh[a]
+            clo.append(max(box[0][a] + origin[a], ulo + (ulo > 0)))
+            chi.append(min(box[1][a] + origin[a], uhi - (uhi < dom[a])))
+        if any(chi[a] - clo[a] < 2 for a in range(3)):
+            break
+        boxes.append((tuple(clo), tuple(chi)))
+        origin, gl, gh, n_cell = AO.patch_geometry(dom, clo, chi)
+        dom = tuple(2 * d for d in dom)
+    return boxes
+
+
 def test_three_level_random_inclusions_fp64_and_fp32():
     """Configuration of the 3-level AMR case at a 33^3 base: three seeded
     inclusions (r ~ U[0.05, 0.15], E ~ U[2, 20]) with interfaces two base
@@ -127,7 +163,7 @@ def test_three_level_random_inclusions_fp64_and_fp32():
     import time
     spheres = P.random_spheres(3, seed=0)
     w = 2.0 / 32
-    boxes = P.amr_hierarchy(32, spheres, width=w)
+    boxes = _device_boxes(32, spheres, w)
     assert len(boxes) == 2
     ops, bs, _ = hierarchy(32, boxes, spheres=spheres, width=w)
     res = {}
@@ -174,3 +210,100 @@ def test_refinement_tagging_matches_oracle(ncomp):
             assert np.abs(v - thr).min() > 1e-9 * thr
         assert op.tag_box(torch.from_numpy(eta).cuda(), thr, buf) == O.tag_box(eta, dx, thr, buf)
     assert op.tag_box(torch.from_numpy(eta).cuda(), 1e6, 2) == (None, 0)
+
+
+def test_device_tagged_hierarchy_matches_oracle_tags():
+    """amr_levels (device tags, alamo_el_tag_bbox on every level's indicator
+    field, then the clipped box) builds the same three levels as the oracle's
+    tags (src/Integrator/Mechanics.H:298-313) on the 33^3-base inclusion case
+    and on a sharp single sphere."""
+    for n0, spheres, width in ((32, P.random_spheres(3, seed=0), 2.0 / 32), (16, SPHERE, 0.0),
+                               (32, P.random_spheres(3, seed=7), 2.0 / 32)):
+        dev, ora = _device_boxes(n0, spheres, width), _oracle_boxes(n0, spheres, width)
+        print(n0, dev)
+        assert dev == ora and len(dev) >= 1
+
+
+@pytest.mark.parametrize("with_old", [False, True])
+def test_fillpatch_matches_oracle(with_old):
+    """alamo_amr_fillpatch (Integrator::FillPatch / FillCoarsePatch,
+    src/Integrator/Integrator.cpp:394-478) against the oracle on random fields:
+    a level-1 patch filled from the base and from an overlapping old level-1
+    patch, then a level-2 patch of the new level 1 filled from an old level-2
+    patch whose parent was the old level 1.  Copied nodes agree bit for bit,
+    interpolated ones to 1e-15 (the oracle sums repeated corners)."""
+    from types import SimpleNamespace as NS
+    rng = np.random.default_rng(8)
+    base = el.ElasticOperator((16, 12, 20), 1.0 / 16, M.ISOTROPIC)
+    new1, old1 = base.refine((2, 1, 3), (11, 9, 14)), base.refine((5, 0, 8), (15, 7, 19))
+    new2, old2 = new1.refine((6, 4, 8), (18, 14, 26)), old1.refine((12, 3, 18), (26, 12, 36))
+
+    def ns(op):
+        return NS(origin=op.origin, shape=op.level_shape(0), clo=op.clo, chi=op.chi)
+
+    def field(op):
+        return rng.standard_normal((3,) + op.level_shape(0))
+
+    c0, o1, o2 = field(base), field(old1), field(old2)
+    par0 = NS(origin=(0, 0, 0), shape=base.level_shape(0))
+    f1 = new1.fill_patch(torch.from_numpy(c0).cuda(), old1 if with_old else None,
+                         torch.from_numpy(o1).cuda() if with_old else None)
+    r1 = AO.fill_patch(c0, par0, ns(new1), ns(old1) if with_old else None, o1 if with_old else None)
+    f2 = new2.fill_patch(f1, old2 if with_old else None, torch.from_numpy(o2).cuda() if with_old else None)
+    r2 = AO.fill_patch(f1.cpu().numpy(), ns(new1), ns(new2), ns(old2) if with_old else None,
+                       o2 if with_old else None)
+    for f, r in ((f1, r1), (f2, r2)):
+        d = np.abs(f.cpu().numpy() - r)
+        print(f"max |device - oracle| {d.max():.2e}, exact at {np.mean(d == 0):.3f} of the nodes")
+        assert d.max() <= 1e-15 * np.abs(r).max()
+    if with_old:  # nodes held by the old patches are copied exactly
+        for f, new, old, o in ((f1, new1, old1, o1), (f2, new2, old2, o2)):
+            lo = [max(new.origin[a], 2 * old.clo[a]) for a in range(3)]
+            hi = [min(new.origin[a] + new.level_shape(0)[2 - a] - 1, 2 * old.chi[a]) for a in range(3)]
+            sn = tuple(slice(lo[a] - new.origin[a], hi[a] - new.origin[a] + 1) for a in (2, 1, 0))
+            so = tuple(slice(lo[a] - old.origin[a], hi[a] - old.origin[a] + 1) for a in (2, 1, 0))
+            assert np.array_equal(f.cpu().numpy()[(slice(None),) + sn], o[(slice(None),) + so])
+    with pytest.raises(el.AlamoError):
+        new2.fill_patch(f1, old1, torch.from_numpy(o1).cuda())   # old patch of another level
+
+
+def test_regrid_after_the_inclusions_move():
+    """The adaptive flow on the 33^3-base three-inclusion case: device tags
+    build three levels, the composite solve converges; the inclusions move by
+    one base cell, amr_regrid retags and fills the new levels from the old
+    solution, and the composite solve from the filled fields converges to the
+    solution reached from zero (1e-6 relative on every level)."""
+    n0, w = 32, 2.0 / 32
+    spheres = P.random_spheres(3, seed=0)
+    ops = el.amr_levels(el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC), _eta_of(spheres, w), levels=3)
+    bs = _models(ops, spheres, w)
+    xs = [torch.zeros_like(b) for b in bs]
+    info = el.amr_solve(ops, xs, bs, 1e-8, 0.0, max_iter=300, cycle="V")
+    assert info.status == 0
+    moved = [((c[0] + 1.0 / n0, c[1], c[2]), r, E) for c, r, E in spheres]
+    ops2, xs2 = el.amr_regrid(ops, xs, _eta_of(moved, w), levels=3)
+    assert len(ops2) == 3 and [(o.clo, o.chi) for o in ops2[1:]] == _oracle_boxes(n0, moved, w)
+    assert [(o.clo, o.chi) for o in ops2[1:]] != [(o.clo, o.chi) for o in ops[1:]]
+    bs2 = _models(ops2, moved, w)
+    sol = {}
+    for name, x0 in (("filled", xs2), ("zero", [torch.zeros_like(b) for b in bs2])):
+        x = [v.clone() for v in x0]
+        info = el.amr_solve(ops2, x, bs2, 1e-8, 0.0, max_iter=300, cycle="V")
+        print(f"from {name}: {info.iters} composite V-cycles, |r0| {info.res0:.2e}, |b| {info.rhsnorm:.2e}, "
+              f"|r| {info.res:.2e}")
+        assert info.status == 0
+        sol[name] = x
+    for a, b in zip(sol["filled"], sol["zero"]):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-6
+
+
+def _models(ops, spheres, width):
+    bs = []
+    for op in ops:
+        shape = op.level_shape(0)
+        C, F0 = P.inclusions_on_box(op.origin, shape, op.dx[0], spheres, width=width)
+        Cd = torch.from_numpy(C).cuda()
+        op.SetModel(Cd)
+        b0 = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
+        bs.append(op.newton_rhs(Cd, torch.zeros_like(b0), b0, F0=torch.from_numpy(F0).cuda()))
+    return bs

@@ -360,7 +360,9 @@ class ElasticOperator:
         (src/Integrator/Mechanics.H:298-313): ((clo, chi), ntag) with [clo, chi]
         the tagged cells' bounding box grown by `buffer` cells (level cell
         indices), or (None, 0) when no cell is tagged.  eta: (ncomp,) + level
-        shape, float64 on the GPU."""
+        shape, float64 on the GPU.  threshold is eta_ref_threshold (0.01 by
+        default in the reference, src/Integrator/Mechanics.H:137; 0.1 for the
+        three-inclusion AMR case)."""
         shape = self.level_shape(0)
         if eta.dim() != 4 or tuple(eta.shape[1:]) != shape:
             raise AlamoError(f"tag_box: expected eta of shape (ncomp,) + {shape}, got {tuple(eta.shape)}")

@@ -241,7 +241,8 @@ int alamo_el_learned_guess(alamo_el_t h, const double* b, double* x, hipStream_t
  * AMR levels. --- */
 /* Integrator::Mechanics::TagCellsForRefinement on the finest level of h
  * (src/Integrator/Mechanics.H:298-313): cell (i, j, k) is tagged when
- * |grad eta_n| |dx| > threshold at node (i, j, k) for some component n of eta
+ * |grad eta_n| |dx| > threshold (eta_ref_threshold, 0.01 by default in the
+ * reference, src/Integrator/Mechanics.H:137) at node (i, j, k) for some component n of eta
  * (ncomp device planes of the level's nodes; the gradient central inside and
  * one-sided on the box faces).  *ntag receives the number of tagged cells and,
  * when it is not 0, [clo, chi] the tagged cells' bounding box grown by buffer

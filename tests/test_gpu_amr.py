@@ -307,3 +307,30 @@ def _models(ops, spheres, width):
         b0 = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
         bs.append(op.newton_rhs(Cd, torch.zeros_like(b0), b0, F0=torch.from_numpy(F0).cuda()))
     return bs
+
+
+def test_plotfile_of_a_composite_solution(tmp_path):
+    """The solved three-level hierarchy written as the reference's node and
+    cell plotfiles (eta, disp, stress per level, stress from alamo_el_stress
+    on each handle) reads back with the solution on every level's unknowns."""
+    from alamo_amd import plotfile as PF
+    n0, w = 16, 2.0 / 16
+    spheres = [((0.5, 0.5, 0.5), 0.2, 10.0)]
+    ops = el.amr_levels(el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC), _eta_of(spheres, w), levels=3)
+    bs = _models(ops, spheres, w)
+    xs = [torch.zeros_like(b) for b in bs]
+    assert el.amr_solve(ops, xs, bs, 1e-8, 0.0, max_iter=300, cycle="V").status == 0
+    eta, sig = [], []
+    for op, x in zip(ops, xs):
+        C, F0 = P.inclusions_on_box(op.origin, op.level_shape(0), op.dx[0], spheres, width=w)
+        sig.append(op.stress(torch.from_numpy(C).cuda(), x, F0=torch.from_numpy(F0).cuda()))
+        eta.append(_eta_of(spheres, w)(op))
+    paths = PF.write_mechanics(str(tmp_path / "out"), 0, ops, [("eta", eta), ("disp", xs), ("stress", sig)])
+    names, _, levels = PF.read(paths[0])
+    assert names[:4] == ["eta", "disp_x", "disp_y", "disp_z"] and len(levels) == len(ops) == 3
+    for (lo, hi, data), op, x in zip(levels, ops, xs):
+        gl, gh = op.ghost_lo, op.ghost_hi
+        v = x.cpu().numpy()
+        nz, ny, nx = v.shape[1:]
+        assert lo == tuple(op.origin[a] + gl[a] for a in range(3))
+        assert np.array_equal(data[1:4], v[:, gl[2]:nz - gh[2], gl[1]:ny - gh[1], gl[0]:nx - gh[0]])

@@ -26,6 +26,7 @@
 #include "elastic_kernels.h"
 #include "amr_kernels.h"
 #include "learned_kernels.h"
+#include "psi_kernels.h"
 #include "stencil.h"
 #include "stencil_lds.h"
 
@@ -74,6 +75,7 @@ struct Level {
     int* dwbase = nullptr;           // per interior wave of the launch: first compact D position
     int nflag = 0;                   // interior nodes with a nonzero gmask
     bool tiled = false;              // MajorMinor level served by k_stencil_lds (else k_stencil_slab)
+    double* psi = nullptr;           // SetPsi weight on this level's cells (null until set)
     ael::TileGrid tg{}, tg32{};      // its tile grids (chunk lengths) of the FP64 and FP32 launches
 };
 
@@ -97,6 +99,8 @@ struct alamo_el_s {
     double* red_partial = nullptr;
     double* red_out = nullptr;
     double* red_host = nullptr;  // pinned
+    bool psi_set = false;        // SetPsi called: every operator row carries the psi weight
+    double psi_small = 1e-8;
     double* red_host_vec = nullptr;  // pinned, FGMRES coefficients
     double* bottom_work = nullptr;
     int* bottom_status = nullptr;
@@ -190,8 +194,8 @@ int stencil_variant() {
 }
 
 template <typename T, int OP>
-void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const T* diag, const T* C, T* out,
-                    hipStream_t s) {
+void launch_stencil_raw(alamo_el_t h, const Level& l, const T* x, const T* b, const T* diag, const T* C, T* out,
+                        hipStream_t s) {
     const Lvl L = l.L;
     const T w = (T)h->omega;
     const int u = h->uniform;
@@ -243,6 +247,37 @@ void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const 
     }
 }
 
+// Operator rows with the psi weight (psi_kernels.h) when SetPsi was called:
+// the unweighted rows of x first, then k_psi_post finishes Fapply (OP 0), the
+// residual (OP 1) or the Jacobi sweep (OP 2).  out must differ from x and b.
+template <typename T, int OP>
+void launch_stencil(alamo_el_t h, const Level& l, const T* x, const T* b, const T* diag, const T* C, T* out,
+                    hipStream_t s) {
+    if (!h->psi_set || !l.psi) {
+        launch_stencil_raw<T, OP>(h, l, x, b, diag, C, out, s);
+        return;
+    }
+    launch_stencil_raw<T, 0>(h, l, x, (const T*)nullptr, (const T*)nullptr, C, out, s);
+    const Lvl L = l.L;
+    const T w = (T)h->omega;
+    switch (h->sym) {
+        case 0: ael::k_psi_post<T, 0><<<nblk(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, l.psi, h->psi_small, L, w, OP, out); break;
+        case 1: ael::k_psi_post<T, 1><<<nblk(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, l.psi, h->psi_small, L, w, OP, out); break;
+        default: ael::k_psi_post<T, 2><<<nblk(L.N), kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, l.psi, h->psi_small, L, w, OP, out); break;
+    }
+}
+
+// The psi weight on the rows of M unit-vector columns (dense assemblies).
+void psi_post_columns(alamo_el_t h, const Level& l, const double* I, double* A, int M, hipStream_t s) {
+    if (!h->psi_set || !l.psi) return;
+    const dim3 grid(nblk(l.L.N), M);
+    switch (h->sym) {
+        case 0: ael::k_psi_post<double, 0><<<grid, kBlock, 0, s>>>(I, nullptr, nullptr, l.d.C, h->bct_dev, l.psi, h->psi_small, l.L, 0.0, 0, A); break;
+        case 1: ael::k_psi_post<double, 1><<<grid, kBlock, 0, s>>>(I, nullptr, nullptr, l.d.C, h->bct_dev, l.psi, h->psi_small, l.L, 0.0, 0, A); break;
+        default: ael::k_psi_post<double, 2><<<grid, kBlock, 0, s>>>(I, nullptr, nullptr, l.d.C, h->bct_dev, l.psi, h->psi_small, l.L, 0.0, 0, A); break;
+    }
+}
+
 template <typename T>
 void launch_fapply(alamo_el_t h, const Level& l, const T* u, const T* C, T* f, hipStream_t s) {
     launch_stencil<T, 0>(h, l, u, (const T*)nullptr, (const T*)nullptr, C, f, s);
@@ -267,6 +302,7 @@ void launch_diagonal(alamo_el_t h, const Level& l, const T* C, T* diag, hipStrea
         case 1: ael::k_diagonal<T, 1><<<nblk8(L.N), kBlock, 0, s>>>(C, h->bct_dev, L, diag); break;
         default: ael::k_diagonal<T, 2><<<nblk8(L.N), kBlock, 0, s>>>(C, h->bct_dev, L, diag); break;
     }
+    if (h->psi_set && l.psi) ael::k_psi_diag<T><<<nblk(L.N), kBlock, 0, s>>>(diag, h->bct_dev, l.psi, h->psi_small, L);
 }
 
 template <typename T>
@@ -493,6 +529,7 @@ int assemble_dense(alamo_el_t h, hipStream_t s) {
         case 1: ael::k_assemble_dense<1><<<grid, kBlock, 0, s>>>(h->dense_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->dense_A); break;
         default: ael::k_assemble_dense<2><<<grid, kBlock, 0, s>>>(h->dense_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->dense_A); break;
     }
+    psi_post_columns(h, lb, h->dense_I, h->dense_A, M, s);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -547,6 +584,7 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
         case 1: ael::k_assemble_dense<1><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
         default: ael::k_assemble_dense<2><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
     }
+    psi_post_columns(h, lb, h->direct_I, h->direct_A, M, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h->direct_Ainv, h->direct_I, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
     const ael::GjState zero{0, 0, 0.0};
@@ -657,6 +695,13 @@ int prepare(alamo_el_t h, hipStream_t s) {
         ael::k_restrict<double><<<nblk8(c.L.N), kBlock, 0, s>>>(f.d.C, f.L, c.d.C, c.L, h->nc);
         int rc = build_gmask(h, c, s);
         if (rc) return rc;
+        if (h->psi_set) {
+            // (Elastic.cpp:839-907) every coarse cell covers two fine cells per axis
+            if (f.L.nx - 1 != 2 * (c.L.nx - 1) || f.L.ny - 1 != 2 * (c.L.ny - 1) || f.L.nz - 1 != 2 * (c.L.nz - 1))
+                return fail(ALAMO_EL_ESTATE, "psi restriction needs exact coarsening by 2");
+            const long nc = (long)(c.L.nx - 1) * (c.L.ny - 1) * (c.L.nz - 1);
+            ael::k_psi_restrict<<<nblk(nc), kBlock, 0, s>>>(f.psi, f.L, c.psi, c.L);
+        }
     }
     // Diagonal(true) on every MG level (src/Operator/Operator.cpp:16-29)
     for (auto& l : h->lv) launch_diagonal<double>(h, l, l.d.C, l.d.diag, s);
@@ -936,6 +981,7 @@ int alamo_el_destroy(alamo_el_t h) {
             if (p) (void)hipFree(p);
         if (l.gmask) (void)hipFree(l.gmask);
         if (l.dwbase) (void)hipFree(l.dwbase);
+        if (l.psi) (void)hipFree(l.psi);
     }
     free_direct(h);
     for (void* p : {(void*)h->bct_dev, (void*)h->bcv_dev, (void*)h->red_partial, (void*)h->red_out,
@@ -1006,6 +1052,33 @@ int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t s)
     int rc = build_gmask(h, l, s);
     if (rc) return rc;
     h->model_set = true;
+    h->prepared = false;
+    h->prepared32 = false;
+    return 0;
+}
+
+// k_bottom_solve (the coarsest-level BiCGStab for levels above kDenseMax
+// unknowns) applies the unweighted operator, so with psi the bottom must be
+// the dense or the direct one.
+static int psi_bottom_check(alamo_el_t h, const alamo_el_opts& o) {
+    if (!h->psi_set) return 0;
+    if ((o.bottom_solver == ALAMO_EL_BOTTOM_DIRECT && h->direct_Ainv) || h->dense_A) return 0;
+    return fail(ALAMO_EL_EINVAL, "with psi the coarsest level needs at most 1536 unknowns, or the direct bottom");
+}
+
+int alamo_el_set_psi(alamo_el_t h, const double* psi, double psi_small, hipStream_t s) {
+    if (!h || !psi) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (!(psi_small >= 0.0 && psi_small < 1.0)) return fail(ALAMO_EL_EINVAL, "psi_small must lie in [0, 1)");
+    for (auto& l : h->lv) {
+        if (l.psi) continue;
+        const long nc = (long)(l.L.nx - 1) * (l.L.ny - 1) * (l.L.nz - 1);
+        HIPCHK(hipMalloc((void**)&l.psi, sizeof(double) * std::max(nc, 1L)));
+    }
+    Level& l0 = h->lv[0];
+    const long nc0 = (long)(l0.L.nx - 1) * (l0.L.ny - 1) * (l0.L.nz - 1);
+    HIPCHK(hipMemcpyAsync(l0.psi, psi, sizeof(double) * nc0, hipMemcpyDeviceToDevice, s));
+    h->psi_set = true;
+    h->psi_small = psi_small;
     h->prepared = false;
     h->prepared32 = false;
     return 0;
@@ -1243,6 +1316,7 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
     h->omega = o.omega;
     int rc;
     if (!h->prepared && (rc = prepare(h, s))) return rc;
+    if ((rc = psi_bottom_check(h, o))) return rc;
     if (o.precision == ALAMO_EL_PREC_FP32_IR && (rc = prepare32(h, s))) return rc;
     Level& l0 = h->lv[0];
     const long M0 = 3 * l0.L.N;
@@ -1687,6 +1761,7 @@ int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const doub
     for (int l = 0; l < nlev; ++l) {
         hs[l]->omega = o.omega;
         if (!hs[l]->prepared && (rc = prepare(hs[l], s))) return rc;
+        if (l == 0 && (rc = psi_bottom_check(hs[0], o))) return rc;
     }
     const bool fp32 = o.precision == ALAMO_EL_PREC_FP32_IR;
     if (fp32 && (rc = prepare32(hs[0], s))) return rc;

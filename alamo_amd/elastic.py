@@ -61,7 +61,7 @@ EXPORTS = [
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
-    "alamo_el_tag_bbox", "alamo_amr_fillpatch",
+    "alamo_el_tag_bbox", "alamo_amr_fillpatch", "alamo_el_set_psi",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -87,6 +87,7 @@ def lib():
         L.alamo_el_set_uniform.argtypes = [vp, ctypes.c_int]
         L.alamo_el_set_omega.argtypes = [vp, ctypes.c_double]
         L.alamo_el_set_model.argtypes = [vp, vp, ctypes.c_int, vp]
+        L.alamo_el_set_psi.argtypes = [vp, vp, ctypes.c_double, vp]
         L.alamo_el_prepare.argtypes = [vp, vp]
         L.alamo_el_fapply.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_diagonal.argtypes = [vp, ctypes.c_int, vp, vp]
@@ -239,6 +240,15 @@ class ElasticOperator:
             raise AlamoError(f"SetModel: expected shape {want}, got {tuple(C.shape)}")
         _check(lib().alamo_el_set_model(self._h, _ptr(C), int(layout), _stream()), "alamo_el_set_model")
         self._C = C
+
+    def SetPsi(self, psi, psi_small: float = 1e-8):
+        """Operator::Elastic::SetPsi (src/Operator/Elastic.H:55-57): the cell
+        weight psi (nz-1, ny-1, nx-1) of the finest level."""
+        shape = tuple(n - 1 for n in self.level_shape(0))
+        if tuple(psi.shape) != shape:
+            raise AlamoError(f"SetPsi: expected psi of shape {shape} (cells), got {tuple(psi.shape)}")
+        _check(lib().alamo_el_set_psi(self._h, _ptr(psi), float(psi_small), _stream()), "alamo_el_set_psi")
+        self._psi = psi
 
     def prepare(self):
         _check(lib().alamo_el_prepare(self._h, _stream()), "alamo_el_prepare")

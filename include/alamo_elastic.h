@@ -139,6 +139,18 @@ int alamo_el_set_omega(alamo_el_t h, double omega);
  * caller's per-node stiffness into the operator's own storage for MG level 0
  * (src/Operator/Elastic.cpp:87-116). */
 int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t stream);
+/* Operator::Elastic::SetPsi(amrlev, psi, psi_small) (src/Operator/Elastic.H:55-57,
+ * src/Operator/Elastic.cpp:118-137): psi holds one value per cell of the finest
+ * level ((nx-1)(ny-1)(nz-1) device doubles, x fastest; cells outside the domain
+ * count as 1).  Every row of Fapply, Diagonal, Fsmooth, the residuals and the
+ * solves then carries the weight psi_avg = (1 - psi_small) * (mean of the 8
+ * cells around the node) + psi_small, and interior rows gain
+ * (1 - psi_small) (C : grad u) . grad psi (Elastic.cpp:190-259, 340-363).  The
+ * coarse MG levels take psi averaged down as the reference does
+ * (Elastic.cpp:839-907) at the next alamo_el_prepare.  The reference's default
+ * psi_small is 1e-8.  A solve with psi needs a coarsest level of at most 1536
+ * unknowns or ALAMO_EL_BOTTOM_DIRECT. */
+int alamo_el_set_psi(alamo_el_t h, const double* psi, double psi_small, hipStream_t stream);
 /* prepareForSolve: averageDownCoeffsSameAmrLevel on every MG level and
  * Diagonal(true) (src/Operator/Operator.cpp:253-260,
  * src/Operator/Elastic.cpp:605-626,746-910). */

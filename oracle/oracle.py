@@ -250,3 +250,152 @@ def tag_box(eta, dx, threshold=0.1, buffer=2):
     clo = tuple(max(int(v.min()) - buffer, 0) for v in (i, j, k))
     chi = tuple(min(int(v.max()) + 1 + buffer, nc[a]) for a, v in enumerate((i, j, k)))
     return (clo, chi), ntag
+
+
+# --- psi weight (Operator::Elastic::SetPsi, src/Operator/Elastic.cpp:118-137) -------
+def _psi_padded(psi):
+    """Cells (nz-1, ny-1, nx-1) with one layer of 1 around them: the operator
+    fills psi with 1 at define (Elastic.cpp:49-52) and SetPsi copies only the
+    domain's cells (:124-134)."""
+    return np.pad(np.asarray(psi, dtype=np.float64), 1, constant_values=1.0)
+
+
+def psi_avg(psi, small=1e-8):
+    """(1 - small) CellToNodeAverage + small at every node
+    (src/Numeric/Stencil.H:1390-1406, Elastic.cpp:191-192)."""
+    p = _psi_padded(psi)
+    s = sum(p[dz:dz + p.shape[0] - 1, dy:dy + p.shape[1] - 1, dx:dx + p.shape[2] - 1]
+            for dz in (0, 1) for dy in (0, 1) for dx in (0, 1))
+    return (1.0 - small) * (s / 8.0) + small
+
+
+def psi_grad(psi, dx):
+    """Numeric::CellGradientOnNode (src/Numeric/Stencil.H:699-715) at every node:
+    (3, nz, ny, nx), x, y, z components."""
+    p = _psi_padded(psi)
+    h = _dx(dx)
+    nz, ny, nx = p.shape[0] - 1, p.shape[1] - 1, p.shape[2] - 1
+
+    def c(dz, dy, dx_):  # cell (i - 1 + dx_, j - 1 + dy, k - 1 + dz) of node (i, j, k)
+        return p[dz:dz + nz, dy:dy + ny, dx_:dx_ + nx]
+    out = np.zeros((3, nz, ny, nx))
+    for a, h_a in enumerate(h):
+        acc = 0.0
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx_ in (0, 1):
+                    up = (dx_, dy, dz)[a]
+                    acc = acc + (1 if up else -1) * c(dz, dy, dx_)
+        out[a] = 0.25 * acc / h_a
+    return out
+
+
+def regions(shape):
+    """Boundary region of every node in the order of bc.FACES (-1 inside),
+    classified by the set of domain faces the node lies on."""
+    from alamo_amd import bc as B
+    nz, ny, nx = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    parts = [("xlo", i == 0), ("xhi", i == nx - 1), ("ylo", j == 0), ("yhi", j == ny - 1), ("zlo", k == 0),
+             ("zhi", k == nz - 1)]
+    out = np.full(shape, -1, dtype=np.int64)
+    for r, name in enumerate(B.FACES):
+        want = {name[q:q + 3] for q in range(0, len(name), 3)}
+        m = np.ones(shape, dtype=bool)
+        for p, on in parts:
+            m &= on if p in want else ~on
+        out[m] = r
+    return out
+
+
+def _traction_rows(bc, shape):
+    """(3,) + shape mask of the boundary rows of traction type."""
+    from alamo_amd import bc as B
+    reg = regions(shape)
+    t = np.asarray(bc.type)
+    m = np.zeros((3,) + tuple(shape), dtype=bool)
+    for p in range(3):
+        m[p] = (reg >= 0) & (t[np.maximum(reg, 0), p] == B.TRACTION)
+    return m, reg < 0
+
+
+def fapply_psi(u, C, sym, bc, dx, psi, small=1e-8, uniform=False):
+    """Fapply with psi (Elastic.cpp:190-259): inside, psi_avg times the
+    unweighted row plus (1 - small) (C : grad u) . grad psi; on the boundary the
+    traction rows times psi_avg (Constant::set with sigma scaled)."""
+    f = fapply(u, C, sym, bc, dx, uniform=uniform)
+    shape = f.shape[1:]
+    pa = psi_avg(psi, small)
+    gp = psi_grad(psi, dx) * (1.0 - small)
+    trac, inner = _traction_rows(bc, shape)
+    sig = stress(u, C, sym, dx)
+    out = f.copy()
+    for p in range(3):
+        g = sum(sig[3 * p + a] * gp[a] for a in range(3))
+        out[p] = np.where(inner, pa * f[p] + g, np.where(trac[p], pa * f[p], f[p]))
+    return out
+
+
+def diagonal_psi(C, sym, bc, dx, shape, psi, small=1e-8):
+    """Diagonal with psi (Elastic.cpp:340-363)."""
+    d = diagonal(C, sym, bc, dx, shape)
+    pa = psi_avg(psi, small)
+    trac, inner = _traction_rows(bc, shape)
+    return np.where(inner[None] | trac, pa[None] * d, d)
+
+
+def residual_psi(x, b, C, sym, bc, dx, psi, small=1e-8, uniform=False):
+    return _f64(b) - fapply_psi(x, C, sym, bc, dx, psi, small, uniform)
+
+
+def fsmooth_psi(x, b, C, sym, bc, dx, psi, small=1e-8, omega=2.0 / 3.0, nsmooth=1):
+    """Fsmooth with psi: 2 nsmooth damped Jacobi sweeps on the weighted rows
+    and the weighted diagonal (Operator.cpp:88-154)."""
+    d = diagonal_psi(C, sym, bc, dx, x.shape[1:], psi, small)
+    x = _f64(x).copy()
+    for _ in range(2 * nsmooth):
+        x = x + omega * (_f64(b) - fapply_psi(x, C, sym, bc, dx, psi, small)) / d
+    return x
+
+
+def psi_restrict(pf):
+    """psi on the next MG level (Elastic.cpp:839-907), cell by cell as the
+    reference writes it: low-face cells by corner, edge (1-2-1) and face
+    (1-2-1 x 1-2-1) weights, every other cell by the 27-cell weights 1/8, 1/16,
+    1/32, 1/64 around the fine cell (2I, 2J, 2K)."""
+    pf = np.asarray(pf, dtype=np.float64)
+    cz, cy, cx = pf.shape[0] // 2, pf.shape[1] // 2, pf.shape[2] // 2
+    out = np.empty((cz, cy, cx))
+
+    def F(a, b, c):
+        return pf[c, b, a]
+    for K in range(cz):
+        for J in range(cy):
+            for I in range(cx):
+                i, j, k = 2 * I, 2 * J, 2 * K
+                if I == 0 and J == 0 and K == 0:
+                    v = F(i, j, k)
+                elif J == 0 and K == 0:
+                    v = 0.25 * F(i - 1, j, k) + 0.5 * F(i, j, k) + 0.25 * F(i + 1, j, k)
+                elif K == 0 and I == 0:
+                    v = 0.25 * F(i, j - 1, k) + 0.5 * F(i, j, k) + 0.25 * F(i, j + 1, k)
+                elif I == 0 and J == 0:
+                    v = 0.25 * F(i, j, k - 1) + 0.5 * F(i, j, k) + 0.25 * F(i, j, k + 1)
+                elif I == 0 or J == 0 or K == 0:
+                    ax = 0 if I == 0 else (1 if J == 0 else 2)
+                    v = 0.0
+                    for s in (-1, 0, 1):
+                        for t in (-1, 0, 1):
+                            o = [0, 0, 0]
+                            o[(ax + 1) % 3], o[(ax + 2) % 3] = s, t
+                            v += (2 - abs(s)) * (2 - abs(t)) * F(i + o[0], j + o[1], k + o[2])
+                    v /= 16.0
+                else:
+                    v = 0.0
+                    for dz in (-1, 0, 1):
+                        for dy in (-1, 0, 1):
+                            for dx_ in (-1, 0, 1):
+                                n = abs(dz) + abs(dy) + abs(dx_)
+                                v += F(i + dx_, j + dy, k + dz) / (8.0 * 2 ** n)
+                out[K, J, I] = v
+    return out

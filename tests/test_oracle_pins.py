@@ -488,3 +488,77 @@ def test_refinement_tagging_pins():
     box, n = O.tag_box(spike, dx, threshold=1e-6, buffer=1)
     assert n > 0 and all(box[0][a] >= 0 and box[1][a] <= (6, 5, 4)[a] for a in range(3))
     assert box[0] == (2, 1, 0) and box[1] == (6, 5, 4)
+
+
+def test_psi_weight_pins():
+    """The oracle's SetPsi weight (src/Operator/Elastic.cpp:118-259, 340-363,
+    839-907) pinned by mathematics:
+    - psi = 1 with psi_small = 0 leaves Fapply and Diagonal unchanged, bit for bit;
+    - a linear psi with uniform isotropic C and u = (xy, yz, zx) gives, at
+      every interior node, the product rule div(psi' sigma) = psi' div(sigma)
+      + sigma . grad psi' with psi' = (1 - s) psi + s, from closed forms;
+    - a constant psi c scales the traction rows of a boundary node by
+      (1 - s)(n c + 8 - n)/8 + s, n the node's cells inside the domain (the
+      cells outside count as 1), and leaves the displacement rows alone;
+    - the restriction keeps linear fields (value of the fine cell 2I, 2J, 2K)
+      and spreads unit spikes with the reference's weights."""
+    from alamo_amd import bc as B
+    from alamo_amd import model as M
+    n_cell = (6, 8, 10)
+    shape = (n_cell[2] + 1, n_cell[1] + 1, n_cell[0] + 1)
+    dx = (0.5, 0.25, 0.2)
+    lam, mu = 1.3, 0.7
+    C = np.empty((2,) + shape)
+    C[0], C[1] = lam, mu
+    z, y, x = np.meshgrid(*[np.arange(shape[2 - a]) * dx[a] for a in (2, 1, 0)], indexing="ij")
+    u = np.stack([x * y, y * z, z * x])
+    bc = B.Constant()
+    rng = np.random.default_rng(3)
+    bc.type[:] = rng.choice([B.DISPLACEMENT, B.TRACTION, B.NEUMANN], size=bc.type.shape)
+    ones = np.ones((shape[0] - 1, shape[1] - 1, shape[2] - 1))
+    assert np.array_equal(O.fapply_psi(u, C, M.ISOTROPIC, bc, dx, ones, small=0.0), O.fapply(u, C, M.ISOTROPIC, bc, dx))
+    assert np.array_equal(O.diagonal_psi(C, M.ISOTROPIC, bc, dx, shape, ones, small=0.0),
+                          O.diagonal(C, M.ISOTROPIC, bc, dx, shape))
+    # product rule on a linear psi sampled at the cell centres
+    g, c0, s = np.array([0.8, -0.5, 1.1]), 2.0, 1e-3
+    zc, yc, xc = np.meshgrid(*[(np.arange(shape[2 - a] - 1) + 0.5) * dx[a] for a in (2, 1, 0)], indexing="ij")
+    psi = c0 + g[0] * xc + g[1] * yc + g[2] * zc
+    f = O.fapply_psi(u, C, M.ISOTROPIC, bc, dx, psi, small=s)
+    ps = (1 - s) * (c0 + g[0] * x + g[1] * y + g[2] * z) + s
+    tr = x + y + z
+    sig = [[lam * tr + 2 * mu * y, mu * x, mu * z], [mu * x, lam * tr + 2 * mu * z, mu * y],
+           [mu * z, mu * y, lam * tr + 2 * mu * x]]
+    inner = np.s_[1:-1, 1:-1, 1:-1]
+    for p in range(3):
+        want = ps * (lam + mu) + (1 - s) * sum(sig[p][a] * g[a] for a in range(3))
+        assert np.allclose(f[p][inner], want[inner], rtol=1e-12, atol=1e-12)
+    # constant psi on the boundary: traction rows scaled, the cells outside read 1
+    cst = 0.3 * ones
+    f1 = O.fapply_psi(u, C, M.ISOTROPIC, bc, dx, cst, small=s)
+    f0 = O.fapply(u, C, M.ISOTROPIC, bc, dx)
+    d1 = O.diagonal_psi(C, M.ISOTROPIC, bc, dx, shape, cst, small=s)
+    d0 = O.diagonal(C, M.ISOTROPIC, bc, dx, shape)
+    reg = O.regions(shape)
+    for node, n_in in (((0, 0, 0), 1), ((0, 0, 3), 2), ((0, 4, 3), 4), ((5, 4, 3), 8)):
+        w = (1 - s) * (n_in * 0.3 + 8 - n_in) / 8 + s
+        for p in range(3):
+            scaled = reg[node] < 0 or bc.type[reg[node], p] == B.TRACTION
+            if reg[node] >= 0:
+                assert f1[(p,) + node] == pytest.approx(w * f0[(p,) + node] if scaled else f0[(p,) + node], rel=1e-14)
+            assert d1[(p,) + node] == pytest.approx(w * d0[(p,) + node] if scaled else d0[(p,) + node], rel=1e-14)
+    # restriction: linear fields are kept, spikes spread with the reference's weights
+    fz, fy, fx = np.meshgrid(np.arange(8.0), np.arange(6.0), np.arange(4.0), indexing="ij")
+    lin = 1.0 + 0.5 * fx - 2.0 * fy + 0.25 * fz
+    assert np.allclose(O.psi_restrict(lin), lin[::2, ::2, ::2], rtol=0, atol=1e-13)
+    spike = np.zeros((8, 6, 4))
+    spike[2, 3, 2] = 1.0                      # (x, y, z) = (2, 3, 2): a face neighbour of coarse (1, 1, 1) and (1, 2, 1)
+    r = O.psi_restrict(spike)
+    assert r[1, 1, 1] == 1 / 16 and r[1, 2, 1] == 1 / 16 and np.count_nonzero(r) == 2
+    spike = np.zeros((8, 6, 4))
+    spike[3, 2, 0] = 1.0                      # on the low x face: coarse (0, 1, 1) and (0, 1, 2) in their face rule
+    r = O.psi_restrict(spike)
+    assert r[1, 1, 0] == 2 / 16 and r[2, 1, 0] == 2 / 16 and np.count_nonzero(r) == 2
+    spike = np.zeros((8, 6, 4))
+    spike[0, 0, 0] = 1.0                      # the corner cell: coarse (0, 0, 0) only, by copy
+    r = O.psi_restrict(spike)
+    assert r[0, 0, 0] == 1.0 and np.count_nonzero(r) == 1

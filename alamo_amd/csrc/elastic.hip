@@ -65,6 +65,7 @@ struct Vecs {
     T* D = nullptr;  // compact grad-C matrices, 18 x nflag (MajorMinor)
     T* Ct = nullptr; // tiled levels: tile-blocked interior copy of C (ael::TileC)
     T* Cb = nullptr; // tiled levels: C of the boundary nodes, in boundary-node order
+    T* pal = nullptr; // tiled levels with a palette: the table of distinct C rows (ael::kPalStride)
 };
 
 struct Level {
@@ -76,6 +77,8 @@ struct Level {
     int nflag = 0;                   // interior nodes with a nonzero gmask
     bool tiled = false;              // MajorMinor level served by k_stencil_lds (else k_stencil_slab)
     double* psi = nullptr;           // SetPsi weight on this level's cells (null until set)
+    unsigned char* pidx = nullptr;   // palette row of every interior node, tile-blocked (null: no palette)
+    int npal = 0;
     ael::TileGrid tg{}, tg32{};      // its tile grids (chunk lengths) of the FP64 and FP32 launches
 };
 
@@ -222,14 +225,18 @@ void launch_stencil_raw(alamo_el_t h, const Level& l, const T* x, const T* b, co
         // level's C stays in the 256 MB last-level cache between sweeps, which
         // the hint gives up (65^3 FP32 sweep: 11.9 us without, 12.6 us with).
         const ael::TileGrid& G = sizeof(T) == 8 ? l.tg : l.tg32;
-        if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
-            ael::k_stencil_lds<TileT, T, OP, true, true><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
+        if (l.pidx && vecs<T>(lv).pal)
+            ael::k_stencil_lds<TileT, T, OP, true, 2><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
+                vecs<T>(lv).D, l.nflag, out, l.pidx);
+        else if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
+            ael::k_stencil_lds<TileT, T, OP, true, 1><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
                 x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out);
+                vecs<T>(lv).D, l.nflag, out, nullptr);
         else
-            ael::k_stencil_lds<TileT, T, OP, true, false><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
+            ael::k_stencil_lds<TileT, T, OP, true, 0><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
                 x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out);
+                vecs<T>(lv).D, l.nflag, out, nullptr);
         return;
     }
     // compact grad-C matrices in slab order: MajorMinor, masked variant, built for this level
@@ -615,6 +622,82 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
 // LDS-tiled MajorMinor kernel (unless ALAMO_EL_STENCIL=slab-only).
 constexpr int kTiledMin = 32;
 
+// The palette of a tiled level (ael::k_pal_index): the distinct 21-slot
+// tensors of its nodes when there are at most ael::kPalMax of them, found by
+// hashing every node on the device and counting distinct hashes on the host;
+// every interior node's row is then checked slot by slot against its own
+// values, and any mismatch leaves the level on the C copy.
+// ALAMO_EL_PALETTE=0 turns the palette off (cross-checks).
+int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
+    const char* e = std::getenv("ALAMO_EL_PALETTE");
+    if (e && std::strcmp(e, "0") == 0) return 0;
+    const long N = l.L.N;
+    unsigned long long* hs = nullptr;
+    HIPCHK(hipMalloc((void**)&hs, sizeof(unsigned long long) * N));
+    ael::k_c_hash<<<nblk(N), kBlock, 0, s>>>(l.d.C, N, hs);
+    std::vector<unsigned long long> hh(N);
+    int rc = 0;
+    if (hipMemcpyAsync(hh.data(), hs, sizeof(unsigned long long) * N, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipFree(hs);
+        return fail(ALAMO_EL_EHIP, "palette hash copy failed");
+    }
+    std::vector<std::pair<unsigned long long, long>> rows;  // (hash, first node), sorted by hash
+    for (long id = 0; id < N && (int)rows.size() <= ael::kPalMax; ++id) {
+        const unsigned long long v = hh[id];
+        auto it = std::lower_bound(rows.begin(), rows.end(), std::make_pair(v, 0L),
+                                   [](const std::pair<unsigned long long, long>& a,
+                                      const std::pair<unsigned long long, long>& b) { return a.first < b.first; });
+        if (it == rows.end() || it->first != v) rows.insert(it, std::make_pair(v, id));
+    }
+    const int npal = (int)rows.size();
+    if (npal > ael::kPalMax) {
+        (void)hipFree(hs);
+        return 0;
+    }
+    std::vector<unsigned long long> ph(npal);
+    std::vector<long> rep(npal);
+    for (int q = 0; q < npal; ++q) {
+        ph[q] = rows[q].first;
+        rep[q] = rows[q].second;
+    }
+    unsigned long long* ph_d = nullptr;
+    long* rep_d = nullptr;
+    int* bad_d = nullptr;
+    auto done = [&](int r) {
+        for (void* p : {(void*)hs, (void*)ph_d, (void*)rep_d, (void*)bad_d})
+            if (p) (void)hipFree(p);
+        return r;
+    };
+    if (hipMalloc((void**)&ph_d, sizeof(unsigned long long) * npal) != hipSuccess ||
+        hipMalloc((void**)&rep_d, sizeof(long) * npal) != hipSuccess ||
+        hipMalloc((void**)&bad_d, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&l.d.pal, sizeof(double) * npal * ael::kPalStride) != hipSuccess ||
+        hipMalloc((void**)&l.pidx, ael::TileIdx<TileT>::elems(l.L, l.tg)) != hipSuccess)
+        return done(fail(ALAMO_EL_EHIP, "palette allocation failed"));
+    int bad = 0;
+    if (hipMemcpyAsync(ph_d, ph.data(), sizeof(unsigned long long) * npal, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(rep_d, rep.data(), sizeof(long) * npal, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(bad_d, &bad, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
+        return done(fail(ALAMO_EL_EHIP, "palette copy failed"));
+    ael::k_pal_gather<double><<<nblk((long)npal * ael::kPalStride), kBlock, 0, s>>>(l.d.C, N, rep_d, npal, l.d.pal);
+    ael::k_pal_index<TileT><<<(unsigned)ael::tile_planes(l.L, l.tg), TileT::NT, 0, s>>>(
+        l.d.C, l.L, l.tg, hs, ph_d, npal, l.d.pal, l.pidx, bad_d);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&bad, bad_d, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return done(fail(ALAMO_EL_EHIP, "palette build failed"));
+    if (bad) {  // a hash collision: keep the C copy
+        (void)hipFree(l.d.pal);
+        (void)hipFree(l.pidx);
+        l.d.pal = nullptr;
+        l.pidx = nullptr;
+        return done(0);
+    }
+    l.npal = npal;
+    return done(rc);
+}
+
 int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     if (h->sym != 1) return 0;
     l.tiled = stencil_variant() == kVariantSlab && l.L.nx - 2 >= kTiledMin && l.L.ny - 2 >= kTiledMin &&
@@ -629,8 +712,13 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
                                  : ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
     const long nw = (long)nib * wpb;
     if (l.dwbase) (void)hipFree(l.dwbase);
-    for (void* p : {(void*)l.d.D, (void*)l.f.D, (void*)l.d.Ct, (void*)l.d.Cb, (void*)l.f.Ct, (void*)l.f.Cb})
+    for (void* p : {(void*)l.d.D, (void*)l.f.D, (void*)l.d.Ct, (void*)l.d.Cb, (void*)l.f.Ct, (void*)l.f.Cb,
+                    (void*)l.d.pal, (void*)l.f.pal, (void*)l.pidx})
         if (p) (void)hipFree(p);
+    l.d.pal = nullptr;
+    l.f.pal = nullptr;
+    l.pidx = nullptr;
+    l.npal = 0;
     l.dwbase = nullptr;
     l.d.D = nullptr;
     l.f.D = nullptr;
@@ -645,6 +733,8 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
                                                                                                    l.d.Ct);
         ael::k_pack_bnd_c<double><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.d.Cb);
         HIPCHK(hipGetLastError());
+        int rc = build_palette(h, l, s);
+        if (rc) return rc;
     }
     if (nw == 0) return 0;
     HIPCHK(hipMalloc((void**)&l.dwbase, sizeof(int) * nw));
@@ -733,6 +823,13 @@ int prepare32(alamo_el_t h, hipStream_t s) {
             ael::k_pack_tile_c<TileT, float><<<(unsigned)ael::tile_planes(l.L, l.tg), TileT::NT, 0, s>>>(l.d.C, l.L,
                                                                                                      l.tg, l.f.Ct);
             ael::k_pack_bnd_c<float><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.f.Cb);
+        }
+        if (l.f.pal) (void)hipFree(l.f.pal);
+        l.f.pal = nullptr;
+        if (l.d.pal) {  // the FP32 table: the same rows rounded as the C copy rounds them
+            const long np = (long)l.npal * ael::kPalStride;
+            HIPCHK(hipMalloc((void**)&l.f.pal, sizeof(float) * np));
+            ael::k_cast<double, float><<<nblk(np), kBlock, 0, s>>>(l.d.pal, l.f.pal, np);
         }
         if (l.d.D) {
             if (l.f.D) (void)hipFree(l.f.D);
@@ -975,10 +1072,13 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
 int alamo_el_destroy(alamo_el_t h) {
     if (!h) return 0;
     for (auto& l : h->lv) {
-        for (double* p : {l.d.C, l.d.diag, l.d.cor, l.d.cor_hold, l.d.res, l.d.rescor, l.d.tmp, l.d.D, l.d.Ct, l.d.Cb})
+        for (double* p : {l.d.C, l.d.diag, l.d.cor, l.d.cor_hold, l.d.res, l.d.rescor, l.d.tmp, l.d.D, l.d.Ct, l.d.Cb,
+                          l.d.pal})
             if (p) (void)hipFree(p);
-        for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp, l.f.D, l.f.Ct, l.f.Cb})
+        for (float* p : {l.f.C, l.f.diag, l.f.cor, l.f.cor_hold, l.f.res, l.f.rescor, l.f.tmp, l.f.D, l.f.Ct, l.f.Cb,
+                         l.f.pal})
             if (p) (void)hipFree(p);
+        if (l.pidx) (void)hipFree(l.pidx);
         if (l.gmask) (void)hipFree(l.gmask);
         if (l.dwbase) (void)hipFree(l.dwbase);
         if (l.psi) (void)hipFree(l.psi);
@@ -1081,6 +1181,12 @@ int alamo_el_set_psi(alamo_el_t h, const double* psi, double psi_small, hipStrea
     h->psi_small = psi_small;
     h->prepared = false;
     h->prepared32 = false;
+    return 0;
+}
+
+int alamo_el_palette_size(alamo_el_t h, int mglev, int* npal) {
+    if (bad_level(h, mglev) || !npal) return fail(ALAMO_EL_EINVAL, "bad argument");
+    *npal = h->lv[mglev].pidx ? h->lv[mglev].npal : 0;
     return 0;
 }
 

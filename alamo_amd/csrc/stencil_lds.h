@@ -76,6 +76,98 @@ __global__ void __launch_bounds__(TS::NT) k_pack_tile_c(const double* __restrict
     T* blk = Ct + TileC<TS>::block(G, k, it, jt) + threadIdx.x;
     for (int s = 0; s < TileC<TS>::NC; ++s) blk[(long)s * TS::NT] = v ? (T)C[s * L.N + id] : T(0);
 }
+// Palette of a level's C: when the interior nodes' 21-slot tensors take at
+// most kPalMax distinct values (a polycrystal: one per grain), the tiled
+// kernel reads a one-byte index per node, tile-blocked like TileC, and the
+// tensor from a table of kPalStride-value rows that stays in the caches.
+// The table holds the nodes' own values, so every product is the same.
+constexpr int kPalMax = 256;
+constexpr int kPalStride = 24;  // row length: 21 slots, padded to 16-byte multiples in FP64 and FP32
+template <class TS>
+struct TileIdx {
+    __host__ __device__ static long block(const TileGrid& G, int k, int it, int jt) {
+        return (((long)(k - 1) * G.nty + jt) * G.ntx + it) * TS::NT;
+    }
+    static long elems(const Lvl& L, const TileGrid& G) { return (long)(L.nz - 2) * G.nty * G.ntx * TS::NT; }
+};
+// 64-bit hash of each node's 21 slots (bit patterns).
+__global__ void __launch_bounds__(kBlock) k_c_hash(const double* __restrict__ C, long N,
+                                                   unsigned long long* __restrict__ hs) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= N) return;
+    unsigned long long h = 0xcbf29ce484222325ull;
+    for (int q = 0; q < 21; ++q) {
+        h ^= (unsigned long long)__double_as_longlong(C[q * N + id]);
+        h *= 0x100000001b3ull;
+        h ^= h >> 31;
+    }
+    hs[id] = h;
+}
+// Row q of the table from node rep[q] (rows padded with zeros).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pal_gather(const double* __restrict__ C, long N, const long* __restrict__ rep,
+                                                       int npal, T* __restrict__ pal) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npal * kPalStride) return;
+    const int q = t / kPalStride, s = t % kPalStride;
+    pal[t] = s < 21 ? (T)C[s * N + rep[q]] : T(0);
+}
+// Tile-blocked index of every interior node: the row whose hash matches
+// (ph sorted, binary search), checked slot by slot against the node's own
+// values; a mismatch (hash collision) sets bad[0].  Grid as k_pack_tile_c.
+template <class TS>
+__global__ void __launch_bounds__(TS::NT) k_pal_index(const double* __restrict__ C, Lvl L, TileGrid G,
+                                                      const unsigned long long* __restrict__ hs,
+                                                      const unsigned long long* __restrict__ ph, int npal,
+                                                      const double* __restrict__ pal, unsigned char* __restrict__ pidx,
+                                                      int* __restrict__ bad) {
+    const long tp = blockIdx.x;
+    const int it = (int)(tp % G.ntx), jt = (int)(tp / G.ntx % G.nty), k = 1 + (int)(tp / ((long)G.ntx * G.nty));
+    const int i = 1 + it * TS::TX + (int)threadIdx.x % TS::TX, j = 1 + jt * TS::TY + (int)threadIdx.x / TS::TX;
+    unsigned char q8 = 0;
+    if (i <= L.nx - 2 && j <= L.ny - 2) {
+        const long id = ((long)k * L.ny + j) * L.nx + i;
+        const unsigned long long h = hs[id];
+        int lo = 0, hi = npal - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ph[mid] < h) lo = mid + 1;
+            else hi = mid;
+        }
+        bool ok = ph[lo] == h;
+        for (int q = 0; q < 21 && ok; ++q)
+            ok = __double_as_longlong(C[q * L.N + id]) == __double_as_longlong(pal[lo * kPalStride + q]);
+        if (!ok) bad[0] = 1;
+        q8 = (unsigned char)lo;
+    }
+    pidx[TileIdx<TS>::block(G, k, it, jt) + threadIdx.x] = q8;
+}
+// The 21 slots of palette row pr (16-byte loads).
+template <typename T>
+__device__ __forceinline__ void pal_row(const T* __restrict__ pr, T cv[21]) {
+    if constexpr (sizeof(T) == 8) {
+        const double2* p2 = reinterpret_cast<const double2*>(pr);
+#pragma unroll
+        for (int v = 0; v < 10; ++v) {
+            const double2 t = p2[v];
+            cv[2 * v] = t.x;
+            cv[2 * v + 1] = t.y;
+        }
+        cv[20] = pr[20];
+    } else {
+        const float4* p4 = reinterpret_cast<const float4*>(pr);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            const float4 t = p4[v];
+            cv[4 * v] = t.x;
+            cv[4 * v + 1] = t.y;
+            cv[4 * v + 2] = t.z;
+            cv[4 * v + 3] = t.w;
+        }
+        cv[20] = pr[20];
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pack_bnd_c(const double* __restrict__ C, Lvl L, int nc,
                                                        T* __restrict__ Cb) {
@@ -199,12 +291,16 @@ __device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PR
     }
 }
 
-template <class TS, typename T, int OP, bool DC, bool NTC>
+// CM: how the interior nodes read C -- 0 the tile-blocked copy Ct, 1 the same
+// with the non-temporal hint, 2 the palette (Ct is then the table, pidx the
+// tile-blocked row indices).
+template <class TS, typename T, int OP, bool DC, int CM>
 __global__ void __launch_bounds__(TS::NT)
     k_stencil_lds(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
                   const T* __restrict__ Ct, const T* __restrict__ Cb, const int* __restrict__ bct, int uniform,
                   Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
-                  const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out) {
+                  const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out,
+                  const unsigned char* __restrict__ pidx) {
     __shared__ T su[3][TS::RING];  // [ring slot][component][hy][hx], padded to RING
     // The nbb boundary workgroups (a multiple of kXcd, so that tile ib keeps
     // ib % kXcd == XCD) come after the tile ones in FP64 and before them in
@@ -266,6 +362,9 @@ __global__ void __launch_bounds__(TS::NT)
             tile_store<TS, T>(su[s], pre);
         }
     }
+    // palette: the row index of plane kfirst now, each next plane's one step ahead
+    unsigned pi_next = 0;
+    if constexpr (CM == 2) pi_next = pidx[TileIdx<TS>::block(G, kfirst, it, jt) + threadIdx.x];
     unsigned long long gpk = 0;
     int dwb_lane = 0;
     if constexpr (DC) {
@@ -297,14 +396,22 @@ __global__ void __launch_bounds__(TS::NT)
         const int sm = d > 0 ? sb : sa, sp = d > 0 ? sa : sb;         // slots of planes k-1 and k+1
         const bool more = kk + 1 < nstep;
         const unsigned o = (unsigned)k * sxy + col;
-        const T* cblk = Ct + TileC<TS>::block(G, k, it, jt);  // this (tile, plane) of the C copy
-
         // every global load of the plane leaves here, before the LDS work
         T cv[21], dv[18], pre[TS::PRE], bv[3], gv[3];
+        unsigned pi = 0;
+        if constexpr (CM == 2) {
+            pi = pi_next;
+            if (more) pi_next = pidx[TileIdx<TS>::block(G, k + d, it, jt) + threadIdx.x];
+        }
         if (valid) {
+            if constexpr (CM == 2) {
+                pal_row<T>(Ct + (size_t)pi * kPalStride, cv);
+            } else {
+                const T* cblk = Ct + TileC<TS>::block(G, k, it, jt);  // this (tile, plane) of the C copy
 #pragma unroll
-            for (int s = 0; s < 21; ++s)
-                cv[s] = NTC ? ld32nt(cblk + s * TS::NT, threadIdx.x) : ld32(cblk + s * TS::NT, threadIdx.x);
+                for (int s = 0; s < 21; ++s)
+                    cv[s] = CM == 1 ? ld32nt(cblk + s * TS::NT, threadIdx.x) : ld32(cblk + s * TS::NT, threadIdx.x);
+            }
 #pragma unroll
             for (int p = 0; p < 3; ++p) bv[p] = (OP >= 1) ? ld32(b + (size_t)p * N, o) : T(0);
         }

@@ -61,7 +61,7 @@ EXPORTS = [
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
-    "alamo_el_tag_bbox", "alamo_amr_fillpatch", "alamo_el_set_psi",
+    "alamo_el_tag_bbox", "alamo_amr_fillpatch", "alamo_el_set_psi", "alamo_el_palette_size",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -88,6 +88,7 @@ def lib():
         L.alamo_el_set_omega.argtypes = [vp, ctypes.c_double]
         L.alamo_el_set_model.argtypes = [vp, vp, ctypes.c_int, vp]
         L.alamo_el_set_psi.argtypes = [vp, vp, ctypes.c_double, vp]
+        L.alamo_el_palette_size.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.alamo_el_prepare.argtypes = [vp, vp]
         L.alamo_el_fapply.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_diagonal.argtypes = [vp, ctypes.c_int, vp, vp]
@@ -252,6 +253,12 @@ class ElasticOperator:
 
     def prepare(self):
         _check(lib().alamo_el_prepare(self._h, _stream()), "alamo_el_prepare")
+
+    def palette_size(self, mglev: int = 0) -> int:
+        """Rows of the level's C palette (0: the level reads its C copy)."""
+        n = ctypes.c_int()
+        _check(lib().alamo_el_palette_size(self._h, int(mglev), ctypes.byref(n)), "alamo_el_palette_size")
+        return n.value
 
     # -- operator pieces (mglev 0 = finest) -----------------------------------
     def _v(self, t, mglev: int, ncomp: int = 3):

@@ -151,6 +151,14 @@ int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t st
  * psi_small is 1e-8.  A solve with psi needs a coarsest level of at most 1536
  * unknowns or ALAMO_EL_BOTTOM_DIRECT. */
 int alamo_el_set_psi(alamo_el_t h, const double* psi, double psi_small, hipStream_t stream);
+/* Diagnostic, no counterpart in the reference: the number of rows of MG level
+ * mglev's C palette (0 when the level reads its C copy).  A MajorMinor level
+ * served by the LDS-tiled kernel whose nodes take at most 256 distinct
+ * stiffness tensors (a polycrystal: one per grain) reads a one-byte row index
+ * per interior node and the tensor from that table instead of 21 values per
+ * node; the products are the same.  ALAMO_EL_PALETTE=0 in the environment at
+ * alamo_el_set_model turns it off. */
+int alamo_el_palette_size(alamo_el_t h, int mglev, int* npal);
 /* prepareForSolve: averageDownCoeffsSameAmrLevel on every MG level and
  * Diagonal(true) (src/Operator/Operator.cpp:253-260,
  * src/Operator/Elastic.cpp:605-626,746-910). */

@@ -181,6 +181,9 @@ def _tiled_case(field, seed):
     op.SetBC(bc)
     op.SetUniform(uniform)
     op.SetModel(dev(C))
+    # the grain and uniform fields take the palette (one row per distinct tensor), the random one cannot
+    assert op.palette_size() == _palette_rows(C)
+    assert (op.palette_size() > 0) == (field != "random")
     f = host(op.Fapply(dev(u)))
     assert relerr(f, O.fapply(u, C, sym, bc, dx, uniform=uniform)) < REL
     r = host(op.residual(dev(u), dev(b)))
@@ -190,6 +193,43 @@ def _tiled_case(field, seed):
     op.Fsmooth(xd, dev(b), nsmooth=2)
     diag = O.diagonal(C, sym, bc, dx, shape)
     assert relerr(host(xd), O.fsmooth(u, b, C, sym, bc, dx, diag, nsmooth=2, uniform=uniform)) < REL
+
+
+def _palette_rows(C):
+    """Distinct stiffness tensors among the nodes, 0 above the palette's 256 rows."""
+    n = len(np.unique(np.ascontiguousarray(C.reshape(C.shape[0], -1).T), axis=0))
+    return n if n <= 256 else 0
+
+
+def test_palette_gives_the_bits_of_the_c_copy(monkeypatch):
+    """The tiled kernel reading C through the palette (one byte a node and a
+    table of the distinct tensors) against the same kernel reading the
+    tile-blocked C copy (ALAMO_EL_PALETTE=0): Fapply, residual and Fsmooth in
+    FP64, and a solve with FP32 cycles, bit for bit, on a 41^3-node
+    polycrystal of 16 grains."""
+    el = _el()
+    from alamo_amd import problems as P
+    p = P.polycrystal(40, ngrains=16, seed=5)
+    rng = np.random.default_rng(9)
+    shape = (41, 41, 41)
+    u = dev(rng.standard_normal((3,) + shape))
+    b = dev(rng.standard_normal((3,) + shape))
+    out = {}
+    for mode in ("palette", "copy"):
+        if mode == "copy":
+            monkeypatch.setenv("ALAMO_EL_PALETTE", "0")
+        op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
+        op.SetBC(p["bc"])
+        op.SetModel(dev(p["C"]))
+        assert op.palette_size() == (_palette_rows(p["C"]) if mode == "palette" else 0) and _palette_rows(p["C"]) > 1
+        op.prepare()
+        xs = u.clone()
+        op.Fsmooth(xs, b, nsmooth=2)
+        x = torch.zeros_like(b)
+        el.Linear(op, precision="fp32", cycle="V", max_iter=4, abort_on_fail=False).solve(x, b, 1e-12, 0.0)
+        out[mode] = [host(op.Fapply(u)), host(op.residual(u, b)), host(xs), host(x)]
+    for a, c in zip(out["palette"], out["copy"]):
+        assert np.array_equal(a, c)
 
 
 def test_transfers_and_coefficient_averaging_match_oracle():

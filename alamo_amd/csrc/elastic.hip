@@ -667,6 +667,12 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
     auto done = [&](int r) {
         for (void* p : {(void*)hs, (void*)ph_d, (void*)rep_d, (void*)bad_d})
             if (p) (void)hipFree(p);
+        if (r) {  // an error leaves the level on its C copy, with nothing half built
+            if (l.d.pal) (void)hipFree(l.d.pal);
+            if (l.pidx) (void)hipFree(l.pidx);
+            l.d.pal = nullptr;
+            l.pidx = nullptr;
+        }
         return r;
     };
     if (hipMalloc((void**)&ph_d, sizeof(unsigned long long) * npal) != hipSuccess ||

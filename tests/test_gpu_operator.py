@@ -386,3 +386,31 @@ def test_psi_weighted_operator_matches_oracle(sym):
         xd = dev(u)
         op.Fsmooth(xd, dev(b), mglev=mglev, nsmooth=2)
         assert relerr(host(xd), O.fsmooth_psi(u, b, Cl, sym, bc, h, pl, small, nsmooth=2)) < REL
+
+
+def test_psi_on_a_tiled_palette_level_matches_oracle():
+    """The psi weight on a level served by the LDS-tiled kernel through the C
+    palette (a 37 x 35 x 41-node grain field): Fapply, the residual and
+    Fsmooth match the oracle's psi restatement."""
+    el = _el()
+    rng = np.random.default_rng(31)
+    n_cell = (36, 34, 40)
+    shape = (41, 35, 37)
+    dx = (1.0 / 36, 0.9 / 34, 1.1 / 40)
+    sym = M.MAJORMINOR
+    C = grain_C(shape, rng)
+    bc = rand_bc(rng)
+    psi = rng.uniform(0.1, 1.0, (shape[0] - 1, shape[1] - 1, shape[2] - 1))
+    u = rng.standard_normal((3,) + shape)
+    b = rng.standard_normal((3,) + shape)
+    op = el.ElasticOperator(n_cell, dx, sym)
+    op.SetBC(bc)
+    op.SetModel(dev(C))
+    op.SetPsi(dev(psi), 1e-4)
+    assert op.palette_size() == _palette_rows(C) > 0
+    assert relerr(host(op.Fapply(dev(u))), O.fapply_psi(u, C, sym, bc, dx, psi, 1e-4)) < REL
+    assert relerr(host(op.residual(dev(u), dev(b))), O.residual_psi(u, b, C, sym, bc, dx, psi, 1e-4)) < REL
+    op.prepare()
+    xd = dev(u)
+    op.Fsmooth(xd, dev(b), nsmooth=1)
+    assert relerr(host(xd), O.fsmooth_psi(u, b, C, sym, bc, dx, psi, 1e-4, nsmooth=1)) < REL

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/alamo_elastic.h"
@@ -77,7 +78,7 @@ struct Level {
     int nflag = 0;                   // interior nodes with a nonzero gmask
     bool tiled = false;              // MajorMinor level served by k_stencil_lds (else k_stencil_slab)
     double* psi = nullptr;           // SetPsi weight on this level's cells (null until set)
-    unsigned char* pidx = nullptr;   // palette row of every interior node, tile-blocked (null: no palette)
+    ael::PalIdx* pidx = nullptr;     // palette row of every interior node, tile-blocked (null: no palette)
     int npal = 0;
     ael::TileGrid tg{}, tg32{};      // its tile grids (chunk lengths) of the FP64 and FP32 launches
 };
@@ -623,15 +624,17 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
 constexpr int kTiledMin = 32;
 
 // The palette of a tiled level (ael::k_pal_index): the distinct 21-slot
-// tensors of its nodes when there are at most ael::kPalMax of them, found by
-// hashing every node on the device and counting distinct hashes on the host;
-// every interior node's row is then checked slot by slot against its own
-// values, and any mismatch leaves the level on the C copy.
+// tensors of its nodes when there are at most ael::kPalMax of them and at most
+// one per 16 nodes (a table near the size of C itself would only add a
+// gather), found by hashing every node on the device and counting distinct
+// hashes on the host; every interior node's row is then checked slot by slot
+// against its own values, and any mismatch leaves the level on the C copy.
 // ALAMO_EL_PALETTE=0 turns the palette off (cross-checks).
 int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
     const char* e = std::getenv("ALAMO_EL_PALETTE");
     if (e && std::strcmp(e, "0") == 0) return 0;
     const long N = l.L.N;
+    const long limit = std::min<long>(ael::kPalMax, N / 16);
     unsigned long long* hs = nullptr;
     HIPCHK(hipMalloc((void**)&hs, sizeof(unsigned long long) * N));
     ael::k_c_hash<<<nblk(N), kBlock, 0, s>>>(l.d.C, N, hs);
@@ -642,19 +645,15 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
         (void)hipFree(hs);
         return fail(ALAMO_EL_EHIP, "palette hash copy failed");
     }
-    std::vector<std::pair<unsigned long long, long>> rows;  // (hash, first node), sorted by hash
-    for (long id = 0; id < N && (int)rows.size() <= ael::kPalMax; ++id) {
-        const unsigned long long v = hh[id];
-        auto it = std::lower_bound(rows.begin(), rows.end(), std::make_pair(v, 0L),
-                                   [](const std::pair<unsigned long long, long>& a,
-                                      const std::pair<unsigned long long, long>& b) { return a.first < b.first; });
-        if (it == rows.end() || it->first != v) rows.insert(it, std::make_pair(v, id));
-    }
-    const int npal = (int)rows.size();
-    if (npal > ael::kPalMax) {
+    std::unordered_map<unsigned long long, long> first;  // hash -> first node
+    for (long id = 0; id < N && (long)first.size() <= limit; ++id) first.emplace(hh[id], id);
+    if ((long)first.size() > limit) {
         (void)hipFree(hs);
         return 0;
     }
+    std::vector<std::pair<unsigned long long, long>> rows(first.begin(), first.end());
+    std::sort(rows.begin(), rows.end());
+    const int npal = (int)rows.size();
     std::vector<unsigned long long> ph(npal);
     std::vector<long> rep(npal);
     for (int q = 0; q < npal; ++q) {
@@ -679,7 +678,7 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
         hipMalloc((void**)&rep_d, sizeof(long) * npal) != hipSuccess ||
         hipMalloc((void**)&bad_d, sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&l.d.pal, sizeof(double) * npal * ael::kPalStride) != hipSuccess ||
-        hipMalloc((void**)&l.pidx, ael::TileIdx<TileT>::elems(l.L, l.tg)) != hipSuccess)
+        hipMalloc((void**)&l.pidx, sizeof(ael::PalIdx) * ael::TileIdx<TileT>::elems(l.L, l.tg)) != hipSuccess)
         return done(fail(ALAMO_EL_EHIP, "palette allocation failed"));
     int bad = 0;
     if (hipMemcpyAsync(ph_d, ph.data(), sizeof(unsigned long long) * npal, hipMemcpyHostToDevice, s) != hipSuccess ||

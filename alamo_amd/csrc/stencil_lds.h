@@ -76,12 +76,14 @@ __global__ void __launch_bounds__(TS::NT) k_pack_tile_c(const double* __restrict
     T* blk = Ct + TileC<TS>::block(G, k, it, jt) + threadIdx.x;
     for (int s = 0; s < TileC<TS>::NC; ++s) blk[(long)s * TS::NT] = v ? (T)C[s * L.N + id] : T(0);
 }
-// Palette of a level's C: when the interior nodes' 21-slot tensors take at
-// most kPalMax distinct values (a polycrystal: one per grain), the tiled
-// kernel reads a one-byte index per node, tile-blocked like TileC, and the
-// tensor from a table of kPalStride-value rows that stays in the caches.
-// The table holds the nodes' own values, so every product is the same.
-constexpr int kPalMax = 256;
+// Palette of a level's C: when the nodes' 21-slot tensors take few distinct
+// values (a polycrystal: one per grain; its averaged coarse levels: the grains
+// and their boundary mixtures), the tiled kernel reads a two-byte index per
+// node, tile-blocked like TileC, and the tensor from a table of
+// kPalStride-value rows that stays in the caches.  The table holds the nodes'
+// own values, so every product is the same.
+using PalIdx = unsigned short;
+constexpr int kPalMax = 65536;
 constexpr int kPalStride = 24;  // row length: 21 slots, padded to 16-byte multiples in FP64 and FP32
 template <class TS>
 struct TileIdx {
@@ -119,12 +121,12 @@ template <class TS>
 __global__ void __launch_bounds__(TS::NT) k_pal_index(const double* __restrict__ C, Lvl L, TileGrid G,
                                                       const unsigned long long* __restrict__ hs,
                                                       const unsigned long long* __restrict__ ph, int npal,
-                                                      const double* __restrict__ pal, unsigned char* __restrict__ pidx,
+                                                      const double* __restrict__ pal, PalIdx* __restrict__ pidx,
                                                       int* __restrict__ bad) {
     const long tp = blockIdx.x;
     const int it = (int)(tp % G.ntx), jt = (int)(tp / G.ntx % G.nty), k = 1 + (int)(tp / ((long)G.ntx * G.nty));
     const int i = 1 + it * TS::TX + (int)threadIdx.x % TS::TX, j = 1 + jt * TS::TY + (int)threadIdx.x / TS::TX;
-    unsigned char q8 = 0;
+    PalIdx q8 = 0;
     if (i <= L.nx - 2 && j <= L.ny - 2) {
         const long id = ((long)k * L.ny + j) * L.nx + i;
         const unsigned long long h = hs[id];
@@ -138,7 +140,7 @@ __global__ void __launch_bounds__(TS::NT) k_pal_index(const double* __restrict__
         for (int q = 0; q < 21 && ok; ++q)
             ok = __double_as_longlong(C[q * L.N + id]) == __double_as_longlong(pal[lo * kPalStride + q]);
         if (!ok) bad[0] = 1;
-        q8 = (unsigned char)lo;
+        q8 = (PalIdx)lo;
     }
     pidx[TileIdx<TS>::block(G, k, it, jt) + threadIdx.x] = q8;
 }
@@ -300,7 +302,7 @@ __global__ void __launch_bounds__(TS::NT)
                   const T* __restrict__ Ct, const T* __restrict__ Cb, const int* __restrict__ bct, int uniform,
                   Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
                   const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out,
-                  const unsigned char* __restrict__ pidx) {
+                  const PalIdx* __restrict__ pidx) {
     __shared__ T su[3][TS::RING];  // [ring slot][component][hy][hx], padded to RING
     // The nbb boundary workgroups (a multiple of kXcd, so that tile ib keeps
     // ib % kXcd == XCD) come after the tile ones in FP64 and before them in

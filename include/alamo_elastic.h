@@ -153,10 +153,11 @@ int alamo_el_set_model(alamo_el_t h, const double* C, int layout, hipStream_t st
 int alamo_el_set_psi(alamo_el_t h, const double* psi, double psi_small, hipStream_t stream);
 /* Diagnostic, no counterpart in the reference: the number of rows of MG level
  * mglev's C palette (0 when the level reads its C copy).  A MajorMinor level
- * served by the LDS-tiled kernel whose nodes take at most 256 distinct
- * stiffness tensors (a polycrystal: one per grain) reads a one-byte row index
- * per interior node and the tensor from that table instead of 21 values per
- * node; the products are the same.  ALAMO_EL_PALETTE=0 in the environment at
+ * served by the LDS-tiled kernel whose nodes take at most 65536 distinct
+ * stiffness tensors, and at most one per 16 nodes (a polycrystal: one per
+ * grain, plus boundary mixtures on its averaged levels), reads a two-byte row
+ * index per interior node and the tensor from that table instead of 21 values
+ * per node; the products are the same.  ALAMO_EL_PALETTE=0 in the environment at
  * alamo_el_set_model turns it off. */
 int alamo_el_palette_size(alamo_el_t h, int mglev, int* npal);
 /* prepareForSolve: averageDownCoeffsSameAmrLevel on every MG level and

@@ -646,7 +646,12 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
         return fail(ALAMO_EL_EHIP, "palette hash copy failed");
     }
     std::unordered_map<unsigned long long, long> first;  // hash -> first node
-    for (long id = 0; id < N && (long)first.size() <= limit; ++id) first.emplace(hh[id], id);
+    unsigned long long prev = 0;
+    for (long id = 0; id < N && (long)first.size() <= limit; ++id) {
+        if (id > 0 && hh[id] == prev) continue;  // runs along x inside a grain: one lookup per run
+        prev = hh[id];
+        first.emplace(prev, id);
+    }
     if ((long)first.size() > limit) {
         (void)hipFree(hs);
         return 0;

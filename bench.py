@@ -154,14 +154,22 @@ def main():
     op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
     op.SetBC(p["bc"])
     C = torch.from_numpy(p["C"]).cuda()
-    op.SetModel(C)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.SetModel(C)  # gradient masks, compact grad-C matrices, C copies, palette
+    torch.cuda.synchronize()
+    set_model_s = time.perf_counter() - t0
     shape = op.level_shape(0)
     nodes = int(np.prod(shape))
     b = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
     u0 = torch.zeros_like(b)
     rhs = op.newton_rhs(C, u0, b)
-    op.prepare()
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.prepare()  # coarse levels and the direct bottom's inverse
+    torch.cuda.synchronize()
+    prepare_s = time.perf_counter() - t0
+    log(f"set-up once per model: SetModel {set_model_s * 1e3:.0f} ms, prepare {prepare_s * 1e3:.0f} ms")
 
     # --- Fapply roofline (HIP events on the stream the kernels run on) ------
     u = torch.randn((3,) + shape, dtype=torch.float64, device="cuda")
@@ -296,6 +304,7 @@ def main():
         "speedup_vs_plain_fp64_V": (plain["V"]["seconds"] / fast_t) if "V" in plain else None,
         "network_gain_vs_fp32_only": fp32_only["seconds"] / fast_t if learned else None,
         "speedup_vs_plain_fp64_F": (plain["F"]["seconds"] / fast_t) if "F" in plain else None,
+        "setup_once_per_model_s": {"set_model": set_model_s, "prepare": prepare_s},
         "fapply": {"us_per_launch": fapply_ms * 1e3, "gbs": fapply_gbs, "bytes_per_node": FAPPLY_BYTES_PER_NODE[p["sym"]],
                    "c_palette_rows": palette_rows},
         "roofline": {"bound": "hbm", "achieved": fapply_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -310,6 +310,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": fapply_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": fapply_gbs / PEAK_HBM_GBS, "traffic": traffic_bytes,
                      "kernel": "k_stencil_lds<Tile<32,8>,double,Fapply,compact grad-C,C palette>",
+                     "note": ("achieved counts the reference layout's 216 B/node; with C read through the palette "
+                              "the kernel moves `traffic` bytes, so frac near 1 is not an HBM bound"),
                      "algorithmic_bytes": bytes_per_launch,
                      "traffic_passes": traffic},
         "cpu_baseline": cpu,

@@ -544,10 +544,15 @@ int assemble_dense(alamo_el_t h, hipStream_t s) {
 
 // The direct bottom level: the finest MG level with at most kDirectMax
 // unknowns (-1 when even the coarsest has more).
+// ALAMO_EL_DIRECT_MAX lowers the limit (read at every prepare): a caller whose
+// model changes every step trades a few coarse sweeps per cycle for a smaller
+// inversion (at 129^3: 375 unknowns on the 5^3 level instead of 2187 on 9^3).
 int direct_level_of(alamo_el_t h) {
+    long lim = kDirectMax;
+    if (const char* e = std::getenv("ALAMO_EL_DIRECT_MAX")) lim = std::min<long>(lim, std::max(0L, std::atol(e)));
     int l = (int)h->lv.size() - 1;
-    if (3 * h->lv[l].L.N > kDirectMax) return -1;
-    while (l > 0 && 3 * h->lv[l - 1].L.N <= kDirectMax) --l;
+    if (3 * h->lv[l].L.N > lim) return -1;
+    while (l > 0 && 3 * h->lv[l - 1].L.N <= lim) --l;
     return l;
 }
 

@@ -94,8 +94,11 @@ def test_polycrystal_fp64_matches_oracle(n_cell):
 # coarsened to the same level (max_coarsening_level) with its BiCGStab bottom
 # run to 1e-14 on those 2187 unknowns (a direct solve to rounding): the two
 # agree to 1e-10.
-@pytest.mark.parametrize("n_cell,coarsenings", [(16, 1), (32, 2)])
-def test_direct_bottom_cycles_match_oracle(n_cell, coarsenings):
+# ALAMO_EL_DIRECT_MAX=375 moves the direct bottom to the 5^3 level (375 unknowns).
+@pytest.mark.parametrize("n_cell,coarsenings,direct_max", [(16, 1, None), (32, 2, None), (16, 2, "375")])
+def test_direct_bottom_cycles_match_oracle(n_cell, coarsenings, direct_max, monkeypatch):
+    if direct_max:
+        monkeypatch.setenv("ALAMO_EL_DIRECT_MAX", direct_max)
     p = P.polycrystal(n_cell, ngrains=8, seed=4)
     el, op, rhs = gpu_setup(p)
     rhs_o = oracle_rhs(p)

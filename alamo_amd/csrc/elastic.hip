@@ -123,7 +123,10 @@ struct alamo_el_s {
     double* direct_A = nullptr;     // operator, destroyed by the inversion
     double* direct_Ainv = nullptr;  // inverse, column-major (null when singular)
     float* direct_Ainv32 = nullptr; // the inverse rounded to FP32, for the FP32 cycles
-    double* direct_f = nullptr;     // Gauss-Jordan multipliers
+    double* direct_f = nullptr;     // blocked Gauss-Jordan: the panel's transform (M x ael::kBgjMaxB)
+    double* direct_A2 = nullptr;    // second copies of the operator and the inverse (the update
+    double* direct_X2 = nullptr;    //   of each panel reads one and writes the other)
+    int* direct_piv = nullptr;      // the panel's row map (M)
     double* direct_part = nullptr;  // partial sums of the bottom product
     ael::GjState* direct_st = nullptr;
     // FGMRES storage, allocated on demand
@@ -558,9 +561,12 @@ int direct_level_of(alamo_el_t h) {
 
 void free_direct(alamo_el_t h) {
     for (void* p : {(void*)h->direct_I, (void*)h->direct_A, (void*)h->direct_Ainv, (void*)h->direct_f,
-                    (void*)h->direct_part, (void*)h->direct_st, (void*)h->direct_Ainv32})
+                    (void*)h->direct_part, (void*)h->direct_st, (void*)h->direct_Ainv32, (void*)h->direct_A2,
+                    (void*)h->direct_X2, (void*)h->direct_piv})
         if (p) (void)hipFree(p);
     h->direct_I = h->direct_A = h->direct_Ainv = h->direct_f = h->direct_part = nullptr;
+    h->direct_A2 = h->direct_X2 = nullptr;
+    h->direct_piv = nullptr;
     h->direct_Ainv32 = nullptr;
     h->direct_st = nullptr;
     h->direct_M = 0;
@@ -583,7 +589,10 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
         ALLOC(h->direct_I, (long)M * M);
         ALLOC(h->direct_A, (long)M * M);
         ALLOC(h->direct_Ainv, (long)M * M);
-        ALLOC(h->direct_f, M);
+        ALLOC(h->direct_f, (long)M * ael::kBgjMaxB);
+        ALLOC(h->direct_A2, (long)M * M);
+        ALLOC(h->direct_X2, (long)M * M);
+        ALLOC(h->direct_piv, M);
         ALLOC(h->direct_part, (long)((M + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk) * M);
         ALLOC(h->direct_st, 1);
         std::vector<double> I((size_t)M * M, 0.0);
@@ -602,13 +611,23 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
     HIPCHK(hipMemcpyAsync(h->direct_Ainv, h->direct_I, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
     const ael::GjState zero{0, 0, 0.0};
     HIPCHK(hipMemcpyAsync(h->direct_st, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
-    for (int c = 0; c < M; ++c) {
-        ael::k_gj_pivot<<<1, ael::kGjBlock, 0, s>>>(h->direct_A, M, c, h->direct_f, h->direct_st);
-        ael::k_gj_swap_scale<<<nblk(2L * M), kBlock, 0, s>>>(h->direct_A, h->direct_Ainv, M, c, h->direct_st);
-        ael::k_gj_eliminate<<<dim3(nblk(M), 2 * M - c), kBlock, 0, s>>>(h->direct_A, h->direct_Ainv, M, c,
-                                                                        h->direct_f);
+    // blocked Gauss-Jordan with partial pivoting: panels of bb columns, each
+    // factored in LDS and applied to the rest of [A | inv] in one pass
+    const int bb = std::max(1, std::min(ael::kBgjMaxB, (ael::kBgjLds / 8) / M));
+    double *ain = h->direct_A, *aout = h->direct_A2, *xin = h->direct_Ainv, *xout = h->direct_X2;
+    for (int c0 = 0; c0 < M; c0 += bb) {
+        const int b = std::min(bb, M - c0);
+        ael::k_bgj_panel<<<1, ael::kGjBlock, 0, s>>>(ain, M, c0, b, h->direct_f, h->direct_piv, h->direct_st);
+        const long nq = (long)(M - c0 - b) + M;
+        ael::k_bgj_update<<<dim3(nblk(M), (unsigned)((nq + ael::kBgjCols - 1) / ael::kBgjCols)), kBlock, 0, s>>>(
+            ain, xin, aout, xout, M, c0, b, h->direct_f, h->direct_piv);
+        std::swap(ain, aout);
+        std::swap(xin, xout);
     }
     HIPCHK(hipGetLastError());
+    // the inverse is in xin, the operator's remains in ain: keep the buffers' roles by their contents
+    if (xin != h->direct_Ainv) std::swap(h->direct_Ainv, h->direct_X2);
+    if (ain != h->direct_A) std::swap(h->direct_A, h->direct_A2);
     ael::GjState st{};
     HIPCHK(hipMemcpyAsync(&st, h->direct_st, sizeof(st), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

@@ -988,66 +988,115 @@ struct GjState {
     double d;     // pivot value
 };
 constexpr int kGjBlock = 1024;
-// Pivot row (largest |a(r, c)|, r >= c, first on ties) and the multipliers
-// f(r) = a(r, c) of the column after the row swap.
-__global__ void __launch_bounds__(kGjBlock) k_gj_pivot(const double* __restrict__ a, int M, int c,
-                                                       double* __restrict__ f, GjState* __restrict__ st) {
-    __shared__ double bv[kGjBlock];
-    __shared__ int bi[kGjBlock];
+// Blocked Gauss-Jordan inversion: panels of bb columns.  k_bgj_panel runs
+// the bb pivoted steps of in-place Gauss-Jordan on the M x bb panel in LDS
+// (column j becomes column j of the panel's transform once it is eliminated),
+// and writes the transform T (M x bb, column-major) and the row map of the
+// panel's swaps (map[r]: the row whose value lands on row r).  Every other
+// column y of [a | inv] then takes, in k_bgj_update, the swaps and
+// y <- T y_P + y with the panel rows zeroed: one pass over the matrix per
+// panel instead of one per pivot.
+constexpr int kBgjMaxB = 16;
+constexpr int kBgjLds = 152 * 1024;  // bytes of panel in LDS: M * bb * 8 at most
+__global__ void __launch_bounds__(kGjBlock) k_bgj_panel(const double* __restrict__ a, int M, int c0, int bb,
+                                                        double* __restrict__ T, int* __restrict__ map,
+                                                        GjState* __restrict__ st) {
+    __shared__ double sw[kBgjLds / 8];  // sw[j * M + r]
+    __shared__ double prow[kBgjMaxB];
+    __shared__ int pvs[kBgjMaxB];
+    __shared__ double wv[kGjBlock / 64];
+    __shared__ int wi[kGjBlock / 64];
     __shared__ int piv_s;
-    const int t = (int)threadIdx.x;
-    double best = -1.0;
-    int idx = c;
-    for (int r = c + t; r < M; r += kGjBlock) {
-        const double v = fabs(a[(long)c * M + r]);
-        if (v > best) { best = v; idx = r; }
-    }
-    bv[t] = best;
-    bi[t] = idx;
+    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int j = 0; j < bb; ++j)
+        for (int r = t; r < M; r += kGjBlock) sw[j * M + r] = a[(long)(c0 + j) * M + r];
     __syncthreads();
-    for (int w = kGjBlock / 2; w > 0; w >>= 1) {
-        if (t < w && (bv[t + w] > bv[t] || (bv[t + w] == bv[t] && bi[t + w] < bi[t]))) {
-            bv[t] = bv[t + w];
-            bi[t] = bi[t + w];
+    for (int j = 0; j < bb; ++j) {
+        const int c = c0 + j;
+        // pivot: largest |value| in column j at rows >= c, first on ties
+        double best = -1.0;
+        int idx = c;
+        for (int r = c + t; r < M; r += kGjBlock) {
+            const double v = fabs(sw[j * M + r]);
+            if (v > best) { best = v; idx = r; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(idx, off);
+            if (ob > best || (ob == best && oi < idx)) { best = ob; idx = oi; }
+        }
+        if (lane == 0) { wv[wave] = best; wi[wave] = idx; }
+        __syncthreads();
+        if (t == 0) {
+            double b0 = wv[0];
+            int i0 = wi[0];
+            for (int w = 1; w < kGjBlock / 64; ++w)
+                if (wv[w] > b0 || (wv[w] == b0 && wi[w] < i0)) { b0 = wv[w]; i0 = wi[w]; }
+            piv_s = i0;
+            pvs[j] = i0;
+            if (!(b0 > 0.0)) st->singular = 1;
+        }
+        __syncthreads();
+        const int p = piv_s;
+        if (t < bb && p != c) {  // rows c and p exchanged in every panel column
+            const double v = sw[t * M + c];
+            sw[t * M + c] = sw[t * M + p];
+            sw[t * M + p] = v;
+        }
+        __syncthreads();
+        const double d = sw[j * M + c];
+        if (t < bb) prow[t] = t == j ? 1.0 / d : sw[t * M + c] / d;
+        __syncthreads();
+        for (int r = t; r < M; r += kGjBlock) {
+            if (r == c) {
+                for (int q = 0; q < bb; ++q) sw[q * M + r] = prow[q];
+            } else {
+                const double f = sw[j * M + r];
+                for (int q = 0; q < bb; ++q)
+                    if (q != j) sw[q * M + r] -= f * prow[q];
+                sw[j * M + r] = -f / d;
+            }
         }
         __syncthreads();
     }
-    if (t == 0) {
-        piv_s = bi[0];
-        st->piv = bi[0];
-        st->d = a[(long)c * M + bi[0]];
-        if (!(bv[0] > 0.0)) st->singular = 1;
-    }
-    __syncthreads();
-    const int piv = piv_s;
-    for (int r = t; r < M; r += kGjBlock) {
-        const int src = r == c ? piv : (r == piv ? c : r);
-        f[r] = a[(long)c * M + src];
+    for (int j = 0; j < bb; ++j)
+        for (int r = t; r < M; r += kGjBlock) T[(long)j * M + r] = sw[j * M + r];
+    for (int r = t; r < M; r += kGjBlock) {  // undo the swaps from the last: the source of row r
+        int src = r;
+        for (int j = bb - 1; j >= 0; --j) {
+            if (src == c0 + j) src = pvs[j];
+            else if (src == pvs[j]) src = c0 + j;
+        }
+        map[r] = src;
     }
 }
-// Rows c and piv exchanged in a and inv, row c divided by the pivot.
-__global__ void __launch_bounds__(kBlock) k_gj_swap_scale(double* __restrict__ a, double* __restrict__ inv, int M,
-                                                          int c, const GjState* __restrict__ st) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2L * M) return;
-    double* m = t < M ? a : inv;
-    const long k = t < M ? t : t - M;
-    const int piv = st->piv;
-    const double d = st->d;
-    double* col = m + k * M;
-    const double vc = col[c], vp = col[piv];
-    col[piv] = vc;
-    col[c] = vp / d;
-}
-// a(r, k) -= f(r) a(c, k) for every row r != c (columns k >= c of a, which
-// left of c are unit columns already, every column of inv).
-__global__ void __launch_bounds__(kBlock) k_gj_eliminate(double* __restrict__ a, double* __restrict__ inv, int M,
-                                                         int c, const double* __restrict__ f) {
+// Columns of [a trailing columns c0 + bb .. M - 1 | inv] from the in to the
+// out buffers, kBgjCols per thread (row r's transform row is loaded once for
+// them): the panel's swaps (map), then T y_P plus the other rows.
+constexpr int kBgjCols = 8;
+__global__ void __launch_bounds__(kBlock) k_bgj_update(const double* __restrict__ ain, const double* __restrict__ xin,
+                                                       double* __restrict__ aout, double* __restrict__ xout, int M,
+                                                       int c0, int bb, const double* __restrict__ T,
+                                                       const int* __restrict__ map) {
     const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const long kk = blockIdx.y;  // [0, M - c): column c + kk of a; [M - c, 2M - c): column of inv
-    if (r >= M || r == c) return;
-    double* col = kk < M - c ? a + (long)(c + kk) * M : inv + (kk - (M - c)) * M;
-    col[r] -= f[r] * col[c];
+    const long na = M - c0 - bb, nq = na + M;
+    if (r >= M) return;
+    double tr[kBgjMaxB];
+#pragma unroll
+    for (int j = 0; j < kBgjMaxB; ++j) tr[j] = j < bb ? T[(long)j * M + r] : 0.0;
+    const int sr = map[r];
+    const bool panel_row = r >= c0 && r < c0 + bb;
+    for (int u = 0; u < kBgjCols; ++u) {
+        const long q = (long)blockIdx.y * kBgjCols + u;
+        if (q >= nq) break;
+        const double* yin = q < na ? ain + (c0 + bb + q) * M : xin + (q - na) * M;
+        double* yout = q < na ? aout + (c0 + bb + q) * M : xout + (q - na) * M;
+        double acc = panel_row ? 0.0 : yin[sr];
+#pragma unroll
+        for (int j = 0; j < kBgjMaxB; ++j)
+            if (j < bb) acc += tr[j] * yin[map[c0 + j]];
+        yout[r] = acc;
+    }
 }
 
 // Block-wide max (op 0) or sum (op 1) for kBlock threads; all threads get it.

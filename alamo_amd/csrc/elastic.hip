@@ -1161,7 +1161,14 @@ int alamo_el_set_bc(alamo_el_t h, const int type_host[78], const double val_host
 
 int alamo_el_set_uniform(alamo_el_t h, int uniform) {
     if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
-    h->uniform = uniform ? 1 : 0;
+    const int u = uniform ? 1 : 0;
+    if (u != h->uniform) {
+        // the diagonals and the dense and direct bottom operators are built
+        // with the flag: the next solve prepares again
+        h->prepared = false;
+        h->prepared32 = false;
+    }
+    h->uniform = u;
     return 0;
 }
 

@@ -392,22 +392,28 @@ class ElasticOperator:
             return None, 0
         return (tuple(clo[:]), tuple(chi[:])), int(n.value)
 
-    def child_box(self, eta, threshold: float = 0.1, buffer: int = 2, min_cells: int = 2):
+    def child_box(self, eta, threshold: float = 0.1, buffer: int = 2, min_cells: int = 2,
+                  blocking_factor: int = 0):
         """Cells [clo, chi] (global, this level) of the next level's patch from
-        this level's tags: tag_box's bounding box clipped so that the patch and
-        its ghost layer lie on this handle's unknowns (the nesting
-        alamo_amr_patch_create requires); None when nothing is tagged or fewer
-        than min_cells cells remain along an axis."""
+        this level's tags: tag_box's bounding box, widened so that the patch's
+        extent on the next level is a multiple of blocking_factor (amr.blocking_factor,
+        when above 1) and clipped so that the patch and its ghost layer lie on
+        this handle's unknowns (the nesting alamo_amr_patch_create requires);
+        None when nothing is tagged or fewer than min_cells cells remain along
+        an axis."""
         box, _ = self.tag_box(eta, threshold, buffer)
         if box is None:
             return None
         nodes = self.level_shape(0)[::-1]
+        q = max(1, int(blocking_factor) // 2)
         clo, chi = [], []
         for a in range(3):
             ulo = self.origin[a] + self.ghost_lo[a]
             uhi = self.origin[a] + nodes[a] - 1 - self.ghost_hi[a]
-            lo = max(box[0][a] + self.origin[a], ulo + (1 if ulo > 0 else 0))
-            hi = min(box[1][a] + self.origin[a], uhi - (1 if uhi < self.dom[a] else 0))
+            blo = (box[0][a] + self.origin[a]) // q * q
+            bhi = -(-(box[1][a] + self.origin[a]) // q) * q
+            lo = max(blo, ulo + (1 if ulo > 0 else 0))
+            hi = min(bhi, uhi - (1 if uhi < self.dom[a] else 0))
             if hi - lo < min_cells:
                 return None
             clo.append(lo)
@@ -532,7 +538,7 @@ class Linear:
         return self.info.res
 
 
-def amr_levels(base, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2):
+def amr_levels(base, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2, blocking_factor: int = 0):
     """AMR levels from the device tags (Integrator::Mechanics::
     TagCellsForRefinement, src/Integrator/Mechanics.H:298-313, one box per
     level): starting from the base operator, each level tags its cells from
@@ -541,7 +547,7 @@ def amr_levels(base, eta_of, levels: int = 3, threshold: float = 0.1, buffer: in
     `levels` levels.  Returns [base, patch 1, ...]; set each patch's model."""
     ops = [base]
     while len(ops) < levels:
-        box = ops[-1].child_box(eta_of(ops[-1]), threshold, buffer)
+        box = ops[-1].child_box(eta_of(ops[-1]), threshold, buffer, blocking_factor=blocking_factor)
         if box is None:
             break
         ops.append(ops[-1].refine(*box))

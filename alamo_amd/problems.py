@@ -111,3 +111,38 @@ def sphere_indicators(origin, shape, dx, spheres, width=0.0):
     z = ((origin[2] + np.arange(nz)) * dx)[:, None, None]
     return np.ascontiguousarray(np.stack([np.broadcast_to(sphere_eta(x, y, z, c, r, width), shape)
                                           for c, r, _ in spheres]))
+
+
+def ellipse_eta(origin, shape, dx, prob_lo=(-8.0, -8.0, -8.0), a=(1.0, 0.75, 0.5), x0=(0.0, 0.0, 0.0), eps=0.1):
+    """The two-component indicator of IC::Ellipse (src/IC/Ellipse.H, single
+    ellipse given by its radii a, diffuse width eps) at the nodes
+    prob_lo + (origin + index) * dx of a box: eta_0 = 1/2 - 1/2 erf((r^T A r - 1)
+    / eps / |A r|) with A = diag(1/a^2) and r = x - x0, eta_1 = 1 - eta_0.  At
+    r = 0 the quotient is -inf and eta_0 = 1, as in the reference."""
+    from scipy.special import erf
+
+    nz, ny, nx = shape
+    r = [prob_lo[0] + (origin[0] + np.arange(nx)) * dx - x0[0],
+         prob_lo[1] + (origin[1] + np.arange(ny)) * dx - x0[1],
+         prob_lo[2] + (origin[2] + np.arange(nz)) * dx - x0[2]]
+    X = r[0][None, None, :]
+    Y = r[1][None, :, None]
+    Z = r[2][:, None, None]
+    A = [1.0 / (a[d] * a[d]) for d in range(3)]
+    q = A[0] * X * X + A[1] * Y * Y + A[2] * Z * Z
+    nrm = np.sqrt((A[0] * X) ** 2 + (A[1] * Y) ** 2 + (A[2] * Z) ** 2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s = (q - 1.0) / eps / nrm
+    e0 = 0.5 - 0.5 * erf(s)
+    e0 = np.broadcast_to(e0, shape)
+    return np.ascontiguousarray(np.stack([e0, 1.0 - e0]))
+
+
+def eshelby_models(E=210.0, nu=0.3, eigenstrain=1e-3):
+    """The two Affine::Isotropic models of tests/Eshelby/input:57-64 as mixing
+    tables: C rows (lambda, mu) and F0 rows (9 values, 3a+b); model 1 (eta_0,
+    inside the ellipse) carries the dilatational eigenstrain, model 2 none."""
+    lam, mu = _model.lame_from_E_nu(E, nu)
+    Ct = np.array([[lam, mu], [lam, mu]])
+    Ft = np.array([(eigenstrain * np.eye(3)).ravel(), np.zeros(9)])
+    return Ct, Ft

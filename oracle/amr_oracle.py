@@ -133,6 +133,31 @@ def inject(f, lev, c, parent):
 
 
 def reflux(rf, lev, rc, parent):
+    """Operator::reflux (src/Operator/Operator.cpp:526-633): the parent
+    residual at the covered and coarse-fine nodes [clo, chi] becomes the nodal
+    full weighting of the patch residual (ghost rows zeroed by the caller),
+    with the domain-face rules of the MG restriction on domain faces.  Computed
+    as the MG restriction of the patch residual extended by zero over a box
+    one parent node wider than [clo, chi] on every side inside the domain;
+    reflux_loops states the same per node."""
+    lo = [2 * lev.clo[a] - 2 if lev.clo[a] > 0 else 0 for a in range(3)]
+    hi = [2 * lev.chi[a] + 2 if lev.chi[a] < parent.dom[a] else 2 * parent.dom[a] for a in range(3)]
+    buf = np.zeros((rf.shape[0], hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1))
+    o = [lev.origin[a] - lo[a] for a in range(3)]
+    nz, ny, nx = rf.shape[1:]
+    buf[:, o[2]:o[2] + nz, o[1]:o[1] + ny, o[0]:o[0] + nx] = rf
+    c = O.restrict(buf)
+    cl = [lev.clo[a] - lo[a] // 2 for a in range(3)]
+    n = [lev.chi[a] - lev.clo[a] + 1 for a in range(3)]
+    p = [lev.clo[a] - parent.origin[a] for a in range(3)]
+    rc[:, p[2]:p[2] + n[2], p[1]:p[1] + n[1], p[0]:p[0] + n[0]] = \
+        c[:, cl[2]:cl[2] + n[2], cl[1]:cl[1] + n[1], cl[0]:cl[0] + n[0]]
+
+
+def reflux_loops(rf, lev, rc, parent):
+    """reflux, node by node: the 27-point weights inside the domain, the
+    1D weights (1/2, 1, 1/2) normalised over the directions a domain face does
+    not pin."""
     ox, oy, oz = lev.origin
     px, py, pz = parent.origin
     dnx, dny, dnz = parent.dom

@@ -9,7 +9,11 @@ Outputs (data, not source):
                        extracted from the accessor's uid comparisons;
   solid_linear_cubic.dat, solid_affine_cubic.dat
                        the reference data of tests/Solid (thermo.dat time
-                       series of disp_xhi_x and trac_xhi_x), copied verbatim.
+                       series of disp_xhi_x and trac_xhi_x), copied verbatim;
+  eshelby_displacement.csv, eshelby_sigma_normal.csv
+                       the line-out data of tests/Eshelby/reference (z,
+                       displacement x 1e4 and normal stress x 1000/E along
+                       x = y = 0.25), copied verbatim.
 """
 import json
 import os
@@ -44,6 +48,9 @@ def main():
                     os.path.join(HERE, "solid_linear_cubic.dat"))
     shutil.copyfile(os.path.join(REF, "tests/Solid/reference/affine-cubic.dat"),
                     os.path.join(HERE, "solid_affine_cubic.dat"))
+    for name in ("displacement", "sigma_normal"):
+        shutil.copyfile(os.path.join(REF, f"tests/Eshelby/reference/Eshelby_{name}.csv"),
+                        os.path.join(HERE, f"eshelby_{name}.csv"))
 
 
 if __name__ == "__main__":

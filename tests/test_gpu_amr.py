@@ -334,3 +334,44 @@ def test_plotfile_of_a_composite_solution(tmp_path):
         nz, ny, nx = v.shape[1:]
         assert lo == tuple(op.origin[a] + gl[a] for a in range(3))
         assert np.array_equal(data[1:4], v[:, gl[2]:nz - gh[2], gl[1]:ny - gh[1], gl[0]:nx - gh[0]])
+
+
+def test_uniaxial_tension_deck_three_levels_closed_form():
+    """tests/UniaxialTension in 3D (input:37-99: base 32^3 cells over
+    [-8, 8]^3, explicit mesh of the middle half on each of two levels, E = 210,
+    nu = 0.3): the device's composite FP64 solve reproduces the checker's
+    closed form u_x = (x + 8) 0.1/16, sigma_xx = E 0.1/16 (test:27-28,
+    tolerance 1e-8 on the squared relative error) on every level, and so do
+    the FP32 cycles."""
+    from test_oracle_closed_forms import uniaxial_tension_deck_bc, uniaxial_tension_levels
+    bc = uniaxial_tension_deck_bc()
+    lam, mu = M.lame_from_E_nu(210.0, 0.3)
+    eps = 0.1 / 16
+    for precision in ("fp64", "fp32"):
+        base = el.ElasticOperator(32, 0.5, M.ISOTROPIC)
+        base.SetBC(bc)
+        ops = [base]
+        for clo, chi in uniaxial_tension_levels(32):
+            ops.append(ops[-1].refine(clo, chi))
+        bs, Cs = [], []
+        for op in ops:
+            shape = op.level_shape(0)
+            C = torch.empty((2,) + shape, dtype=torch.float64, device="cuda")
+            C[0], C[1] = lam, mu
+            op.SetModel(C)
+            Cs.append(C)
+            bs.append(op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda")))
+        xs = [torch.zeros_like(b) for b in bs]
+        info = el.amr_solve(ops, xs, bs, 1e-10, 0.0, max_iter=200, precision=precision)
+        assert info.status == 0
+        for l, op in enumerate(ops):
+            nz, ny, nx = op.level_shape(0)
+            h = op.dx[0]
+            x = -8.0 + (op.origin[0] + torch.arange(nx, device="cuda", dtype=torch.float64)) * h
+            ux = ((x + 8.0) * eps).expand(nz, ny, nx)
+            err = float(((xs[l][0] - ux) ** 2).sum() / (ux ** 2).sum())
+            sig = op.stress(Cs[l], xs[l])
+            serr = float(((sig[0] - 210.0 * eps) ** 2).sum() / ((210.0 * eps) ** 2 * sig[0].numel()))
+            print(f"{precision} level {l}: {info.iters} iterations, u_x squared rel error {err:.2e}, "
+                  f"sigma_xx {serr:.2e}")
+            assert err < 1e-8 and serr < 1e-8

@@ -288,6 +288,23 @@ def test_bc_init_and_newton_rhs_match_oracle():
     assert relerr(r, O.newton_rhs(u, b, p["C"], p["sym"], bc, p["dx"], F0=p["F0"])) < REL
 
 
+@pytest.mark.parametrize("with_u", [False, True])
+def test_newton_rhs_closed_form_on_device(with_u):
+    """k_newton_rhs against the closed form of the oracle's pin: C and F0
+    linear in space, u quadratic; the interior rows are b - div(C:(grad u - F0))
+    exactly (Newton.H:216-231, central divergence)."""
+    from test_oracle_closed_forms import newton_closed_form_case
+    el = _el()
+    u, b, C, F0, bc, dx, want, div, inner = newton_closed_form_case(with_u)
+    nz, ny, nx = u.shape[1:]
+    op = el.ElasticOperator((nx - 1, ny - 1, nz - 1), dx, M.MAJORMINOR)
+    op.SetBC(bc)
+    r = host(op.newton_rhs(dev(C), dev(u), dev(b), F0=dev(F0)))
+    err = np.abs(r[inner] - want[inner]).max() / np.abs(div[inner]).max()
+    assert err < 1e-11, err
+    assert relerr(r, O.newton_rhs(u, b, C, M.MAJORMINOR, bc, dx, F0=F0)) < REL
+
+
 def test_shape_mismatch_is_rejected_before_launch():
     el = _el()
     op = el.ElasticOperator(8, 0.125, M.ISOTROPIC)

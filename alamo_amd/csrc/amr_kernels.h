@@ -34,14 +34,10 @@ __device__ __forceinline__ bool ghost_node(const Lvl& L, const Patch& P, int i, 
 __global__ void __launch_bounds__(kBlock) k_tag_init(int* __restrict__ bb) {
     if (threadIdx.x < 7) bb[threadIdx.x] = threadIdx.x < 3 ? 0x7fffffff : (threadIdx.x < 6 ? -1 : 0);
 }
-__global__ void __launch_bounds__(kBlock) k_tag_eta(const double* __restrict__ eta, int ncomp, Lvl L, double thr,
-                                                    int* __restrict__ bb) {
+// Whether node (i, j, k) tags cell (i, j, k).
+__device__ __forceinline__ bool eta_tagged(const double* __restrict__ eta, int ncomp, const Lvl& L, double thr,
+                                           long id, int i, int j, int k) {
 #pragma clang fp contract(off)
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= L.N) return;
-    int i, j, k;
-    node_ijk(id, L, i, j, k);
-    if (i >= L.nx - 1 || j >= L.ny - 1 || k >= L.nz - 1) return;
     const double dxn = sqrt(L.dx * L.dx + L.dy * L.dy + L.dz * L.dz);
     const long sy = L.nx, sz = (long)L.nx * L.ny;
     bool tag = false;
@@ -51,7 +47,16 @@ __global__ void __launch_bounds__(kBlock) k_tag_eta(const double* __restrict__ e
                      g2 = d1(f, id, sz, k, L.nz, L.dz);
         tag |= sqrt(g0 * g0 + g1 * g1 + g2 * g2) * dxn > thr;
     }
-    if (!tag) return;
+    return tag;
+}
+__global__ void __launch_bounds__(kBlock) k_tag_eta(const double* __restrict__ eta, int ncomp, Lvl L, double thr,
+                                                    int* __restrict__ bb) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    if (i >= L.nx - 1 || j >= L.ny - 1 || k >= L.nz - 1) return;
+    if (!eta_tagged(eta, ncomp, L, thr, id, i, j, k)) return;
     atomicMin(bb + 0, i);
     atomicMin(bb + 1, j);
     atomicMin(bb + 2, k);
@@ -59,6 +64,17 @@ __global__ void __launch_bounds__(kBlock) k_tag_eta(const double* __restrict__ e
     atomicMax(bb + 4, j);
     atomicMax(bb + 5, k);
     atomicAdd(bb + 6, 1);
+}
+
+// The same tags, one byte per cell of the box ((nx-1)(ny-1)(nz-1), x fastest).
+__global__ void __launch_bounds__(kBlock) k_tag_cells(const double* __restrict__ eta, int ncomp, Lvl L, double thr,
+                                                      unsigned char* __restrict__ tags) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    if (i >= L.nx - 1 || j >= L.ny - 1 || k >= L.nz - 1) return;
+    tags[((long)k * (L.ny - 1) + j) * (L.nx - 1) + i] = eta_tagged(eta, ncomp, L, thr, id, i, j, k) ? 1 : 0;
 }
 
 // v = 0 on ghost nodes (mode 0) or v = src on ghost nodes (mode 1).
@@ -193,8 +209,8 @@ __global__ void __launch_bounds__(kBlock) k_patch_reflux(const double* __restric
 
 // out = v with the parent nodes strictly inside the patch (covered by the
 // finer level, excluded from the composite norm) set to 0.
-__global__ void __launch_bounds__(kBlock) k_mask_covered(const double* __restrict__ v, Lvl L, int pox, int poy,
-                                                         int poz, Patch P, double* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock) k_mask_covered(const double* v, Lvl L, int pox, int poy, int poz, Patch P,
+                                                         double* out) {  // out may be v
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= L.N) return;
     int i, j, k;

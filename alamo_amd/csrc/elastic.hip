@@ -23,6 +23,7 @@
 #include "../../include/alamo_elastic.h"
 #include "../../include/alamo_unet.h"
 #include "bottom_kernels.h"
+#include "cluster.h"
 #include "common.h"
 #include "elastic_kernels.h"
 #include "amr_kernels.h"
@@ -122,7 +123,6 @@ struct alamo_el_s {
     double* direct_I = nullptr;     // unit vectors of the assembly
     double* direct_A = nullptr;     // operator, destroyed by the inversion
     double* direct_Ainv = nullptr;  // inverse, column-major (null when singular)
-    float* direct_Ainv32 = nullptr; // the inverse rounded to FP32, for the FP32 cycles
     double* direct_f = nullptr;     // blocked Gauss-Jordan: the panel's transform (M x ael::kBgjMaxB)
     double* direct_A2 = nullptr;    // second copies of the operator and the inverse (the update
     double* direct_X2 = nullptr;    //   of each panel reads one and writes the other)
@@ -157,6 +157,13 @@ constexpr int kRedBlocks = 1024;
 constexpr int kDotBlocks = 2048;  // multi-dot partials: 8 waves per SIMD over the 256 CUs
 constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M doubles
 constexpr int kDirectMax = 4096; // unknowns of the direct bottom level (an M x M FP64 inverse)
+// Weights of the red-black Gauss-Seidel smoother.  Interior rows: 0.9; the
+// undamped 1.0 converges fastest on a polycrystal (V-cycle factor 0.775 vs
+// 0.802 at 33^3) but diverges below 1e-7 at the sharp E = 10 interface of the
+// spherical inclusion, where 0.9 takes 25 V-cycles to 1e-10 (damped Jacobi
+// 28).  Boundary rows: 0.5; traction rows are one-sided first-order rows, and
+// at 1.0 the V-cycle on a traction-loaded polycrystal diverges.
+constexpr double kRbInterior = 0.9, kRbBoundary = 0.5;
 
 // ---------------------------------------------------------------------------
 // Kernel dispatch
@@ -353,26 +360,38 @@ struct Cycle {
     int nu1, nu2, nuf, bottom_max_iter;
     double bottom_tol;
     int bottom_solver = ALAMO_EL_BOTTOM_BICGSTAB;
+    int smoother = ALAMO_EL_SMOOTH_JACOBI;
 
     int nlev() const { return (int)h->lv.size(); }
     Level& L(int l) { return h->lv[l]; }
     Vecs<T>& V(int l) { return vecs<T>(h->lv[l]); }
     long M(int l) { return 3 * h->lv[l].L.N; }
 
-    // Operator<Grid::Node>::Fsmooth x n: 2n Jacobi sweeps (Operator.cpp:107).
+    // Operator<Grid::Node>::Fsmooth x n: 2n Jacobi sweeps (Operator.cpp:107);
+    // with the red-black smoother, n red and black half sweeps.
     void smooth(int l, T* x, const T* b, int n) {
         Vecs<T>& v = V(l);
+        if (smoother == ALAMO_EL_SMOOTH_RBGS) {
+            for (int q = 0; q < n; ++q)
+                for (int color = 0; color < 2; ++color) {
+                    launch_residual<T>(h, L(l), x, b, v.C, v.tmp, s);
+                    ael::k_rb_update<T><<<nblk(L(l).L.N), kBlock, 0, s>>>(x, v.tmp, v.diag, L(l).L, color,
+                                                                          T(kRbInterior), T(kRbBoundary));
+                }
+            return;
+        }
         for (int q = 0; q < n; ++q) {
             launch_jacobi<T>(h, L(l), x, b, v.diag, v.C, v.tmp, s);
             launch_jacobi<T>(h, L(l), v.tmp, b, v.diag, v.C, x, s);
         }
     }
-    // x = 0, then Fsmooth x n; the first sweep from zero is w b / D, one
-    // elementwise kernel instead of the fill and a stencil sweep.
+    // x = 0, then Fsmooth x n; the first Jacobi sweep from zero is w b / D,
+    // one elementwise kernel instead of the fill and a stencil sweep.
     void smooth_from_zero(int l, T* x, const T* b, int n) {
         Vecs<T>& v = V(l);
-        if (n <= 0) {
+        if (n <= 0 || smoother == ALAMO_EL_SMOOTH_RBGS) {
             launch_fill<T>(x, T(0), M(l), s);
+            smooth(l, x, b, n);
             return;
         }
         ael::k_jacobi_from_zero<T><<<nblk(M(l)), kBlock, 0, s>>>(b, v.diag, (T)h->omega, v.tmp, M(l));
@@ -396,12 +415,11 @@ struct Cycle {
             const int l = bot();
             const int Mb = (int)M(l);
             const dim3 grid(nblk(Mb), (Mb + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk);
-            if (std::is_same<T, float>::value && h->direct_Ainv32)
-                ael::k_dense_mv_partial<T, float><<<grid, kBlock, 0, s>>>(h->direct_Ainv32, V(l).res, Mb,
-                                                                          h->direct_part);
-            else
-                ael::k_dense_mv_partial<T, double><<<grid, kBlock, 0, s>>>(h->direct_Ainv, V(l).res, Mb,
-                                                                           h->direct_part);
+            // the FP32 cycles keep this solve in FP64 too: the FP64 inverse times
+            // the residual widened to FP64, the sums in FP64, the result rounded
+            // to the cycle's precision
+            ael::k_dense_mv_partial<T, double><<<grid, kBlock, 0, s>>>(h->direct_Ainv, V(l).res, Mb,
+                                                                       h->direct_part);
             ael::k_dense_mv_final<T><<<nblk(Mb), kBlock, 0, s>>>(h->direct_part, (int)grid.y, Mb, V(l).cor);
             return;
         }
@@ -561,13 +579,12 @@ int direct_level_of(alamo_el_t h) {
 
 void free_direct(alamo_el_t h) {
     for (void* p : {(void*)h->direct_I, (void*)h->direct_A, (void*)h->direct_Ainv, (void*)h->direct_f,
-                    (void*)h->direct_part, (void*)h->direct_st, (void*)h->direct_Ainv32, (void*)h->direct_A2,
+                    (void*)h->direct_part, (void*)h->direct_st, (void*)h->direct_A2,
                     (void*)h->direct_X2, (void*)h->direct_piv})
         if (p) (void)hipFree(p);
     h->direct_I = h->direct_A = h->direct_Ainv = h->direct_f = h->direct_part = nullptr;
     h->direct_A2 = h->direct_X2 = nullptr;
     h->direct_piv = nullptr;
-    h->direct_Ainv32 = nullptr;
     h->direct_st = nullptr;
     h->direct_M = 0;
 }
@@ -635,8 +652,6 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
         (void)hipFree(h->direct_Ainv);
         h->direct_Ainv = nullptr;
     }
-    if (h->direct_Ainv32) (void)hipFree(h->direct_Ainv32);
-    h->direct_Ainv32 = nullptr;  // rounded again by prepare32
     return 0;
 }
 
@@ -871,12 +886,6 @@ int prepare32(alamo_el_t h, hipStream_t s) {
             ael::k_cast<double, float><<<nblk(18L * l.nflag), kBlock, 0, s>>>(l.d.D, l.f.D, 18L * l.nflag);
         }
     }
-    if (h->direct_Ainv) {
-        const long MM = (long)h->direct_M * h->direct_M;
-        if (h->direct_Ainv32) (void)hipFree(h->direct_Ainv32);
-        HIPCHK(hipMalloc((void**)&h->direct_Ainv32, sizeof(float) * MM));
-        ael::k_cast<double, float><<<nblk(MM), kBlock, 0, s>>>(h->direct_Ainv, h->direct_Ainv32, MM);
-    }
     HIPCHK(hipGetLastError());
     h->prepared32 = true;
     return 0;
@@ -1042,6 +1051,7 @@ void alamo_el_default_opts(alamo_el_opts* o) {
     o->restart = 30;
     o->verbose = 0;
     o->bottom_solver = ALAMO_EL_BOTTOM_BICGSTAB;
+    o->smoother = ALAMO_EL_SMOOTH_JACOBI;
 }
 
 int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_coarsening_level, alamo_el_t* out) {
@@ -1274,6 +1284,18 @@ int alamo_el_fsmooth(alamo_el_t h, int mglev, double* x, const double* b, int ns
     return 0;
 }
 
+int alamo_el_gs_smooth(alamo_el_t h, int mglev, double* x, const double* b, int nsmooth, hipStream_t s) {
+    if (bad_level(h, mglev) || !x || !b || nsmooth < 0) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (!h->prepared) {
+        int rc = prepare(h, s);
+        if (rc) return rc;
+    }
+    Cycle<double> c{h, s, 2, 2, 8, 200, 1e-4, ALAMO_EL_BOTTOM_BICGSTAB, ALAMO_EL_SMOOTH_RBGS};
+    c.smooth(mglev, x, b, nsmooth);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int alamo_el_residual(alamo_el_t h, int mglev, const double* x, const double* b, double* r, hipStream_t s) {
     if (bad_level(h, mglev) || !x || !b || !r) return fail(ALAMO_EL_EINVAL, "bad argument");
     if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
@@ -1325,6 +1347,19 @@ int alamo_el_mix_models(alamo_el_t h, const double* eta, int nmodels, const doub
     return 0;
 }
 
+namespace {
+void launch_newton_dw(alamo_el_t h, const double* u, const double* C, long cs, long cn, const double* F0, int kb,
+                      int np, double* gu9, double* dw9, hipStream_t s) {
+    const Lvl L = h->lv[0].L;
+    const unsigned g = nblk((long)np * L.nx * L.ny);
+    switch (h->sym) {
+        case 0: ael::k_newton_dw<0><<<g, kBlock, 0, s>>>(u, C, cs, cn, F0, L, kb, np, gu9, dw9); break;
+        case 1: ael::k_newton_dw<1><<<g, kBlock, 0, s>>>(u, C, cs, cn, F0, L, kb, np, gu9, dw9); break;
+        default: ael::k_newton_dw<2><<<g, kBlock, 0, s>>>(u, C, cs, cn, F0, L, kb, np, gu9, dw9); break;
+    }
+}
+}  // namespace
+
 int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double* F0, const double* u,
                         const double* b, double* rhs, hipStream_t s) {
     if (!h || !C || !u || !b || !rhs) return fail(ALAMO_EL_EINVAL, "bad argument");
@@ -1339,12 +1374,8 @@ int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double*
     } else if (layout != ALAMO_EL_SOA) {
         return fail(ALAMO_EL_EINVAL, "bad layout");
     }
-    switch (h->sym) {
-        case 0: ael::k_newton_dw<0><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, h->gu9, h->dw9); break;
-        case 1: ael::k_newton_dw<1><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, h->gu9, h->dw9); break;
-        default: ael::k_newton_dw<2><<<nblk(L.N), kBlock, 0, s>>>(u, Cs, F0, L, h->gu9, h->dw9); break;
-    }
-    ael::k_newton_rhs<<<nblk(L.N), kBlock, 0, s>>>(u, b, h->gu9, h->dw9, h->bct_dev, L, rhs);
+    launch_newton_dw(h, u, Cs, L.N, 1, F0, 0, L.nz, h->gu9, h->dw9, s);
+    ael::k_newton_rhs<<<nblk(L.N), kBlock, 0, s>>>(u, b, h->gu9, h->dw9, 0, L.nz, h->bct_dev, L, 0, L.nz, rhs);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1457,7 +1488,8 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
     alamo_el_opts o;
     alamo_el_default_opts(&o);
     if (opts_in) o = *opts_in;
-    if (o.max_iter < 0 || o.nu1 < 0 || o.nu2 < 0 || o.bottom_max_iter < 0 || !(o.omega > 0))
+    if (o.max_iter < 0 || o.nu1 < 0 || o.nu2 < 0 || o.bottom_max_iter < 0 || !(o.omega > 0) ||
+        (o.smoother != ALAMO_EL_SMOOTH_JACOBI && o.smoother != ALAMO_EL_SMOOTH_RBGS))
         return fail(ALAMO_EL_EINVAL, "bad solver options");
     const auto t0 = std::chrono::steady_clock::now();
     h->omega = o.omega;
@@ -1492,8 +1524,10 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
         status = fgmres_solve(h, x, b, target, o, iters, nrm, learned_taken, s);
         if (status < 0) return status;
     } else {
-        Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
-        Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+        Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
+                          o.smoother};
+        Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
+                         o.smoother};
         for (int it = 0; it < o.max_iter; ++it) {
             if (learned_pre && it > 0) {
                 // learned coarse correction between cycles, judged like the guess
@@ -1563,8 +1597,10 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         h->krylov_m = m;
     }
     const bool fp32 = o.precision == ALAMO_EL_PREC_FP32_IR;
-    Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
-    Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+    Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
+                      o.smoother};
+    Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
+                     o.smoother};
     std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
     int rc;
     iters = 0;
@@ -1705,70 +1741,101 @@ int amr_reduce_max(alamo_el_t h, const double* v, long n, double* out, hipStream
     return reduce_host<double>(h, v, v, n, 0, out, s);
 }
 
+// The handles of a composite solve: h[0] is the base, every other handle a
+// patch whose parent h[par[i]] comes before it; a level may hold several
+// patches (they share no nodes).  order lists the handles deepest first.
 struct AmrCtx {
-    int nlev;
+    int n;
     const alamo_el_t* h;
     double* const* x;
     const double* const* b;
     hipStream_t s;
-    Level& L(int l) { return h[l]->lv[0]; }
-    const ael::Patch& P(int l) { return h[l]->amr; }
-    int pox(int l) { return h[l - 1]->amr.ox; }
-    int poy(int l) { return h[l - 1]->amr.oy; }
-    int poz(int l) { return h[l - 1]->amr.oz; }
+    std::vector<int> par, depth, order;
+    std::vector<std::vector<int>> kids;
 
-    void reflux(int l, const double* rf, double* rc) {  // fine level l into parent l-1
-        const ael::Patch& Pl = P(l);
-        const long n = (long)(Pl.chi[0] - Pl.clo[0] + 1) * (Pl.chi[1] - Pl.clo[1] + 1) * (Pl.chi[2] - Pl.clo[2] + 1);
-        const int* dom = h[l - 1]->amr_dom;
-        ael::k_patch_reflux<<<nblk(n), kBlock, 0, s>>>(rf, L(l).L, Pl, rc, L(l - 1).L, pox(l), poy(l), poz(l), dom[0],
+    int init() {
+        par.assign(n, -1);
+        depth.assign(n, 0);
+        kids.assign(n, {});
+        for (int i = 1; i < n; ++i) {
+            for (int j = 0; j < i; ++j)
+                if (h[i]->amr_parent == h[j]) par[i] = j;
+            if (par[i] < 0) return fail(ALAMO_EL_EINVAL, "every patch's parent must come before it in the list");
+            depth[i] = depth[par[i]] + 1;
+            kids[par[i]].push_back(i);
+        }
+        // patches of one level share no nodes: on some axis one ends before
+        // the other begins, so that neither reaches the other's ghost layer
+        for (int i = 1; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                if (depth[i] != depth[j]) continue;
+                const ael::Patch &A = P(i), &B = P(j);
+                bool apart = false;
+                for (int a = 0; a < 3; ++a) apart |= A.chi[a] < B.clo[a] || B.chi[a] < A.clo[a];
+                if (!apart) return fail(ALAMO_EL_EINVAL, "patches of one level must not share nodes");
+            }
+        order.resize(n);
+        for (int i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return depth[u] > depth[v]; });
+        return 0;
+    }
+    Level& L(int i) { return h[i]->lv[0]; }
+    const ael::Patch& P(int i) { return h[i]->amr; }
+
+    void reflux(int i, const double* rf, double* rc) {  // patch i into its parent
+        const ael::Patch& Pi = P(i);
+        const int q = par[i];
+        const long m = (long)(Pi.chi[0] - Pi.clo[0] + 1) * (Pi.chi[1] - Pi.clo[1] + 1) * (Pi.chi[2] - Pi.clo[2] + 1);
+        const int* dom = h[q]->amr_dom;
+        ael::k_patch_reflux<<<nblk(m), kBlock, 0, s>>>(rf, L(i).L, Pi, rc, L(q).L, P(q).ox, P(q).oy, P(q).oz, dom[0],
                                                         dom[1], dom[2]);
     }
-    void interp(int l, const double* c, double* f, int mode) {
-        ael::k_patch_interp<<<nblk(L(l).L.N), kBlock, 0, s>>>(c, L(l - 1).L, pox(l), poy(l), poz(l), f, L(l).L, P(l),
+    void interp(int i, const double* c, double* f, int mode) {
+        const int q = par[i];
+        ael::k_patch_interp<<<nblk(L(i).L.N), kBlock, 0, s>>>(c, L(q).L, P(q).ox, P(q).oy, P(q).oz, f, L(i).L, P(i),
                                                                mode);
     }
-    void ghost(int l, double* v, const double* src, int mode) {
-        ael::k_patch_ghost<<<nblk(L(l).L.N), kBlock, 0, s>>>(v, src, L(l).L, P(l), mode);
+    void ghost(int i, double* v, const double* src, int mode) {
+        ael::k_patch_ghost<<<nblk(L(i).L.N), kBlock, 0, s>>>(v, src, L(i).L, P(i), mode);
     }
-    void sync_solution() {  // average down, then refresh the ghost layers
-        for (int l = nlev - 1; l >= 1; --l) {
-            const ael::Patch& Pl = P(l);
-            const long n = (long)(Pl.chi[0] - Pl.clo[0] + 1) * (Pl.chi[1] - Pl.clo[1] + 1) * (Pl.chi[2] - Pl.clo[2] + 1);
-            ael::k_patch_inject<<<nblk(n), kBlock, 0, s>>>(x[l], L(l).L, Pl, x[l - 1], L(l - 1).L, pox(l), poy(l), poz(l));
+    void sync_solution() {  // average down (deepest first), then refresh the ghost layers
+        for (int i : order) {
+            if (i == 0) continue;
+            const ael::Patch& Pi = P(i);
+            const int q = par[i];
+            const long m = (long)(Pi.chi[0] - Pi.clo[0] + 1) * (Pi.chi[1] - Pi.clo[1] + 1) * (Pi.chi[2] - Pi.clo[2] + 1);
+            ael::k_patch_inject<<<nblk(m), kBlock, 0, s>>>(x[i], L(i).L, Pi, x[q], L(q).L, P(q).ox, P(q).oy, P(q).oz);
         }
-        for (int l = 1; l < nlev; ++l) interp(l, x[l - 1], x[l], 2);
+        for (int u = n - 1; u >= 0; --u)
+            if (order[u] != 0) interp(order[u], x[par[order[u]]], x[order[u]], 2);
     }
-    // Composite residual on every level (finest first, reflux into each
-    // parent) and its masked inf-norm.
+    // Composite residual on every handle (deepest first, each patch refluxed
+    // into its parent) and its masked inf-norm.
     int residuals(double* norm) {
-        for (int l = nlev - 1; l >= 0; --l) {
-            Level& lv = L(l);
-            launch_residual<double>(h[l], lv, x[l], b[l], lv.d.C, lv.d.res, s);
-            if (l > 0) ghost(l, lv.d.res, nullptr, 0);
-            if (l + 1 < nlev) reflux(l + 1, L(l + 1).d.res, lv.d.res);
+        for (int i : order) {
+            Level& lv = L(i);
+            launch_residual<double>(h[i], lv, x[i], b[i], lv.d.C, lv.d.res, s);
+            if (i > 0) ghost(i, lv.d.res, nullptr, 0);
+            for (int c : kids[i]) reflux(c, L(c).d.res, lv.d.res);
         }
         return masked_norm(norm, true);
     }
-    // max |v| over the composite grid: each level's res (use_res) or b, ghost
-    // nodes excluded, parent nodes strictly covered by the child excluded.
+    // max |v| over the composite grid: each handle's res (use_res) or b, ghost
+    // nodes excluded, nodes strictly covered by a child excluded.
     int masked_norm(double* norm, bool use_res) {
         *norm = 0.0;
         int rc;
-        for (int l = 0; l < nlev; ++l) {
-            Level& lv = L(l);
+        for (int i = 0; i < n; ++i) {
+            Level& lv = L(i);
             const long M = 3 * lv.L.N;
-            const double* v = use_res ? lv.d.res : b[l];
+            const double* v = use_res ? lv.d.res : b[i];
             double* t = lv.d.rescor;
-            if (l + 1 < nlev) {
-                ael::k_mask_covered<<<nblk(lv.L.N), kBlock, 0, s>>>(v, lv.L, h[l]->amr.ox, h[l]->amr.oy, h[l]->amr.oz,
-                                                                     P(l + 1), t);
-            } else {
-                launch_copy<double>(t, v, M, s);
-            }
-            if (l > 0) ghost(l, t, nullptr, 0);
+            launch_copy<double>(t, v, M, s);
+            for (int c : kids[i])
+                ael::k_mask_covered<<<nblk(lv.L.N), kBlock, 0, s>>>(t, lv.L, P(i).ox, P(i).oy, P(i).oz, P(c), t);
+            if (i > 0) ghost(i, t, nullptr, 0);
             double m;
-            if ((rc = amr_reduce_max(h[l], t, M, &m, s))) return rc;
+            if ((rc = amr_reduce_max(h[i], t, M, &m, s))) return rc;
             if (std::isnan(m)) return fail(ALAMO_EL_ENAN, "NaN in the composite residual");
             *norm = std::max(*norm, m);
         }
@@ -1857,6 +1924,35 @@ int alamo_el_tag_bbox(alamo_el_t h, const double* eta, int ncomp, double thresho
     return 0;
 }
 
+int alamo_el_tag_cells(alamo_el_t h, const double* eta, int ncomp, double threshold, unsigned char* tags,
+                       hipStream_t s) {
+    if (!h || !eta || ncomp < 1 || !tags) return fail(ALAMO_EL_EINVAL, "bad argument");
+    const Lvl L = h->lv[0].L;
+    ael::k_tag_cells<<<nblk(L.N), kBlock, 0, s>>>(eta, ncomp, L, threshold, tags);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_amr_cluster(const unsigned char* tags_host, const int ncell[3], const int origin[3], int buffer,
+                      int blocking_factor, double grid_eff, int max_boxes, int* boxes, int* nboxes) {
+    if (!tags_host || !ncell || !origin || !boxes || !nboxes || buffer < 0 || blocking_factor < 0 ||
+        !(grid_eff > 0.0 && grid_eff <= 1.0) || max_boxes < 1)
+        return fail(ALAMO_EL_EINVAL, "bad argument");
+    for (int a = 0; a < 3; ++a)
+        if (ncell[a] < 1) return fail(ALAMO_EL_EINVAL, "empty tag array");
+    *nboxes = 0;
+    const std::vector<ael::cluster::Box> bs =
+        ael::cluster::make_boxes(tags_host, ncell, origin, buffer, blocking_factor, grid_eff);
+    if ((int)bs.size() > max_boxes) return fail(ALAMO_EL_EINVAL, "more boxes than max_boxes");
+    for (size_t q = 0; q < bs.size(); ++q)
+        for (int a = 0; a < 3; ++a) {
+            boxes[6 * q + a] = bs[q].lo[a];
+            boxes[6 * q + 3 + a] = bs[q].hi[a] + 1;
+        }
+    *nboxes = (int)bs.size();
+    return 0;
+}
+
 int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost_hi[3]) {
     if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
     const int o[3] = {h->amr.ox, h->amr.oy, h->amr.oz};
@@ -1894,17 +1990,21 @@ int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const doub
     if (nlev < 1 || !hs || !x || !b) return fail(ALAMO_EL_EINVAL, "bad argument");
     for (int l = 0; l < nlev; ++l) {
         if (!hs[l] || !x[l] || !b[l]) return fail(ALAMO_EL_EINVAL, "null level");
-        if ((l == 0) != (hs[l]->amr_parent == nullptr) || (l > 0 && hs[l]->amr_parent != hs[l - 1]))
-            return fail(ALAMO_EL_EINVAL, "levels must be a base handle followed by its nested patches");
+        if ((l == 0) != (hs[l]->amr_parent == nullptr))
+            return fail(ALAMO_EL_EINVAL, "the list must hold a base handle followed by its nested patches");
         if (!hs[l]->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
     }
     if (nlev == 1) return alamo_el_solve(hs[0], x[0], b[0], tol_rel, tol_abs, opts_in, info, s);
     alamo_el_opts o;
     alamo_el_default_opts(&o);
     if (opts_in) o = *opts_in;
-    if (o.max_iter < 0 || o.nu1 < 0 || o.nu2 < 0 || !(o.omega > 0)) return fail(ALAMO_EL_EINVAL, "bad solver options");
+    if (o.max_iter < 0 || o.nu1 < 0 || o.nu2 < 0 || !(o.omega > 0) ||
+        (o.smoother != ALAMO_EL_SMOOTH_JACOBI && o.smoother != ALAMO_EL_SMOOTH_RBGS))
+        return fail(ALAMO_EL_EINVAL, "bad solver options");
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
+    AmrCtx A{nlev, hs, x, b, s};
+    if ((rc = A.init())) return rc;
     for (int l = 0; l < nlev; ++l) {
         hs[l]->omega = o.omega;
         if (!hs[l]->prepared && (rc = prepare(hs[l], s))) return rc;
@@ -1912,33 +2012,46 @@ int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const doub
     }
     const bool fp32 = o.precision == ALAMO_EL_PREC_FP32_IR;
     if (fp32 && (rc = prepare32(hs[0], s))) return rc;
-    AmrCtx A{nlev, hs, x, b, s};
     A.sync_solution();
     double r0, bn;
     if ((rc = A.residuals(&r0))) return rc;
     if ((rc = A.masked_norm(&bn, false))) return rc;
     const double target = std::max(tol_abs, std::max(tol_rel, 1.e-16) * std::max(r0, bn));
-    Cycle<double> c64{hs[0], s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
-    Cycle<float> c32{hs[0], s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver};
+    Cycle<double> c64{hs[0], s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
+                      o.smoother};
+    Cycle<float> c32{hs[0], s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
+                     o.smoother};
     double nrm = r0;
     int iters = 0, status = nrm <= target ? 0 : ALAMO_EL_NOT_CONVERGED;
     const long M0 = 3 * A.L(0).L.N;
+    // nu Fsmooth calls of patch i on its correction (ghost rows held)
+    auto smooth_patch = [&](int i, int nu) {
+        Level& lv = A.L(i);
+        if (o.smoother == ALAMO_EL_SMOOTH_RBGS) {
+            Cycle<double> ci{hs[i], s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel,
+                             o.bottom_solver, o.smoother};
+            ci.smooth(0, lv.d.cor, lv.d.res, nu);
+            return;
+        }
+        for (int q = 0; q < 2 * nu; ++q) {
+            double* src = (q % 2 == 0) ? lv.d.cor : lv.d.tmp;
+            double* dst = (q % 2 == 0) ? lv.d.tmp : lv.d.cor;
+            launch_jacobi<double>(hs[i], lv, src, lv.d.res, lv.d.diag, lv.d.C, dst, s);
+        }
+    };
     for (int it = 0; it < o.max_iter && status != 0; ++it) {
-        // down: pre-smooth each patch on its composite residual, reflux the
-        // remaining residual into the parent
-        for (int l = nlev - 1; l >= 1; --l) {
-            Level& lv = A.L(l);
+        // down: pre-smooth each patch on its composite residual (deepest
+        // first), reflux the remaining residual into its parent
+        for (int i : A.order) {
+            if (i == 0) continue;
+            Level& lv = A.L(i);
             const long M = 3 * lv.L.N;
             launch_fill<double>(lv.d.cor, 0.0, M, s);
-            A.ghost(l, lv.d.res, lv.d.cor, 1);
-            for (int q = 0; q < 2 * o.nu1; ++q) {
-                double* src = (q % 2 == 0) ? lv.d.cor : lv.d.tmp;
-                double* dst = (q % 2 == 0) ? lv.d.tmp : lv.d.cor;
-                launch_jacobi<double>(hs[l], lv, src, lv.d.res, lv.d.diag, lv.d.C, dst, s);
-            }
-            launch_residual<double>(hs[l], lv, lv.d.cor, lv.d.res, lv.d.C, lv.d.rescor, s);
-            A.ghost(l, lv.d.rescor, nullptr, 0);
-            A.reflux(l, lv.d.rescor, A.L(l - 1).d.res);
+            A.ghost(i, lv.d.res, lv.d.cor, 1);
+            smooth_patch(i, o.nu1);
+            launch_residual<double>(hs[i], lv, lv.d.cor, lv.d.res, lv.d.C, lv.d.rescor, s);
+            A.ghost(i, lv.d.rescor, nullptr, 0);
+            A.reflux(i, lv.d.rescor, A.L(A.par[i]).d.res);
         }
         // base level: one multigrid cycle on its composite residual
         if (fp32) {
@@ -1948,16 +2061,14 @@ int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const doub
         } else {
             c64.run(o.cycle);
         }
-        // up: add the interpolated parent correction, post-smooth
-        for (int l = 1; l < nlev; ++l) {
-            Level& lv = A.L(l);
-            A.interp(l, A.L(l - 1).d.cor, lv.d.cor, 1);
-            A.ghost(l, lv.d.res, lv.d.cor, 1);
-            for (int q = 0; q < 2 * o.nu2; ++q) {
-                double* src = (q % 2 == 0) ? lv.d.cor : lv.d.tmp;
-                double* dst = (q % 2 == 0) ? lv.d.tmp : lv.d.cor;
-                launch_jacobi<double>(hs[l], lv, src, lv.d.res, lv.d.diag, lv.d.C, dst, s);
-            }
+        // up: add the interpolated parent correction, post-smooth (parents first)
+        for (int u = nlev - 1; u >= 0; --u) {
+            const int i = A.order[u];
+            if (i == 0) continue;
+            Level& lv = A.L(i);
+            A.interp(i, A.L(A.par[i]).d.cor, lv.d.cor, 1);
+            A.ghost(i, lv.d.res, lv.d.cor, 1);
+            smooth_patch(i, o.nu2);
         }
         for (int l = 0; l < nlev; ++l) launch_axpy<double>(x[l], 1.0, A.L(l).d.cor, 3 * A.L(l).L.N, s);
         A.sync_solution();

@@ -692,6 +692,30 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_from_zero(const T* __restrict
     if (id < n) out[id] = omega * b[id] / diag[id];
 }
 
+// Red-black Gauss-Seidel half sweep (no counterpart in the reference, whose
+// smoother is damped Jacobi, src/Operator/Operator.cpp:88-154): the nodes with
+// (i + j + k) % 2 == color take x += w r / D, r = b - A x of the current x,
+// with w = w_in on interior rows and w_bd on boundary rows; the other color
+// keeps its values.  Colors split the face neighbours of the 19-point
+// stencil; edge neighbours share a color and see each other's old values.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_rb_update(T* __restrict__ x, const T* __restrict__ r,
+                                                      const T* __restrict__ diag, Lvl L, int color, T w_in,
+                                                      T w_bd) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= L.N) return;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    if (((i + j + k) & 1) != color) return;
+    const bool bnd = i == 0 || j == 0 || k == 0 || i == L.nx - 1 || j == L.ny - 1 || k == L.nz - 1;
+    const T w = bnd ? w_bd : w_in;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const long q = (long)p * L.N + id;
+        x[q] += w * r[q] / diag[q];
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_axpy(T* __restrict__ y, T a, const T* __restrict__ x, long n) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -788,32 +812,43 @@ __global__ void __launch_bounds__(kBlock) k_bc_init(double* __restrict__ rhs, co
 }
 
 // Newton set-up, pass 1 (src/Solver/Nonlocal/Newton.H:168-184): per node
-// grad u (one-sided at faces) and DW = C:(grad u - F0).  C in SoA.
-template <int SYM>
-__global__ void __launch_bounds__(kBlock) k_newton_dw(const double* __restrict__ u, const double* __restrict__ C,
-                                                      const double* __restrict__ F0, Lvl L, double* __restrict__ gu9,
-                                                      double* __restrict__ dw9) {
-    constexpr int NC = NComp<SYM>::value;
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= L.N) return;
-    int i, j, k;
-    node_ijk(id, L, i, j, k);
+// grad u (one-sided at faces) and DW = C:(grad u - F0).  The pass covers the
+// planes [kb, kb + np): node q of that slab (q = id - kb nx ny) writes
+// gu9 (when not null) and dw9 at (3p + a) S + q, S = np nx ny.  C slot s of
+// node id is read at C[s cs + id cn] (SoA: cs = N, cn = 1; AoS: cs = 1,
+// cn = ncomp).
+__device__ __forceinline__ void newton_gradu(const double* __restrict__ u, const Lvl& L, long id, int i, int j,
+                                             int k, double gu[3][3]) {
     const long sy = L.nx, sz = (long)L.nx * L.ny;
-    double gu[3][3], A[3][3], s[3][3], c[NC];
     for (int p = 0; p < 3; ++p) {
         const double* f = u + p * L.N;
         gu[p][0] = d1(f, id, 1, i, L.nx, L.dx);
         gu[p][1] = d1(f, id, sy, j, L.ny, L.dy);
         gu[p][2] = d1(f, id, sz, k, L.nz, L.dz);
     }
+}
+
+template <int SYM>
+__global__ void __launch_bounds__(kBlock) k_newton_dw(const double* __restrict__ u, const double* __restrict__ C,
+                                                      long cs, long cn, const double* __restrict__ F0, Lvl L, int kb,
+                                                      int np, double* __restrict__ gu9, double* __restrict__ dw9) {
+    constexpr int NC = NComp<SYM>::value;
+    const long S = (long)np * L.nx * L.ny;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= S) return;
+    const long id = (long)kb * L.nx * L.ny + q;
+    int i, j, k;
+    node_ijk(id, L, i, j, k);
+    double gu[3][3], A[3][3], s[3][3], c[NC];
+    newton_gradu(u, L, id, i, j, k, gu);
     for (int p = 0; p < 3; ++p)
         for (int a = 0; a < 3; ++a) A[p][a] = gu[p][a] - (F0 ? F0[(3 * p + a) * L.N + id] : 0.0);
-    for (int q = 0; q < NC; ++q) c[q] = C[q * L.N + id];
+    for (int t = 0; t < NC; ++t) c[t] = C[t * cs + id * cn];
     contract2<double, SYM>(c, F0 ? A : gu, s);
     for (int p = 0; p < 3; ++p)
         for (int a = 0; a < 3; ++a) {
-            gu9[(3 * p + a) * L.N + id] = gu[p][a];
-            dw9[(3 * p + a) * L.N + id] = s[p][a];
+            if (gu9) gu9[(3 * p + a) * S + q] = gu[p][a];
+            dw9[(3 * p + a) * S + q] = s[p][a];
         }
 }
 
@@ -897,35 +932,42 @@ __global__ void __launch_bounds__(kFaceBlock) k_face_traction(const double* __re
     }
 }
 
-// Newton set-up, pass 2 (src/Solver/Nonlocal/Newton.H:216-231).
+// Newton set-up, pass 2 (src/Solver/Nonlocal/Newton.H:216-231) on the rows
+// of planes [k0, k1), reading gu9 and dw9 of the slab [kb, kb + np) that pass
+// 1 wrote (it must hold planes k0 - 1 .. k1 inside the domain).  With gu9 null
+// the boundary rows take grad u from u again, by the same expressions.
 __global__ void __launch_bounds__(kBlock) k_newton_rhs(const double* __restrict__ u, const double* __restrict__ b,
                                                        const double* __restrict__ gu9, const double* __restrict__ dw9,
-                                                       const int* __restrict__ bct, Lvl L, double* __restrict__ rhs) {
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= L.N) return;
+                                                       int kb, int np, const int* __restrict__ bct, Lvl L, int k0,
+                                                       int k1, double* __restrict__ rhs) {
+    const long P = (long)L.nx * L.ny, S = (long)np * P;
+    const long id = (long)k0 * P + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)k1 * P) return;
+    const long q = id - (long)kb * P;  // position in the slab of gu9 and dw9
     int i, j, k;
     node_ijk(id, L, i, j, k);
     double nrm[3];
     const int reg = region_of<double>(i, j, k, L, nrm);
     if (reg >= 0) {
         double uu[3], gu[3][3], s[3][3], f[3];
+        if (!gu9) newton_gradu(u, L, id, i, j, k, gu);
         for (int p = 0; p < 3; ++p) {
             uu[p] = u[p * L.N + id];
             for (int a = 0; a < 3; ++a) {
-                gu[p][a] = gu9[(3 * p + a) * L.N + id];
-                s[p][a] = dw9[(3 * p + a) * L.N + id];
+                if (gu9) gu[p][a] = gu9[(3 * p + a) * S + q];
+                s[p][a] = dw9[(3 * p + a) * S + q];
             }
         }
         bc_set<double>(bct + 3 * reg, uu, gu, s, nrm, f);
         for (int d = 0; d < 3; ++d) rhs[d * L.N + id] = b[d * L.N + id] - f[d];
     } else {
         // Numeric::Divergence, central (src/Numeric/Stencil.H:588-657)
-        const long st[3] = {1, (long)L.nx, (long)L.nx * L.ny};
+        const long st[3] = {1, (long)L.nx, P};
         const double h[3] = {L.dx, L.dy, L.dz};
         double dv[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < 3; ++a)
             for (int r = 0; r < 3; ++r)
-                dv[r] += (dw9[(3 * r + a) * L.N + id + st[a]] - dw9[(3 * r + a) * L.N + id - st[a]]) / 2. / h[a];
+                dv[r] += (dw9[(3 * r + a) * S + q + st[a]] - dw9[(3 * r + a) * S + q - st[a]]) / 2. / h[a];
         for (int d = 0; d < 3; ++d) rhs[d * L.N + id] = b[d * L.N + id] - dv[d];
     }
 }

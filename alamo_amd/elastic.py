@@ -44,7 +44,7 @@ class Opts(ctypes.Structure):
                 ("cycle", ctypes.c_int), ("bottom_max_iter", ctypes.c_int), ("final_smooth", ctypes.c_int),
                 ("omega", ctypes.c_double), ("bottom_tol_rel", ctypes.c_double), ("precision", ctypes.c_int),
                 ("outer", ctypes.c_int), ("restart", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("bottom_solver", ctypes.c_int), ("learned", ctypes.c_int)]
+                ("bottom_solver", ctypes.c_int), ("learned", ctypes.c_int), ("smoother", ctypes.c_int)]
 
 
 class Info(ctypes.Structure):
@@ -62,6 +62,7 @@ EXPORTS = [
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
     "alamo_el_tag_bbox", "alamo_amr_fillpatch", "alamo_el_set_psi", "alamo_el_palette_size",
+    "alamo_el_gs_smooth", "alamo_el_tag_cells", "alamo_amr_cluster",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -93,6 +94,7 @@ def lib():
         L.alamo_el_fapply.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_diagonal.argtypes = [vp, ctypes.c_int, vp, vp]
         L.alamo_el_fsmooth.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
+        L.alamo_el_gs_smooth.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
         L.alamo_el_residual.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.alamo_el_restrict.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_interp_add.argtypes = [vp, ctypes.c_int, vp, vp, vp]
@@ -121,6 +123,10 @@ def lib():
                                              ctypes.POINTER(vp)]
         L.alamo_amr_patch_info.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.alamo_amr_fillpatch.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.alamo_el_tag_cells.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, vp]
+        L.alamo_amr_cluster.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]
         L.alamo_amr_solve.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(Opts), ctypes.POINTER(Info), vp]
         _lib = L
@@ -291,6 +297,13 @@ class ElasticOperator:
                "alamo_el_fsmooth")
         return x
 
+    def gs_smooth(self, x, b, mglev: int = 0, nsmooth: int = 1):
+        """In place: nsmooth red-black Gauss-Seidel sweeps (alamo_el_gs_smooth;
+        no counterpart in the reference)."""
+        _check(lib().alamo_el_gs_smooth(self._h, mglev, self._v(x, mglev), self._v(b, mglev), int(nsmooth),
+                                        _stream()), "alamo_el_gs_smooth")
+        return x
+
     def residual(self, x, b, mglev: int = 0, out=None):
         r = self._vec(mglev) if out is None else out
         _check(lib().alamo_el_residual(self._h, mglev, self._v(x, mglev), self._v(b, mglev), self._v(r, mglev), _stream()), "alamo_el_residual")
@@ -312,6 +325,8 @@ class ElasticOperator:
         return rhs
 
     def newton_rhs(self, C, u, b, F0=None, layout: int = SOA, out=None):
+        """Newton<T>::prepareForSolve right-hand side, b - N(u)
+        (src/Solver/Nonlocal/Newton.H:151-231)."""
         r = self._vec(0) if out is None else out
         f0 = self._v(F0, 0, 9) if F0 is not None else ctypes.c_void_p()
         shape = self.level_shape(0)
@@ -319,8 +334,7 @@ class ElasticOperator:
         if tuple(C.shape) != want:
             raise AlamoError(f"newton_rhs: expected C of shape {want}, got {tuple(C.shape)}")
         _check(lib().alamo_el_newton_rhs(self._h, _ptr(C), int(layout), f0, self._v(u, 0), self._v(b, 0),
-                                         self._v(r, 0), _stream()),
-               "alamo_el_newton_rhs")
+                                         self._v(r, 0), _stream()), "alamo_el_newton_rhs")
         return r
 
     def mix_models(self, eta, table, out=None):
@@ -420,6 +434,56 @@ class ElasticOperator:
             chi.append(hi)
         return tuple(clo), tuple(chi)
 
+    def tag_cells(self, eta, threshold: float = 0.1):
+        """The cell tags of tag_box before the bounding box: a uint8 tensor
+        (nz-1, ny-1, nx-1) on the GPU, 1 where a cell is tagged
+        (alamo_el_tag_cells)."""
+        shape = self.level_shape(0)
+        if eta.dim() != 4 or tuple(eta.shape[1:]) != shape:
+            raise AlamoError(f"tag_cells: expected eta of shape (ncomp,) + {shape}, got {tuple(eta.shape)}")
+        torch = _torch()
+        tags = torch.empty(tuple(n - 1 for n in shape), dtype=torch.uint8, device=eta.device)
+        _check(lib().alamo_el_tag_cells(self._h, _ptr(eta), int(eta.shape[0]), float(threshold),
+                                        ctypes.c_void_p(tags.data_ptr()), _stream()), "alamo_el_tag_cells")
+        return tags
+
+    def child_boxes(self, eta, threshold: float = 0.1, buffer: int = 2, blocking_factor: int = 0,
+                    grid_eff: float = 0.7, min_cells: int = 2):
+        """Cells [(clo, chi)] (global, this level) of the next level's patches
+        from this level's tags, clustered into several boxes
+        (alamo_amr_cluster: Berger-Rigoutsos with grid_eff, boxes aligned to
+        the blocking factor, boxes closer than one cell merged), each clipped
+        so that the patch and its ghost layer lie on this handle's unknowns;
+        boxes thinner than min_cells along an axis are dropped.  Only cells
+        among this patch's own unknowns (not its ghost layer) are tagged."""
+        import numpy as np
+        tags = self.tag_cells(eta, threshold).cpu().numpy()
+        nodes = self.level_shape(0)[::-1]
+        # cells of the handle that lie among this patch's own cells
+        lo = [self.ghost_lo[a] for a in range(3)]
+        hi = [nodes[a] - 1 - self.ghost_hi[a] for a in range(3)]  # exclusive cell bound
+        sub = np.ascontiguousarray(tags[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]])
+        if sub.size == 0 or not sub.any():
+            return []
+        nc = (ctypes.c_int * 3)(sub.shape[2], sub.shape[1], sub.shape[0])
+        org = (ctypes.c_int * 3)(*[self.origin[a] + lo[a] for a in range(3)])
+        cap = 4096
+        out = (ctypes.c_int * (6 * cap))()
+        nb = ctypes.c_int()
+        _check(lib().alamo_amr_cluster(sub.ctypes.data, nc, org, int(buffer), int(blocking_factor), float(grid_eff),
+                                       cap, out, ctypes.byref(nb)), "alamo_amr_cluster")
+        boxes = []
+        for q in range(nb.value):
+            clo, chi = [], []
+            for a in range(3):
+                ulo = self.origin[a] + self.ghost_lo[a]
+                uhi = self.origin[a] + nodes[a] - 1 - self.ghost_hi[a]
+                clo.append(max(out[6 * q + a], ulo + (1 if ulo > 0 else 0)))
+                chi.append(min(out[6 * q + 3 + a], uhi - (1 if uhi < self.dom[a] else 0)))
+            if all(chi[a] - clo[a] >= min_cells for a in range(3)):
+                boxes.append((tuple(clo), tuple(chi)))
+        return boxes
+
     def fill_patch(self, parent_u, old=None, old_u=None):
         """This patch's field after a regrid (alamo_amr_fillpatch;
         Integrator::FillPatch / FillCoarsePatch, src/Integrator/Integrator.cpp:
@@ -506,11 +570,12 @@ class Linear:
     the reference's; "direct": the cycles stop at the finest MG level with at
     most 4096 unknowns and solve there with the FP64 inverse of its operator) and
     ``learned`` ("", "guess", "guess+precond": the learned coarse-level solver
-    attached with ElasticOperator.set_learned)."""
+    attached with ElasticOperator.set_learned) and ``smoother`` ("jacobi", the
+    reference's damped Jacobi; "rbgs": red-black Gauss-Seidel)."""
 
     def __init__(self, op: ElasticOperator, max_iter=200, pre_smooth=2, post_smooth=2, bottom_max_iter=200,
                  final_smooth=8, bottom_tol_rel=1e-4, omega=2.0 / 3.0, cycle="F", precision="fp64", outer="mlmg",
-                 restart=30, verbose=0, abort_on_fail=True, bottom_solver="bicgstab", learned=""):
+                 restart=30, verbose=0, abort_on_fail=True, bottom_solver="bicgstab", learned="", smoother="jacobi"):
         self.op = op
         o = default_opts()
         o.max_iter, o.nu1, o.nu2 = int(max_iter), int(pre_smooth), int(post_smooth)
@@ -521,6 +586,7 @@ class Linear:
         o.outer = {"mlmg": OUTER_MLMG, "fgmres": OUTER_FGMRES}[outer]
         o.restart, o.verbose = int(restart), int(verbose)
         o.bottom_solver = {"bicgstab": 0, "direct": 1}[bottom_solver]
+        o.smoother = {"jacobi": 0, "rbgs": 1}[smoother]
         o.learned = {"": 0, "guess": LEARNED_GUESS, "precond": LEARNED_PRECOND,
                      "guess+precond": LEARNED_GUESS | LEARNED_PRECOND}[learned]
         self.opts = o
@@ -538,33 +604,65 @@ class Linear:
         return self.info.res
 
 
-def amr_levels(base, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2, blocking_factor: int = 0):
+def amr_levels(base, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2, blocking_factor: int = 0,
+               grid_eff=None):
     """AMR levels from the device tags (Integrator::Mechanics::
-    TagCellsForRefinement, src/Integrator/Mechanics.H:298-313, one box per
-    level): starting from the base operator, each level tags its cells from
-    eta_of(op), the (ncomp,) + level shape indicator field on op's nodes
-    (float64, GPU), and refines the clipped bounding box (child_box), up to
-    `levels` levels.  Returns [base, patch 1, ...]; set each patch's model."""
+    TagCellsForRefinement, src/Integrator/Mechanics.H:298-313): starting from
+    the base operator, each level tags its cells from eta_of(op), the
+    (ncomp,) + level shape indicator field on op's nodes (float64, GPU), up to
+    `levels` levels.  With grid_eff None every level is one patch, the clipped
+    bounding box of the tags (child_box); with a grid_eff every patch's tags are
+    clustered into several boxes (child_boxes).  Returns [base, patches...],
+    parents before children; set each patch's model."""
     ops = [base]
-    while len(ops) < levels:
-        box = ops[-1].child_box(eta_of(ops[-1]), threshold, buffer, blocking_factor=blocking_factor)
-        if box is None:
+    front = [base]
+    for _ in range(levels - 1):
+        nxt = []
+        for op in front:
+            if grid_eff is None:
+                box = op.child_box(eta_of(op), threshold, buffer, blocking_factor=blocking_factor)
+                boxes = [] if box is None else [box]
+            else:
+                boxes = op.child_boxes(eta_of(op), threshold, buffer, blocking_factor=blocking_factor,
+                                       grid_eff=grid_eff)
+            nxt += [op.refine(*b) for b in boxes]
+        if not nxt:
             break
-        ops.append(ops[-1].refine(*box))
+        ops += nxt
+        front = nxt
     return ops
 
 
-def amr_regrid(old_ops, old_xs, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2):
+def _depth(op):
+    return 0 if op.parent is None else 1 + _depth(op.parent)
+
+
+def amr_regrid(old_ops, old_xs, eta_of, levels: int = 3, threshold: float = 0.1, buffer: int = 2,
+               blocking_factor: int = 0, grid_eff=None):
     """Regrid (AmrCore::regrid with Integrator::RemakeLevel / MakeNewLevelFromCoarse,
     src/Integrator/Integrator.cpp:394-478): new patches over the same base from
-    the tags (amr_levels), and each new level's field filled coarse to fine from
-    the old level of the same depth and the already regridded parent
-    (fill_patch).  Returns (ops, xs) with xs[0] = old_xs[0]."""
-    ops = amr_levels(old_ops[0], eta_of, levels, threshold, buffer)
+    the tags (amr_levels), and each new patch's field filled, parents first:
+    the already regridded parent's field interpolated, overwritten by the
+    values of every old patch of the same level where that patch's unknowns
+    reach (fill_patch).  Returns (ops, xs) with xs[0] = old_xs[0]."""
+    ops = amr_levels(old_ops[0], eta_of, levels, threshold, buffer, blocking_factor, grid_eff)
     xs = [old_xs[0]]
-    for l in range(1, len(ops)):
-        old = old_ops[l] if l < len(old_ops) else None
-        xs.append(ops[l].fill_patch(xs[l - 1], old, old_xs[l] if old is not None else None))
+    for op in ops[1:]:
+        parent_u = xs[next(q for q, o in enumerate(ops) if o is op.parent)]
+        u = op.fill_patch(parent_u)
+        nodes = op.level_shape(0)[::-1]
+        d = _depth(op)
+        for old, old_u in zip(old_ops[1:], old_xs[1:]):
+            if _depth(old) != d:
+                continue
+            lo = [max(2 * old.clo[a], op.origin[a]) - op.origin[a] for a in range(3)]
+            hi = [min(2 * old.chi[a], op.origin[a] + nodes[a] - 1) - op.origin[a] for a in range(3)]
+            if any(hi[a] < lo[a] for a in range(3)):
+                continue
+            t = op.fill_patch(parent_u, old, old_u)
+            sl = (slice(None), slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+            u[sl] = t[sl]
+        xs.append(u)
     return ops, xs
 
 

@@ -58,9 +58,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-# The FP64 MajorMinor Fapply kernel of the finest level (LDS-tiled, C read
-# through the palette of the grains' distinct tensors).
-FAPPLY_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 2>"
+# The FP64 MajorMinor Fapply kernel of the finest level (LDS-tiled): the
+# roofline figure is taken on the variant that reads every node's 21 C values
+# (CM 1: the tile-blocked copy, non-temporal loads), whose HBM traffic carries
+# the algorithmic bytes; the variant that reads C through the palette of the
+# grains' distinct tensors (CM 2) moves far fewer bytes and is reported beside it.
+FAPPLY_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 1>"
+FAPPLY_PALETTE_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 2>"
 COPY_BYTES = 1 << 28  # calibration copy: 256 MiB read and 256 MiB written
 # FGMRES restart of the fast path: the classical Gram-Schmidt passes read
 # every basis vector twice per step, so a short basis is cheaper; 10 took 23
@@ -72,24 +76,46 @@ FAST_RESTART = 10
 FAST_NU = 3
 
 
+def operators(p, C):
+    """The workload's operator twice: reading C through its palette (the
+    default) and through the full per-node copy (ALAMO_EL_PALETTE=0 at
+    SetModel)."""
+    from alamo_amd import elastic as el
+    ops = []
+    for palette in (True, False):
+        op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
+        op.SetBC(p["bc"])
+        prev = os.environ.get("ALAMO_EL_PALETTE")
+        if not palette:
+            os.environ["ALAMO_EL_PALETTE"] = "0"
+        try:
+            op.SetModel(C)
+        finally:
+            if prev is None:
+                os.environ.pop("ALAMO_EL_PALETTE", None)
+            else:
+                os.environ["ALAMO_EL_PALETTE"] = prev
+        ops.append(op)
+    return ops
+
+
 def pmc_child(args):
     """Workload of one traffic pass under rocprofv3 --pmc: the calibration copy
-    (an elementwise kernel of known size) and 3 Fapply launches."""
+    (an elementwise kernel of known size) and 3 launches of each Fapply
+    variant."""
     import torch
 
-    from alamo_amd import elastic as el
     from alamo_amd import problems
     p = problems.polycrystal(args.n_cell, ngrains=args.grains, seed=args.seed)
-    op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
-    op.SetBC(p["bc"])
-    op.SetModel(torch.from_numpy(p["C"]).cuda())
+    ops = operators(p, torch.from_numpy(p["C"]).cuda())
     a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
     c = torch.empty_like(a)
     torch.add(a, 1.0, out=c)
-    u = torch.randn((3,) + op.level_shape(0), dtype=torch.float64, device="cuda")
+    u = torch.randn((3,) + ops[0].level_shape(0), dtype=torch.float64, device="cuda")
     f = torch.empty_like(u)
-    for _ in range(3):
-        op.Fapply(u, out=f)
+    for op in ops:
+        for _ in range(3):
+            op.Fapply(u, out=f)
     torch.cuda.synchronize()
 
 
@@ -115,22 +141,26 @@ def measure_traffic(args):
         if r.returncode != 0 or not files:
             log(f"traffic pass {ctr} failed (rc {r.returncode}): {r.stderr[-300:]}")
             return None
-        fap, cal = [], []
+        fap, pal, cal = [], [], []
         for row in csv.DictReader(open(files[0])):
             if row["Counter_Name"] != ctr:
                 continue
             if FAPPLY_KERNEL in row["Kernel_Name"]:
                 fap.append(float(row["Counter_Value"]))
+            elif FAPPLY_PALETTE_KERNEL in row["Kernel_Name"]:
+                pal.append(float(row["Counter_Value"]))
             elif "elementwise" in row["Kernel_Name"]:
                 cal.append(float(row["Counter_Value"]))
         shutil.rmtree(d, ignore_errors=True)
-        if not fap or not cal:
+        if not fap or not pal or not cal:
             log(f"traffic pass {ctr}: kernels not found")
             return None
-        kib = float(np.mean(fap))
         cal_kib = max(cal)  # the 256 MiB copy is the largest elementwise kernel of the pass
         factor = (COPY_BYTES / 1024.0) / cal_kib
-        out[ctr] = dict(raw_kib=kib, calib_kib=cal_kib, factor=factor, bytes=kib * 1024.0 * factor)
+        for key, vals in (("copy", fap), ("palette", pal)):
+            kib = float(np.mean(vals))
+            out.setdefault(key, {})[ctr] = dict(raw_kib=kib, calib_kib=cal_kib, factor=factor,
+                                                bytes=kib * 1024.0 * factor)
     return out
 
 
@@ -151,9 +181,10 @@ def main():
     t0 = time.time()
     p = problems.polycrystal(args.n_cell, ngrains=args.grains, seed=args.seed)
     log(f"built {p['name']} on the host in {time.time() - t0:.1f}s")
+    C = torch.from_numpy(p["C"]).cuda()
+    op_copy = operators(p, C)[1]
     op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
     op.SetBC(p["bc"])
-    C = torch.from_numpy(p["C"]).cuda()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     op.SetModel(C)  # gradient masks, compact grad-C matrices, C copies, palette
@@ -174,20 +205,38 @@ def main():
     # --- Fapply roofline (HIP events on the stream the kernels run on) ------
     u = torch.randn((3,) + shape, dtype=torch.float64, device="cuda")
     f = torch.empty_like(u)
-    for _ in range(5):
-        op.Fapply(u, out=f)
-    stream = torch.cuda.current_stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.fapply_reps):
-        op.Fapply(u, out=f)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    fapply_ms = ev0.elapsed_time(ev1) / args.fapply_reps
+
+    def fapply_ms_of(o, uu, ff):
+        for _ in range(5):
+            o.Fapply(uu, out=ff)
+        stream = torch.cuda.current_stream()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.fapply_reps):
+            o.Fapply(uu, out=ff)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / args.fapply_reps
+
     bytes_per_launch = FAPPLY_BYTES_PER_NODE[p["sym"]] * nodes
+    fapply_ms = fapply_ms_of(op_copy, u, f)          # every node's 21 C values read
     fapply_gbs = bytes_per_launch / (fapply_ms * 1e-3) / 1e9
+    palette_ms = fapply_ms_of(op, u, f)              # C through the palette
+    palette_gbs = bytes_per_launch / (palette_ms * 1e-3) / 1e9
     palette_rows = op.palette_size()
-    log(f"Fapply {fapply_ms * 1e3:.1f} us/launch, {fapply_gbs:.0f} GB/s algorithmic, C palette of {palette_rows} rows")
+    assert op_copy.palette_size() == 0
+    log(f"Fapply (C copy) {fapply_ms * 1e3:.1f} us/launch, {fapply_gbs:.0f} GB/s algorithmic; "
+        f"with the C palette of {palette_rows} rows {palette_ms * 1e3:.1f} us")
+    del op_copy
+    # isotropic C (2 values per node) at the same size: the spherical inclusion
+    pi = problems.inclusion(args.n_cell)
+    op_iso = el.ElasticOperator(pi["n_cell"], pi["dx"], pi["sym"])
+    op_iso.SetBC(pi["bc"])
+    op_iso.SetModel(torch.from_numpy(pi["C"]).cuda())
+    iso_ms = fapply_ms_of(op_iso, u, f)
+    iso_bytes = FAPPLY_BYTES_PER_NODE[pi["sym"]] * nodes
+    log(f"Fapply isotropic {iso_ms * 1e3:.1f} us/launch, {iso_bytes / (iso_ms * 1e-3) / 1e9:.0f} GB/s algorithmic")
+    del op_iso
 
     # --- plain FP64 multigrid with the reference settings --------------------
     plain = {}
@@ -248,11 +297,13 @@ def main():
         f"{head['learned_steps']} learned steps taken, rel res {head['rel_res']:.2e}")
 
     traffic = None if args.no_traffic else measure_traffic(args)
-    traffic_bytes = None
+    traffic_bytes = palette_traffic = None
     if traffic:
-        traffic_bytes = traffic["FETCH_SIZE"]["bytes"] + traffic["WRITE_SIZE"]["bytes"]
+        traffic_bytes = traffic["copy"]["FETCH_SIZE"]["bytes"] + traffic["copy"]["WRITE_SIZE"]["bytes"]
+        palette_traffic = traffic["palette"]["FETCH_SIZE"]["bytes"] + traffic["palette"]["WRITE_SIZE"]["bytes"]
         log(f"Fapply HBM traffic {traffic_bytes / 1e6:.1f} MB per launch "
-            f"({traffic_bytes / bytes_per_launch:.2f}x the algorithmic {bytes_per_launch / 1e6:.1f} MB)")
+            f"({traffic_bytes / bytes_per_launch:.2f}x the algorithmic {bytes_per_launch / 1e6:.1f} MB); "
+            f"with the palette {palette_traffic / 1e6:.1f} MB")
 
     # --- CPU baseline: the FP64 oracle on the host cores (bounded sample) ----
     cpu = None
@@ -272,7 +323,15 @@ def main():
         _, info = O.mlmg_solve(rhs_h, Ch, p["sym"], p["bc"], p["dx"], tol_rel=args.tol, max_iter=2, cycle="F")
         cpu_cycle_s = (time.perf_counter() - t) / 2.0
         plain_iters = plain.get("F", {}).get("iters", 44)
-        cpu = dict(value=cpu_cycle_s * plain_iters, unit="s", cores=O.num_threads(), kind="port",
+        cpu_model = "unknown"
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = dict(value=cpu_cycle_s * plain_iters, unit="s", cores=O.num_threads(), kind="port", cpu=cpu_model,
                    sample=(f"FP64 oracle: 2 F-cycles of the same {p['name']} problem "
                            f"({cpu_cycle_s:.2f} s/cycle) scaled to the {plain_iters} cycles of the plain FP64 "
                            f"solve; Fapply median of 20 = {cpu_fapply_s * 1e3:.0f} ms "
@@ -306,12 +365,18 @@ def main():
         "speedup_vs_plain_fp64_F": (plain["F"]["seconds"] / fast_t) if "F" in plain else None,
         "setup_once_per_model_s": {"set_model": set_model_s, "prepare": prepare_s},
         "fapply": {"us_per_launch": fapply_ms * 1e3, "gbs": fapply_gbs, "bytes_per_node": FAPPLY_BYTES_PER_NODE[p["sym"]],
-                   "c_palette_rows": palette_rows},
+                   "c": "per-node copy (21 values)"},
+        "fapply_palette": {"us_per_launch": palette_ms * 1e3, "gbs_algorithmic": palette_gbs,
+                           "c_palette_rows": palette_rows, "traffic": palette_traffic,
+                           "hbm_gbs": palette_traffic / (palette_ms * 1e-3) / 1e9 if palette_traffic else None,
+                           "note": ("C read as a 2-byte row index per node into a table of the distinct tensors: "
+                                    "the 216 B/node of the algorithmic figure are not all moved, so its "
+                                    "gbs_algorithmic is not an HBM rate; hbm_gbs is")},
+        "fapply_isotropic": {"us_per_launch": iso_ms * 1e3, "bytes_per_node": FAPPLY_BYTES_PER_NODE[pi["sym"]],
+                             "gbs": iso_bytes / (iso_ms * 1e-3) / 1e9, "workload": pi["name"]},
         "roofline": {"bound": "hbm", "achieved": fapply_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": fapply_gbs / PEAK_HBM_GBS, "traffic": traffic_bytes,
-                     "kernel": "k_stencil_lds<Tile<32,8>,double,Fapply,compact grad-C,C palette>",
-                     "note": ("achieved counts the reference layout's 216 B/node; with C read through the palette "
-                              "the kernel moves `traffic` bytes, so frac near 1 is not an HBM bound"),
+                     "kernel": "k_stencil_lds<Tile<32,8>,double,Fapply,compact grad-C,C copy>",
                      "algorithmic_bytes": bytes_per_launch,
                      "traffic_passes": traffic},
         "cpu_baseline": cpu,

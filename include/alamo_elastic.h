@@ -68,6 +68,14 @@ extern "C" {
 #define ALAMO_EL_OUTER_MLMG 0     /* x += cycle(b - A x) until converged */
 #define ALAMO_EL_OUTER_FGMRES 1   /* flexible GMRES, one cycle as preconditioner */
 
+/* Smoother of the cycles: the reference's damped Jacobi (2 sweeps per Fsmooth
+ * call, src/Operator/Operator.cpp:88-154), or red-black Gauss-Seidel (no
+ * counterpart in the reference): one Fsmooth call is a red and a black half
+ * sweep over the colors (i + j + k) % 2, each x += w (b - A x) / D on its color
+ * with w = 0.9 on interior rows and 1/2 on boundary rows. */
+#define ALAMO_EL_SMOOTH_JACOBI 0
+#define ALAMO_EL_SMOOTH_RBGS 1
+
 #define ALAMO_EL_BOTTOM_BICGSTAB 0 /* the reference bottom solver (Linear.H:258) */
 /* The cycles stop at the finest MG level with at most 4096 unknowns and solve
  * there with the FP64 inverse of its operator (computed on the device by
@@ -102,6 +110,7 @@ typedef struct {
     int verbose;           /* 0 */
     int bottom_solver;     /* ALAMO_EL_BOTTOM_BICGSTAB */
     int learned;           /* 0, or ALAMO_EL_LEARNED_* bits (needs alamo_el_set_learned) */
+    int smoother;          /* ALAMO_EL_SMOOTH_JACOBI */
 } alamo_el_opts;
 
 typedef struct {
@@ -175,6 +184,11 @@ int alamo_el_diagonal(alamo_el_t h, int mglev, double* diag, hipStream_t stream)
  * Jacobi sweeps of x against b (src/Operator/Operator.cpp:88-154).  Needs
  * alamo_el_prepare. */
 int alamo_el_fsmooth(alamo_el_t h, int mglev, double* x, const double* b, int nsmooth, hipStream_t stream);
+/* The red-black Gauss-Seidel smoother (ALAMO_EL_SMOOTH_RBGS) called nsmooth
+ * times on MG level mglev: nsmooth red and black half sweeps of x against b.
+ * No counterpart in the reference (see alamo_el_fsmooth for its smoother).
+ * Needs alamo_el_prepare. */
+int alamo_el_gs_smooth(alamo_el_t h, int mglev, double* x, const double* b, int nsmooth, hipStream_t stream);
 /* correctionResidual / solutionResidual: r = b - A x
  * (src/Operator/Operator.cpp:635-657). */
 int alamo_el_residual(alamo_el_t h, int mglev, const double* x, const double* b, double* r, hipStream_t stream);
@@ -272,6 +286,25 @@ int alamo_el_learned_guess(alamo_el_t h, const double* b, double* x, hipStream_t
  * boxes by AmrCore is not restated).  Synchronizes the stream. */
 int alamo_el_tag_bbox(alamo_el_t h, const double* eta, int ncomp, double threshold, int buffer, int clo[3],
                       int chi[3], long long* ntag, hipStream_t stream);
+/* The same tags, one byte per cell of h's finest level (device array of
+ * (nx-1)(ny-1)(nz-1) bytes, x fastest; 1 = tagged). */
+int alamo_el_tag_cells(alamo_el_t h, const double* eta, int ncomp, double threshold, unsigned char* tags,
+                       hipStream_t stream);
+/* Boxes of the next AMR level from cell tags (AmrCore::MakeNewGrids with
+ * amrex::ClusterList; parameters amr.n_error_buf, amr.blocking_factor,
+ * amr.grid_eff, src/Integrator/Integrator.cpp:23-140).  Host only, no GPU work.
+ * tags_host: ncell[0] x ncell[1] x ncell[2] bytes (x fastest) whose cell
+ * (0, 0, 0) is global cell origin.  Tags are grown by buffer cells, coarsened
+ * by blocking_factor / 2 on the global grid (boxes aligned to it), clustered by
+ * Berger-Rigoutsos (a box is kept when its tagged fraction reaches grid_eff,
+ * else cut at a hole of the tag signature, else at its strongest inflection,
+ * else in the middle of its longest axis), refined back and clipped to the
+ * array; boxes closer than one cell are merged, so that the patches of a level
+ * never share nodes (the composite solve couples a patch with its parent
+ * only).  boxes receives 6 ints per box: clo[3], chi[3] in global cells, the
+ * form alamo_amr_patch_create takes (chi = last cell + 1). */
+int alamo_amr_cluster(const unsigned char* tags_host, const int ncell[3], const int origin[3], int buffer,
+                      int blocking_factor, double grid_eff, int max_boxes, int* boxes, int* nboxes);
 /* Patch of the next AMR level covering the parent's cells [clo, chi] (global
  * parent-level cell indices, clo < chi <= domain cells).  The new handle holds
  * the fine nodes 2 clo .. 2 chi plus one ghost node layer on each face inside

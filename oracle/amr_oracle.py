@@ -190,26 +190,42 @@ def covered_mask(parent, child):
             & ((I > x0) & (I < x1))[None, None, :])
 
 
-def residuals(levels, xs, bs):
+def _tree(levels):
+    """(parent, children, deepest-first order) of a level list: levels[0] is
+    the base; a Level with a `parent` index hangs under that level, otherwise
+    under the one before it (a chain of single-box levels)."""
     n = len(levels)
-    rs = [None] * n
-    for l in range(n - 1, -1, -1):
+    par = [None] + [getattr(levels[i], "parent", None) for i in range(1, n)]
+    par = [None] + [i - 1 if par[i] is None else par[i] for i in range(1, n)]
+    depth = [0] * n
+    for i in range(1, n):
+        depth[i] = depth[par[i]] + 1
+    kids = [[c for c in range(n) if par[c] == i] for i in range(n)]
+    order = sorted(range(n), key=lambda i: -depth[i])
+    return par, kids, order
+
+
+def residuals(levels, xs, bs):
+    par, kids, order = _tree(levels)
+    rs = [None] * len(levels)
+    for l in order:
         lv = levels[l]
         r = O.residual(xs[l], bs[l], lv.C, lv.sym, lv.bc, lv.dx)
         if l > 0:
             r[:, ghost_mask(lv.shape, lv.gl, lv.gh)] = 0.0
-        if l + 1 < n:
-            reflux(rs[l + 1], levels[l + 1], r, lv)
+        for c in kids[l]:
+            reflux(rs[c], levels[c], r, lv)
         rs[l] = r
     return rs
 
 
 def masked_norm(levels, vs):
+    par, kids, order = _tree(levels)
     m = 0.0
     for l, lv in enumerate(levels):
         v = vs[l].copy()
-        if l + 1 < len(levels):
-            v[:, covered_mask(lv, levels[l + 1])] = 0.0
+        for c in kids[l]:
+            v[:, covered_mask(lv, levels[c])] = 0.0
         if l > 0:
             v[:, ghost_mask(lv.shape, lv.gl, lv.gh)] = 0.0
         m = max(m, float(np.abs(v).max()))
@@ -217,11 +233,14 @@ def masked_norm(levels, vs):
 
 
 def sync(levels, xs):
-    for l in range(len(levels) - 1, 0, -1):
-        inject(xs[l], levels[l], xs[l - 1], levels[l - 1])
-    for l in range(1, len(levels)):
-        g = ghost_mask(levels[l].shape, levels[l].gl, levels[l].gh)
-        xs[l][:, g] = interp(xs[l - 1], levels[l - 1], levels[l])[:, g]
+    par, kids, order = _tree(levels)
+    for l in order:
+        if l > 0:
+            inject(xs[l], levels[l], xs[par[l]], levels[par[l]])
+    for l in reversed(order):
+        if l > 0:
+            g = ghost_mask(levels[l].shape, levels[l].gl, levels[l].gh)
+            xs[l][:, g] = interp(xs[par[l]], levels[par[l]], levels[l])[:, g]
 
 
 def jacobi(lv, e, r, sweeps, omega=2.0 / 3.0):
@@ -230,8 +249,12 @@ def jacobi(lv, e, r, sweeps, omega=2.0 / 3.0):
 
 
 def amr_solve(levels, xs, bs, tol_rel, tol_abs=0.0, max_iter=100, nu1=2, nu2=2, cycle="F"):
+    """Composite solve over levels[0] (the base) and its patches: a chain of
+    single-box levels, or a tree when a Level carries the index of its
+    parent (`parent`); patches of one depth share no nodes."""
     xs = [x.copy() for x in xs]
     n = len(levels)
+    par, kids, order = _tree(levels)
     sync(levels, xs)
     rs = residuals(levels, xs, bs)
     r0, bn = masked_norm(levels, rs), masked_norm(levels, bs)
@@ -240,7 +263,9 @@ def amr_solve(levels, xs, bs, tol_rel, tol_abs=0.0, max_iter=100, nu1=2, nu2=2, 
     hist = [r0]
     while nrm > target and it < max_iter:
         es = [None] * n
-        for l in range(n - 1, 0, -1):
+        for l in order:
+            if l == 0:
+                continue
             lv = levels[l]
             g = ghost_mask(lv.shape, lv.gl, lv.gh)
             e = np.zeros((3,) + lv.shape)
@@ -249,15 +274,17 @@ def amr_solve(levels, xs, bs, tol_rel, tol_abs=0.0, max_iter=100, nu1=2, nu2=2, 
             e = jacobi(lv, e, r, 2 * nu1)
             rr = O.residual(e, r, lv.C, lv.sym, lv.bc, lv.dx)
             rr[:, g] = 0.0
-            reflux(rr, lv, rs[l - 1], levels[l - 1])
+            reflux(rr, lv, rs[par[l]], levels[par[l]])
             es[l] = e
         b0 = levels[0]
         es[0], _ = O.mlmg_solve(rs[0], b0.C, b0.sym, b0.bc, b0.dx, x0=np.zeros_like(rs[0]), tol_rel=1e-300,
                                 tol_abs=0.0, max_iter=1, nu1=nu1, nu2=nu2, cycle=cycle)
-        for l in range(1, n):
+        for l in reversed(order):
+            if l == 0:
+                continue
             lv = levels[l]
             g = ghost_mask(lv.shape, lv.gl, lv.gh)
-            es[l] = es[l] + interp(es[l - 1], levels[l - 1], lv)
+            es[l] = es[l] + interp(es[par[l]], levels[par[l]], lv)
             rs[l][:, g] = es[l][:, g]
             es[l] = jacobi(lv, es[l], rs[l], 2 * nu2)
         for l in range(n):
@@ -268,3 +295,179 @@ def amr_solve(levels, xs, bs, tol_rel, tol_abs=0.0, max_iter=100, nu1=2, nu2=2, 
         it += 1
         hist.append(nrm)
     return xs, dict(iters=it, res=nrm, res0=r0, rhsnorm=bn, hist=hist, status=0 if nrm <= target else 1)
+
+
+# --- Grids from tags (AmrCore::MakeNewGrids with amrex::ClusterList) ---------
+def _signature(t, b, a):
+    """Tag counts of the planes of box b (lo, hi inclusive, xyz) normal to axis a."""
+    sub = t[b[0][2]:b[1][2] + 1, b[0][1]:b[1][1] + 1, b[0][0]:b[1][0] + 1]
+    return sub.sum(axis=tuple(ax for ax in (0, 1, 2) if ax != 2 - a)).astype(np.int64)
+
+
+def _bounding(t, b):
+    sub = t[b[0][2]:b[1][2] + 1, b[0][1]:b[1][1] + 1, b[0][0]:b[1][0] + 1]
+    k, j, i = np.nonzero(sub)
+    if len(i) == 0:
+        return None, 0
+    lo = (b[0][0] + int(i.min()), b[0][1] + int(j.min()), b[0][2] + int(k.min()))
+    hi = (b[0][0] + int(i.max()), b[0][1] + int(j.max()), b[0][2] + int(k.max()))
+    return (lo, hi), len(i)
+
+
+def _berger_rigoutsos(t, box, eff, out):
+    """Berger & Rigoutsos (IEEE Trans. SMC 21(5), 1991) restated: shrink to the
+    tags, keep at tagged fraction >= eff, else cut at a hole of the signature
+    (longest axis first, the hole nearest the middle), else at the inflection
+    with the largest jump of the second difference (ties: nearest the middle,
+    then the earlier axis of the longest-first order), else halve the longest
+    axis; recurse on both halves."""
+    b, n = _bounding(t, box)
+    if b is None:
+        return
+    lo, hi = b
+    ext = [hi[a] - lo[a] + 1 for a in range(3)]
+    if n >= eff * ext[0] * ext[1] * ext[2]:
+        out.append(b)
+        return
+    order = sorted(range(3), key=lambda a: -ext[a])     # stable: lowest axis first among equals
+    cut = None
+    for a in order:
+        s = _signature(t, b, a)
+        holes = [m for m in range(1, len(s) - 1) if s[m] == 0]
+        if holes:
+            m = min(holes, key=lambda m: (abs(2 * m - len(s)), m))
+            cut = (a, lo[a] + m)
+            break
+    if cut is None:
+        best = None
+        for a in order:
+            s = _signature(t, b, a)
+            lap = s[:-2] - 2 * s[1:-1] + s[2:]          # lap[m - 1] is at cell m
+            for m in range(1, len(s) - 2):
+                l0, l1 = lap[m - 1], lap[m]
+                if (l0 < 0 < l1) or (l0 > 0 > l1):
+                    key = (-abs(int(l1 - l0)), abs(2 * (m + 1) - len(s)))
+                    if best is None or key < best[0]:
+                        best = (key, a, m + 1)
+        if best is not None:
+            cut = (best[1], lo[best[1]] + best[2])
+    if cut is None:
+        a = order[0]
+        if ext[a] < 2:
+            out.append(b)
+            return
+        cut = (a, lo[a] + ext[a] // 2)
+    a, c = cut
+    h1 = list(hi)
+    h1[a] = c - 1
+    l2 = list(lo)
+    l2[a] = c
+    _berger_rigoutsos(t, (lo, tuple(h1)), eff, out)
+    _berger_rigoutsos(t, (tuple(l2), hi), eff, out)
+
+
+def cluster_boxes(tags, origin=(0, 0, 0), buffer=1, blocking_factor=8, grid_eff=1.0):
+    """Boxes [(clo, chi)] (global cells, chi = last cell + 1) of the next level
+    from cell tags (nz, ny, nx) whose cell (0, 0, 0) is global cell `origin`:
+    tags grown by `buffer`, coarsened by blocking_factor / 2 on the global grid,
+    Berger-Rigoutsos, refined and clipped to the array, boxes closer than one
+    cell merged into their bounding box, sorted by (z, y, x) of their corner."""
+    t = np.asarray(tags, dtype=bool)
+    nz, ny, nx = t.shape
+    if buffer > 0:
+        g = t.copy()
+        for dk in range(-buffer, buffer + 1):
+            for dj in range(-buffer, buffer + 1):
+                for di in range(-buffer, buffer + 1):
+                    src = t[max(0, -dk):nz - max(0, dk), max(0, -dj):ny - max(0, dj), max(0, -di):nx - max(0, di)]
+                    g[max(0, dk):nz - max(0, -dk), max(0, dj):ny - max(0, -dj), max(0, di):nx - max(0, -di)] |= src
+        t = g
+    q = max(1, blocking_factor // 2)
+    n = (nx, ny, nz)
+    clo = [origin[a] // q for a in range(3)]
+    cn = [(origin[a] + n[a] - 1) // q - clo[a] + 1 for a in range(3)]
+    c = np.zeros((cn[2], cn[1], cn[0]), dtype=bool)
+    k, j, i = np.nonzero(t)
+    c[(origin[2] + k) // q - clo[2], (origin[1] + j) // q - clo[1], (origin[0] + i) // q - clo[0]] = True
+    coarse = []
+    _berger_rigoutsos(c, ((0, 0, 0), (cn[0] - 1, cn[1] - 1, cn[2] - 1)), grid_eff, coarse)
+    boxes = []
+    for lo, hi in coarse:
+        boxes.append([[max((lo[a] + clo[a]) * q, origin[a]) for a in range(3)],
+                      [min((hi[a] + clo[a] + 1) * q - 1, origin[a] + n[a] - 1) for a in range(3)]])
+    merged = True
+    while merged:
+        merged = False
+        for x in range(len(boxes)):
+            for y in range(x + 1, len(boxes)):
+                bx, by = boxes[x], boxes[y]
+                if all(bx[1][a] + 1 >= by[0][a] and by[1][a] + 1 >= bx[0][a] for a in range(3)):
+                    boxes[x] = [[min(bx[0][a], by[0][a]) for a in range(3)], [max(bx[1][a], by[1][a]) for a in range(3)]]
+                    del boxes[y]
+                    merged = True
+                    break
+            if merged:
+                break
+    boxes.sort(key=lambda b: (b[0][2], b[0][1], b[0][0]))
+    return [(tuple(b[0]), tuple(v + 1 for v in b[1])) for b in boxes]
+
+
+def tag_cells(eta, dx, threshold):
+    """The cell tags of oracle.tag_box (nz-1, ny-1, nx-1), before the bounding box."""
+    eta = np.asarray(eta, dtype=np.float64)
+    h = np.broadcast_to(np.asarray(dx, dtype=np.float64), (3,))
+    dxn = np.sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2])
+    tags = np.zeros(eta.shape[1:], dtype=bool)
+    for n in range(eta.shape[0]):
+        g = [np.gradient(eta[n], h[a], axis=2 - a, edge_order=1) for a in range(3)]
+        tags |= np.sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) * dxn > threshold
+    return tags[:-1, :-1, :-1]
+
+
+def child_boxes(eta, origin, shape, gl, gh, dom, dx, threshold, buffer, blocking_factor, grid_eff, min_cells=2):
+    """The next level's boxes of one handle (ElasticOperator.child_boxes restated):
+    its cell tags, only those among the patch's own cells, clustered, each box
+    clipped so that the patch and its ghost layer lie on the handle's unknowns;
+    boxes thinner than min_cells dropped.  eta on the handle's nodes; origin,
+    gl, gh, dom (x, y, z); shape (nz, ny, nx)."""
+    tags = tag_cells(eta, dx, threshold)
+    nodes = shape[::-1]
+    lo = [gl[a] for a in range(3)]
+    hi = [nodes[a] - 1 - gh[a] for a in range(3)]
+    sub = tags[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+    if sub.size == 0 or not sub.any():
+        return []
+    out = []
+    for clo, chi in cluster_boxes(sub, tuple(origin[a] + lo[a] for a in range(3)), buffer, blocking_factor, grid_eff):
+        c0, c1 = [], []
+        for a in range(3):
+            ulo, uhi = origin[a] + gl[a], origin[a] + nodes[a] - 1 - gh[a]
+            c0.append(max(clo[a], ulo + (1 if ulo > 0 else 0)))
+            c1.append(min(chi[a], uhi - (1 if uhi < dom[a] else 0)))
+        if all(c1[a] - c0[a] >= min_cells for a in range(3)):
+            out.append((tuple(c0), tuple(c1)))
+    return out
+
+
+def hierarchy(n0, eta_of, levels=3, threshold=0.1, buffer=2, blocking_factor=0, grid_eff=0.7):
+    """Patches of a base of n0^3 cells (unit spacing n0 per axis of the domain)
+    level by level from the oracle's tags: [(parent index, clo, chi,
+    origin, gl, gh, n_cell, dom)], parents before children.  eta_of(origin,
+    shape, h) gives the indicator on a box's nodes at spacing h = 1 / dom."""
+    out = []
+    front = [(-1, (0, 0, 0), (0, 0, 0), (0, 0, 0), (n0,) * 3, (n0,) * 3)]  # (index, origin, gl, gh, n_cell, dom)
+    for _ in range(levels - 1):
+        nxt = []
+        for idx, origin, gl, gh, nc, dom in front:
+            shape = tuple(nc[a] + 1 for a in (2, 1, 0))
+            h = 1.0 / dom[0]
+            boxes = child_boxes(eta_of(origin, shape, h), origin, shape, gl, gh, dom, (h,) * 3, threshold, buffer,
+                                blocking_factor, grid_eff)
+            for clo, chi in boxes:
+                o, g0, g1, ncc = patch_geometry(dom, clo, chi)
+                out.append((idx + 1, clo, chi, o, g0, g1, ncc, tuple(2 * d for d in dom)))
+                nxt.append((len(out) - 1, o, g0, g1, ncc, tuple(2 * d for d in dom)))
+        if not nxt:
+            break
+        front = nxt
+    return out

@@ -375,3 +375,78 @@ def test_uniaxial_tension_deck_three_levels_closed_form():
             print(f"{precision} level {l}: {info.iters} iterations, u_x squared rel error {err:.2e}, "
                   f"sigma_xx {serr:.2e}")
             assert err < 1e-8 and serr < 1e-8
+
+
+def _multibox(n0, spheres, width, grid_eff=0.7, blocking_factor=4, levels=3):
+    """Device hierarchy with several patches per level (clustered tags), each
+    patch's model and right-hand side, and the matching oracle levels."""
+    import alamo_amd.bc as B
+    base = el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC)
+    base.SetBC(B.Constant())
+    ops = el.amr_levels(base, _eta_of(spheres, width), levels=levels, threshold=0.1, buffer=2,
+                        blocking_factor=blocking_factor, grid_eff=grid_eff)
+    bs, lv = [], []
+    for q, op in enumerate(ops):
+        shape = op.level_shape(0)
+        C, F0 = P.inclusions_on_box(op.origin, shape, op.dx[0], spheres, width=width)
+        op.SetModel(torch.from_numpy(C).cuda())
+        b = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
+        bs.append(op.newton_rhs(torch.from_numpy(C).cuda(), torch.zeros_like(b), b, F0=torch.from_numpy(F0).cuda()))
+        if q == 0:
+            lv.append(AO.Level(C, M.ISOTROPIC, B.Constant(), op.dx[0], dom=op.dom))
+        else:
+            par = next(i for i, o in enumerate(ops) if o is op.parent)
+            L = AO.Level(C, M.ISOTROPIC, AO.patch_bc(B.Constant(), op.ghost_lo, op.ghost_hi), op.dx[0],
+                         origin=op.origin, clo=op.clo, chi=op.chi, gl=op.ghost_lo, gh=op.ghost_hi, dom=op.dom)
+            L.parent = par
+            lv.append(L)
+    return ops, bs, lv
+
+
+def test_multibox_hierarchy_matches_oracle_clustering():
+    """Several patches per level (alamo_el_tag_cells + alamo_amr_cluster):
+    the device-built tree equals the oracle's, from its own tags and its own
+    Berger-Rigoutsos, on the three-inclusion case at a 33^3 base and on a
+    second seed."""
+    for seed in (0, 7):
+        spheres = P.random_spheres(3, seed=seed)
+        w = 2.0 / 32
+        base = el.ElasticOperator(32, 1.0 / 32, M.ISOTROPIC)
+        ops = el.amr_levels(base, _eta_of(spheres, w), levels=3, threshold=0.1, buffer=2, blocking_factor=4,
+                            grid_eff=0.7)
+        dev = [(next(i for i, o in enumerate(ops) if o is op.parent), op.clo, op.chi) for op in ops[1:]]
+        ora = [(p, clo, chi) for p, clo, chi, *_ in AO.hierarchy(
+            32, lambda o, s, h: P.sphere_indicators(o, s, h, spheres, w), levels=3, threshold=0.1, buffer=2,
+            blocking_factor=4, grid_eff=0.7)]
+        print(f"seed {seed}: {dev}")
+        assert dev == ora
+        assert len(dev) > 2                                # several patches on a level
+
+
+def test_multibox_composite_matches_oracle():
+    spheres = P.random_spheres(3, seed=0)
+    w = 2.0 / 32
+    ops, bs, lv = _multibox(32, spheres, w)
+    assert len(ops) > 3
+    xs = [torch.zeros_like(b) for b in bs]
+    info = el.amr_solve(ops, xs, bs, 1e-8, 0.0, max_iter=300, cycle="V")
+    bh = [b.cpu().numpy() for b in bs]
+    xo, io = AO.amr_solve(lv, [np.zeros_like(b) for b in bh], bh, 1e-8, max_iter=300, cycle="V")
+    print(f"{len(ops)} handles: device {info.iters} composite V-cycles, oracle {io['iters']}")
+    assert info.status == 0 and io["status"] == 0 and info.iters == io["iters"]
+    for q in range(len(ops)):
+        e = np.linalg.norm(xs[q].cpu().numpy() - xo[q]) / np.linalg.norm(xo[q])
+        print(f"  handle {q} {ops[q].level_shape(0)}: rel L2 device vs oracle {e:.2e}")
+        assert e < 1e-9
+
+
+def test_patches_of_one_level_must_not_share_nodes():
+    base = el.ElasticOperator(16, 1 / 16, M.ISOTROPIC)
+    a = base.refine((2, 2, 2), (8, 8, 8))
+    b = base.refine((8, 3, 3), (12, 9, 9))      # shares the parent nodes of x = 8 with a
+    shape = [op.level_shape(0) for op in (base, a, b)]
+    for op, sh in zip((base, a, b), shape):
+        op.SetModel(torch.ones((2,) + sh, dtype=torch.float64, device="cuda"))
+    xs = [torch.zeros((3,) + sh, dtype=torch.float64, device="cuda") for sh in shape]
+    with pytest.raises(el.AlamoError, match="share nodes"):
+        el.amr_solve([base, a, b], xs, [x.clone() for x in xs], 1e-8, 0.0)

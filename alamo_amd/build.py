@@ -20,7 +20,7 @@ ARCH = os.environ.get("ALAMO_AMD_ARCH", "gfx950")
 SOURCES = ["elastic.hip", "unet.hip"]
 # headers the two sources include (all of csrc/ is compiled on every build)
 HEADERS = ["common.h", "elastic_kernels.h", "stencil.h", "stencil_lds.h", "bottom_kernels.h", "amr_kernels.h",
-           "learned_kernels.h", "unet_kernels.h", "psi_kernels.h"]
+           "learned_kernels.h", "unet_kernels.h", "psi_kernels.h", "cluster.h", "inverse_kernels.h"]
 
 # Packed FP32 instructions are disabled for all device code: hipcc -O3 forms
 # them from ordinary float code and some of their forms have given wrong

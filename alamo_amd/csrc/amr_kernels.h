@@ -207,6 +207,24 @@ __global__ void __launch_bounds__(kBlock) k_patch_reflux(const double* __restric
     }
 }
 
+// Copy the nodes [lo, hi] (global node indices, inclusive) of ncomp planes
+// from src (ns nodes per axis, node 0 at global so) into dst (nd nodes, node 0
+// at global dO): the gather of one box of a multi-box field into a level's
+// single-box layout, or the scatter back.
+__global__ void __launch_bounds__(kBlock) k_box_copy(double* __restrict__ dst, int ndx, int ndy, int ndz, int dox,
+                                                     int doy, int doz, const double* __restrict__ src, int nsx, int nsy,
+                                                     int nsz, int sox, int soy, int soz, int lox, int loy, int loz,
+                                                     int bx, int by, int bz, int ncomp) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)bx * by * bz) return;
+    const int a = (int)(id % bx), b = (int)((id / bx) % by), c = (int)(id / ((long)bx * by));
+    const int gx = lox + a, gy = loy + b, gz = loz + c;
+    const long di = ((long)(gz - doz) * ndy + (gy - doy)) * ndx + (gx - dox);
+    const long si = ((long)(gz - soz) * nsy + (gy - soy)) * nsx + (gx - sox);
+    const long Nd = (long)ndx * ndy * ndz, Ns = (long)nsx * nsy * nsz;
+    for (int p = 0; p < ncomp; ++p) dst[p * Nd + di] = src[p * Ns + si];
+}
+
 // out = v with the parent nodes strictly inside the patch (covered by the
 // finer level, excluded from the composite norm) set to 0.
 __global__ void __launch_bounds__(kBlock) k_mask_covered(const double* v, Lvl L, int pox, int poy, int poz, Patch P,

@@ -26,6 +26,7 @@
 #include "cluster.h"
 #include "common.h"
 #include "elastic_kernels.h"
+#include "inverse_kernels.h"
 #include "amr_kernels.h"
 #include "learned_kernels.h"
 #include "psi_kernels.h"
@@ -80,6 +81,8 @@ struct Level {
     bool tiled = false;              // MajorMinor level served by k_stencil_lds (else k_stencil_slab)
     double* psi = nullptr;           // SetPsi weight on this level's cells (null until set)
     ael::PalIdx* pidx = nullptr;     // palette row of every interior node, tile-blocked (null: no palette)
+    long dcap = 0, fcap = 0;         // capacities (entries) of d.D and f.D, kept across models
+    long nwcap = 0;                  // capacity of dwbase
     int npal = 0;
     ael::TileGrid tg{}, tg32{};      // its tile grids (chunk lengths) of the FP64 and FP32 launches
 };
@@ -120,15 +123,12 @@ struct alamo_el_s {
     // most kDirectMax unknowns and the FP64 inverse of its operator
     int direct_level = -1;
     int direct_M = 0;
-    double* direct_I = nullptr;     // unit vectors of the assembly
-    double* direct_A = nullptr;     // operator, destroyed by the inversion
+    double* direct_W = nullptr;     // work matrix of the inversion (M x M)
     double* direct_Ainv = nullptr;  // inverse, column-major (null when singular)
-    double* direct_f = nullptr;     // blocked Gauss-Jordan: the panel's transform (M x ael::kBgjMaxB)
-    double* direct_A2 = nullptr;    // second copies of the operator and the inverse (the update
-    double* direct_X2 = nullptr;    //   of each panel reads one and writes the other)
-    int* direct_piv = nullptr;      // the panel's row map (M)
+    int* direct_perm = nullptr;     // row swapped with row k at pivot step k (M)
+    int* direct_map = nullptr;      // a block's row map, then the inverse's column sources (M)
     double* direct_part = nullptr;  // partial sums of the bottom product
-    ael::GjState* direct_st = nullptr;
+    ael::InvState* direct_st = nullptr;
     // FGMRES storage, allocated on demand
     double* kh = nullptr;          // Gram-Schmidt coefficients on the device
     double* kpart = nullptr;       // multi-dot partials
@@ -547,9 +547,7 @@ int assemble_dense(alamo_el_t h, hipStream_t s) {
         h->dense_I = h->dense_A = nullptr;
         ALLOC(h->dense_I, (long)M * M);
         ALLOC(h->dense_A, (long)M * M);
-        std::vector<double> I((size_t)M * M, 0.0);
-        for (int q = 0; q < M; ++q) I[(size_t)q * M + q] = 1.0;
-        HIPCHK(hipMemcpy(h->dense_I, I.data(), sizeof(double) * M * M, hipMemcpyHostToDevice));
+        ael::k_eye<<<nblk((long)M * M), kBlock, 0, s>>>(h->dense_I, M);
         h->dense_M = M;
     }
     dim3 grid(nblk(lb.L.N), M);
@@ -578,20 +576,19 @@ int direct_level_of(alamo_el_t h) {
 }
 
 void free_direct(alamo_el_t h) {
-    for (void* p : {(void*)h->direct_I, (void*)h->direct_A, (void*)h->direct_Ainv, (void*)h->direct_f,
-                    (void*)h->direct_part, (void*)h->direct_st, (void*)h->direct_A2,
-                    (void*)h->direct_X2, (void*)h->direct_piv})
+    for (void* p : {(void*)h->direct_W, (void*)h->direct_Ainv, (void*)h->direct_perm, (void*)h->direct_map,
+                    (void*)h->direct_part, (void*)h->direct_st})
         if (p) (void)hipFree(p);
-    h->direct_I = h->direct_A = h->direct_Ainv = h->direct_f = h->direct_part = nullptr;
-    h->direct_A2 = h->direct_X2 = nullptr;
-    h->direct_piv = nullptr;
+    h->direct_W = h->direct_Ainv = h->direct_part = nullptr;
+    h->direct_perm = h->direct_map = nullptr;
     h->direct_st = nullptr;
     h->direct_M = 0;
 }
 
 // Operator of the direct level (A e_q for every unit vector) and its FP64
-// inverse by Gauss-Jordan with partial pivoting on the device.  A singular
-// operator leaves direct_Ainv null (the direct bottom is then unavailable).
+// inverse by blocked Gauss-Jordan with partial pivoting on the device
+// (inverse_kernels.h).  A singular operator leaves direct_Ainv null (the
+// direct bottom is then unavailable).
 int assemble_direct(alamo_el_t h, hipStream_t s) {
     const int l = direct_level_of(h);
     h->direct_level = l;
@@ -603,55 +600,67 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
     const int M = (int)(3 * lb.L.N);
     if (h->direct_M != M || !h->direct_Ainv) {
         free_direct(h);
-        ALLOC(h->direct_I, (long)M * M);
-        ALLOC(h->direct_A, (long)M * M);
+        ALLOC(h->direct_W, (long)M * M);
         ALLOC(h->direct_Ainv, (long)M * M);
-        ALLOC(h->direct_f, (long)M * ael::kBgjMaxB);
-        ALLOC(h->direct_A2, (long)M * M);
-        ALLOC(h->direct_X2, (long)M * M);
-        ALLOC(h->direct_piv, M);
+        ALLOC(h->direct_perm, M);
+        ALLOC(h->direct_map, M);
         ALLOC(h->direct_part, (long)((M + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk) * M);
         ALLOC(h->direct_st, 1);
-        std::vector<double> I((size_t)M * M, 0.0);
-        for (int q = 0; q < M; ++q) I[(size_t)q * M + q] = 1.0;
-        HIPCHK(hipMemcpy(h->direct_I, I.data(), sizeof(double) * M * M, hipMemcpyHostToDevice));
         h->direct_M = M;
     }
+    // the unit vectors (in the buffer the inverse will take), the operator's columns into W
+    const long MM = (long)M * M;
+    ael::k_eye<<<nblk(MM), kBlock, 0, s>>>(h->direct_Ainv, M);
     dim3 grid(nblk(lb.L.N), M);
     switch (h->sym) {
-        case 0: ael::k_assemble_dense<0><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
-        case 1: ael::k_assemble_dense<1><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
-        default: ael::k_assemble_dense<2><<<grid, kBlock, 0, s>>>(h->direct_I, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_A); break;
+        case 0: ael::k_assemble_dense<0><<<grid, kBlock, 0, s>>>(h->direct_Ainv, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_W); break;
+        case 1: ael::k_assemble_dense<1><<<grid, kBlock, 0, s>>>(h->direct_Ainv, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_W); break;
+        default: ael::k_assemble_dense<2><<<grid, kBlock, 0, s>>>(h->direct_Ainv, lb.d.C, h->bct_dev, h->uniform, lb.L, h->direct_W); break;
     }
-    psi_post_columns(h, lb, h->direct_I, h->direct_A, M, s);
+    psi_post_columns(h, lb, h->direct_Ainv, h->direct_W, M, s);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h->direct_Ainv, h->direct_I, sizeof(double) * M * M, hipMemcpyDeviceToDevice, s));
-    const ael::GjState zero{0, 0, 0.0};
+    const ael::InvState zero{0};
     HIPCHK(hipMemcpyAsync(h->direct_st, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
-    // blocked Gauss-Jordan with partial pivoting: panels of bb columns, each
-    // factored in LDS and applied to the rest of [A | inv] in one pass
-    const int bb = std::max(1, std::min(ael::kBgjMaxB, (ael::kBgjLds / 8) / M));
-    double *ain = h->direct_A, *aout = h->direct_A2, *xin = h->direct_Ainv, *xout = h->direct_X2;
-    for (int c0 = 0; c0 < M; c0 += bb) {
-        const int b = std::min(bb, M - c0);
-        ael::k_bgj_panel<<<1, ael::kGjBlock, 0, s>>>(ain, M, c0, b, h->direct_f, h->direct_piv, h->direct_st);
-        const long nq = (long)(M - c0 - b) + M;
-        ael::k_bgj_update<<<dim3(nblk(M), (unsigned)((nq + ael::kBgjCols - 1) / ael::kBgjCols)), kBlock, 0, s>>>(
-            ain, xin, aout, xout, M, c0, b, h->direct_f, h->direct_piv);
-        std::swap(ain, aout);
-        std::swap(xin, xout);
+    // blocks of kInvBlock columns: panels of kInvPanel pivot steps, each
+    // updating the rest of its block, then the block's update of every other
+    // column, out of place (W and the inverse's buffer alternate)
+    double *W = h->direct_W, *W2 = h->direct_Ainv;
+    const unsigned gt = (unsigned)((M + ael::kInvTile - 1) / ael::kInvTile);
+    for (int k0 = 0; k0 < M; k0 += ael::kInvBlock) {
+        const int nbk = std::min(ael::kInvBlock, M - k0), b1 = k0 + nbk;
+        for (int c0 = k0; c0 < b1; c0 += ael::kInvPanel) {
+            const int nb = std::min(ael::kInvPanel, b1 - c0);
+            ael::k_inv_panel<<<1, ael::kInvThreads, 0, s>>>(W, M, c0, nb, h->direct_perm, h->direct_st);
+            if (nbk > nb)
+                ael::k_inv_block<<<(unsigned)(nbk - nb), kBlock, 0, s>>>(W, M, k0, b1, c0, nb, h->direct_perm);
+        }
+        if (M > nbk) {
+            ael::k_inv_rowmap<<<1, kBlock, 0, s>>>(h->direct_perm, M, k0, nbk, h->direct_map);
+            ael::k_inv_gemm<<<dim3(gt, gt), kBlock, 0, s>>>(W, W2, M, k0, nbk, h->direct_map);
+            std::swap(W, W2);
+        }
     }
     HIPCHK(hipGetLastError());
-    // the inverse is in xin, the operator's remains in ain: keep the buffers' roles by their contents
-    if (xin != h->direct_Ainv) std::swap(h->direct_Ainv, h->direct_X2);
-    if (ain != h->direct_A) std::swap(h->direct_A, h->direct_A2);
-    ael::GjState st{};
+    ael::InvState st{};
+    std::vector<int> perm(M), src(M);
     HIPCHK(hipMemcpyAsync(&st, h->direct_st, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(perm.data(), h->direct_perm, sizeof(int) * M, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (st.singular) {
         (void)hipFree(h->direct_Ainv);
         h->direct_Ainv = nullptr;
+        return 0;
     }
+    // W holds the inverse with its columns permuted: undo the swaps from the
+    // last step to the first (column c of the inverse is column src[c] of W)
+    for (int c = 0; c < M; ++c) src[c] = c;
+    for (int k = M - 1; k >= 0; --k) std::swap(src[k], src[perm[k]]);
+    HIPCHK(hipMemcpyAsync(h->direct_map, src.data(), sizeof(int) * M, hipMemcpyHostToDevice, s));
+    ael::k_inv_unpermute<<<nblk(MM), kBlock, 0, s>>>(W, M, h->direct_map, W2);
+    HIPCHK(hipGetLastError());
+    h->direct_Ainv = W2;
+    h->direct_W = W;
+    HIPCHK(hipStreamSynchronize(s));  // src lives on the host stack
     return 0;
 }
 
@@ -674,27 +683,42 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
     if (e && std::strcmp(e, "0") == 0) return 0;
     const long N = l.L.N;
     const long limit = std::min<long>(ael::kPalMax, N / 16);
-    unsigned long long* hs = nullptr;
-    HIPCHK(hipMalloc((void**)&hs, sizeof(unsigned long long) * N));
+    // hashes in the level's tmp vector (3 N doubles), the runs' heads in its
+    // rescor vector: the host reads the heads only (a polycrystal's rows cross
+    // a few grains each), not a hash per node
+    unsigned long long* hs = reinterpret_cast<unsigned long long*>(l.d.tmp);
+    unsigned long long* heads = reinterpret_cast<unsigned long long*>(l.d.rescor);
+    unsigned int* cnt_d = reinterpret_cast<unsigned int*>(h->red_out);
+    const int cap = (int)std::min<long>(3 * N / 2, 1L << 20);
     ael::k_c_hash<<<nblk(N), kBlock, 0, s>>>(l.d.C, N, hs);
-    std::vector<unsigned long long> hh(N);
+    unsigned int cnt = 0;
+    HIPCHK(hipMemcpyAsync(cnt_d, &cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
+    ael::k_run_heads<<<nblk(N), kBlock, 0, s>>>(hs, N, cap, heads, cnt_d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&cnt, cnt_d, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     int rc = 0;
-    if (hipMemcpyAsync(hh.data(), hs, sizeof(unsigned long long) * N, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-        (void)hipFree(hs);
-        return fail(ALAMO_EL_EHIP, "palette hash copy failed");
-    }
     std::unordered_map<unsigned long long, long> first;  // hash -> first node
-    unsigned long long prev = 0;
-    for (long id = 0; id < N && (long)first.size() <= limit; ++id) {
-        if (id > 0 && hh[id] == prev) continue;  // runs along x inside a grain: one lookup per run
-        prev = hh[id];
-        first.emplace(prev, id);
+    if ((long)cnt <= cap) {
+        std::vector<unsigned long long> hd(2 * (size_t)cnt);
+        HIPCHK(hipMemcpyAsync(hd.data(), heads, sizeof(unsigned long long) * 2 * cnt, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (unsigned int q = 0; q < cnt && (long)first.size() <= limit; ++q) {
+            auto it = first.emplace(hd[2 * q], (long)hd[2 * q + 1]).first;
+            it->second = std::min<long>(it->second, (long)hd[2 * q + 1]);
+        }
+    } else {  // more runs than the heads buffer holds: read every hash
+        std::vector<unsigned long long> hh(N);
+        HIPCHK(hipMemcpyAsync(hh.data(), hs, sizeof(unsigned long long) * N, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        unsigned long long prev = 0;
+        for (long id = 0; id < N && (long)first.size() <= limit; ++id) {
+            if (id > 0 && hh[id] == prev) continue;
+            prev = hh[id];
+            first.emplace(prev, id);
+        }
     }
-    if ((long)first.size() > limit) {
-        (void)hipFree(hs);
-        return 0;
-    }
+    if ((long)first.size() > limit) return 0;
     std::vector<std::pair<unsigned long long, long>> rows(first.begin(), first.end());
     std::sort(rows.begin(), rows.end());
     const int npal = (int)rows.size();
@@ -708,7 +732,7 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
     long* rep_d = nullptr;
     int* bad_d = nullptr;
     auto done = [&](int r) {
-        for (void* p : {(void*)hs, (void*)ph_d, (void*)rep_d, (void*)bad_d})
+        for (void* p : {(void*)ph_d, (void*)rep_d, (void*)bad_d})
             if (p) (void)hipFree(p);
         if (r) {  // an error leaves the level on its C copy, with nothing half built
             if (l.d.pal) (void)hipFree(l.d.pal);
@@ -760,24 +784,26 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     const unsigned nib = l.tiled ? (unsigned)ael::tile_planes(l.L, l.tg)
                                  : ael::slab_blocks(l.L, kBlock) - ael::slab_boundary_blocks(l.L, kBlock);
     const long nw = (long)nib * wpb;
-    if (l.dwbase) (void)hipFree(l.dwbase);
-    for (void* p : {(void*)l.d.D, (void*)l.f.D, (void*)l.d.Ct, (void*)l.d.Cb, (void*)l.f.Ct, (void*)l.f.Cb,
-                    (void*)l.d.pal, (void*)l.f.pal, (void*)l.pidx})
+    // The tile copies and dwbase depend on the grid only and stay allocated
+    // from one model to the next (their contents are rebuilt here and, for the
+    // FP32 copies, by prepare32); the compact D arrays grow when needed.
+    for (void* p : {(void*)l.d.pal, (void*)l.f.pal, (void*)l.pidx})
         if (p) (void)hipFree(p);
     l.d.pal = nullptr;
     l.f.pal = nullptr;
     l.pidx = nullptr;
     l.npal = 0;
-    l.dwbase = nullptr;
-    l.d.D = nullptr;
-    l.f.D = nullptr;
-    l.d.Ct = l.d.Cb = nullptr;
-    l.f.Ct = l.f.Cb = nullptr;
     l.nflag = 0;
+    if (!l.tiled) {
+        for (void* p : {(void*)l.d.Ct, (void*)l.d.Cb, (void*)l.f.Ct, (void*)l.f.Cb})
+            if (p) (void)hipFree(p);
+        l.d.Ct = l.d.Cb = nullptr;
+        l.f.Ct = l.f.Cb = nullptr;
+    }
     if (l.tiled) {
         const long nb = ael::boundary_count(l.L);
-        HIPCHK(hipMalloc((void**)&l.d.Ct, sizeof(double) * ael::TileC<TileT>::elems(l.L, l.tg)));
-        HIPCHK(hipMalloc((void**)&l.d.Cb, sizeof(double) * h->nc * nb));
+        if (!l.d.Ct) HIPCHK(hipMalloc((void**)&l.d.Ct, sizeof(double) * ael::TileC<TileT>::elems(l.L, l.tg)));
+        if (!l.d.Cb) HIPCHK(hipMalloc((void**)&l.d.Cb, sizeof(double) * h->nc * nb));
         ael::k_pack_tile_c<TileT, double><<<(unsigned)ael::tile_planes(l.L, l.tg), TileT::NT, 0, s>>>(l.d.C, l.L, l.tg,
                                                                                                    l.d.Ct);
         ael::k_pack_bnd_c<double><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.d.Cb);
@@ -786,7 +812,13 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
         if (rc) return rc;
     }
     if (nw == 0) return 0;
-    HIPCHK(hipMalloc((void**)&l.dwbase, sizeof(int) * nw));
+    if (!l.dwbase || l.nwcap < nw) {
+        if (l.dwbase) (void)hipFree(l.dwbase);
+        l.dwbase = nullptr;
+        l.nwcap = 0;
+        HIPCHK(hipMalloc((void**)&l.dwbase, sizeof(int) * nw));
+        l.nwcap = nw;
+    }
     if (l.tiled)
         ael::k_gradc_count_tiles<TileT><<<nib, kBlock, 0, s>>>(l.gmask, l.L, l.tg, l.dwbase);
     else
@@ -804,7 +836,13 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     HIPCHK(hipMemcpyAsync(l.dwbase, cnt.data(), sizeof(int) * nw, hipMemcpyHostToDevice, s));
     l.nflag = (int)tot;
     if (tot > 0) {
-        HIPCHK(hipMalloc((void**)&l.d.D, sizeof(double) * 18 * tot));
+        if (!l.d.D || l.dcap < tot) {
+            if (l.d.D) (void)hipFree(l.d.D);
+            l.d.D = nullptr;
+            l.dcap = 0;
+            HIPCHK(hipMalloc((void**)&l.d.D, sizeof(double) * 18 * tot));
+            l.dcap = tot;
+        }
         if (l.tiled)
             ael::k_gradc_fill_tiles<TileT><<<nib, kBlock, 0, s>>>(l.d.C, l.gmask, l.L, l.tg, l.dwbase, l.nflag, l.d.D);
         else
@@ -864,11 +902,8 @@ int prepare32(alamo_el_t h, hipStream_t s) {
         ael::k_cast<double, float><<<nblk(3 * N), kBlock, 0, s>>>(l.d.diag, l.f.diag, 3 * N);
         if (l.d.Ct) {
             const long nb = ael::boundary_count(l.L);
-            if (l.f.Ct) (void)hipFree(l.f.Ct);
-            if (l.f.Cb) (void)hipFree(l.f.Cb);
-            l.f.Ct = l.f.Cb = nullptr;
-            HIPCHK(hipMalloc((void**)&l.f.Ct, sizeof(float) * ael::TileC<TileT>::elems(l.L, l.tg)));
-            HIPCHK(hipMalloc((void**)&l.f.Cb, sizeof(float) * h->nc * nb));
+            if (!l.f.Ct) HIPCHK(hipMalloc((void**)&l.f.Ct, sizeof(float) * ael::TileC<TileT>::elems(l.L, l.tg)));
+            if (!l.f.Cb) HIPCHK(hipMalloc((void**)&l.f.Cb, sizeof(float) * h->nc * nb));
             ael::k_pack_tile_c<TileT, float><<<(unsigned)ael::tile_planes(l.L, l.tg), TileT::NT, 0, s>>>(l.d.C, l.L,
                                                                                                      l.tg, l.f.Ct);
             ael::k_pack_bnd_c<float><<<nblk(nb), kBlock, 0, s>>>(l.d.C, l.L, h->nc, l.f.Cb);
@@ -880,9 +915,14 @@ int prepare32(alamo_el_t h, hipStream_t s) {
             HIPCHK(hipMalloc((void**)&l.f.pal, sizeof(float) * np));
             ael::k_cast<double, float><<<nblk(np), kBlock, 0, s>>>(l.d.pal, l.f.pal, np);
         }
-        if (l.d.D) {
-            if (l.f.D) (void)hipFree(l.f.D);
-            HIPCHK(hipMalloc((void**)&l.f.D, sizeof(float) * 18 * (long)l.nflag));
+        if (l.d.D && l.nflag > 0) {
+            if (!l.f.D || l.fcap < l.nflag) {
+                if (l.f.D) (void)hipFree(l.f.D);
+                l.f.D = nullptr;
+                l.fcap = 0;
+                HIPCHK(hipMalloc((void**)&l.f.D, sizeof(float) * 18 * (long)l.nflag));
+                l.fcap = l.nflag;
+            }
             ael::k_cast<double, float><<<nblk(18L * l.nflag), kBlock, 0, s>>>(l.d.D, l.f.D, 18L * l.nflag);
         }
     }
@@ -1950,6 +1990,44 @@ int alamo_amr_cluster(const unsigned char* tags_host, const int ncell[3], const 
             boxes[6 * q + 3 + a] = bs[q].hi[a] + 1;
         }
     *nboxes = (int)bs.size();
+    return 0;
+}
+
+int alamo_amr_reflux(alamo_el_t patch, const double* fine_res, double* parent_res, hipStream_t s) {
+    if (!patch || !fine_res || !parent_res) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (!patch->amr_parent) return fail(ALAMO_EL_EINVAL, "reflux needs a patch (alamo_amr_patch_create)");
+    Level& lv = patch->lv[0];
+    const alamo_el_t par = patch->amr_parent;
+    const ael::Patch& P = patch->amr;
+    // the patch residual with its ghost rows zeroed, in the patch's own scratch
+    launch_copy<double>(lv.d.rescor, fine_res, 3 * lv.L.N, s);
+    ael::k_patch_ghost<<<nblk(lv.L.N), kBlock, 0, s>>>(lv.d.rescor, nullptr, lv.L, P, 0);
+    const long m = (long)(P.chi[0] - P.clo[0] + 1) * (P.chi[1] - P.clo[1] + 1) * (P.chi[2] - P.clo[2] + 1);
+    ael::k_patch_reflux<<<nblk(m), kBlock, 0, s>>>(lv.d.rescor, lv.L, P, parent_res, par->lv[0].L, par->amr.ox,
+                                                    par->amr.oy, par->amr.oz, par->amr_dom[0], par->amr_dom[1],
+                                                    par->amr_dom[2]);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_el_box_copy(double* dst, const int dst_n[3], const int dst_origin[3], const double* src, const int src_n[3],
+                      const int src_origin[3], const int lo[3], const int hi[3], int ncomp, hipStream_t s) {
+    if (!dst || !src || !dst_n || !dst_origin || !src_n || !src_origin || !lo || !hi || ncomp < 1)
+        return fail(ALAMO_EL_EINVAL, "null argument");
+    int b[3];
+    for (int a = 0; a < 3; ++a) {
+        if (dst_n[a] < 1 || src_n[a] < 1 || hi[a] < lo[a]) return fail(ALAMO_EL_EINVAL, "bad extents");
+        if (lo[a] < dst_origin[a] || hi[a] > dst_origin[a] + dst_n[a] - 1 || lo[a] < src_origin[a] ||
+            hi[a] > src_origin[a] + src_n[a] - 1)
+            return fail(ALAMO_EL_EINVAL, "the box must lie inside both arrays");
+        b[a] = hi[a] - lo[a] + 1;
+    }
+    const long n = (long)b[0] * b[1] * b[2];
+    ael::k_box_copy<<<nblk(n), kBlock, 0, s>>>(dst, dst_n[0], dst_n[1], dst_n[2], dst_origin[0], dst_origin[1],
+                                                dst_origin[2], src, src_n[0], src_n[1], src_n[2], src_origin[0],
+                                                src_origin[1], src_origin[2], lo[0], lo[1], lo[2], b[0], b[1], b[2],
+                                                ncomp);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

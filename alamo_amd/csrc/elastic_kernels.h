@@ -1021,126 +1021,6 @@ __global__ void __launch_bounds__(kBlock) k_dense_mv_final(const double* __restr
     y[r] = (T)acc;
 }
 
-// Gauss-Jordan inversion with partial pivoting on the device, one pivot
-// column c per step (three launches): the inverse of an M x M column-major
-// matrix a is built in inv (set to the identity first); a is destroyed.
-struct GjState {
-    int piv;      // pivot row of the current step
-    int singular; // set when a pivot is zero
-    double d;     // pivot value
-};
-constexpr int kGjBlock = 1024;
-// Blocked Gauss-Jordan inversion: panels of bb columns.  k_bgj_panel runs
-// the bb pivoted steps of in-place Gauss-Jordan on the M x bb panel in LDS
-// (column j becomes column j of the panel's transform once it is eliminated),
-// and writes the transform T (M x bb, column-major) and the row map of the
-// panel's swaps (map[r]: the row whose value lands on row r).  Every other
-// column y of [a | inv] then takes, in k_bgj_update, the swaps and
-// y <- T y_P + y with the panel rows zeroed: one pass over the matrix per
-// panel instead of one per pivot.
-constexpr int kBgjMaxB = 16;
-constexpr int kBgjLds = 152 * 1024;  // bytes of panel in LDS: M * bb * 8 at most
-__global__ void __launch_bounds__(kGjBlock) k_bgj_panel(const double* __restrict__ a, int M, int c0, int bb,
-                                                        double* __restrict__ T, int* __restrict__ map,
-                                                        GjState* __restrict__ st) {
-    __shared__ double sw[kBgjLds / 8];  // sw[j * M + r]
-    __shared__ double prow[kBgjMaxB];
-    __shared__ int pvs[kBgjMaxB];
-    __shared__ double wv[kGjBlock / 64];
-    __shared__ int wi[kGjBlock / 64];
-    __shared__ int piv_s;
-    const int t = (int)threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int j = 0; j < bb; ++j)
-        for (int r = t; r < M; r += kGjBlock) sw[j * M + r] = a[(long)(c0 + j) * M + r];
-    __syncthreads();
-    for (int j = 0; j < bb; ++j) {
-        const int c = c0 + j;
-        // pivot: largest |value| in column j at rows >= c, first on ties
-        double best = -1.0;
-        int idx = c;
-        for (int r = c + t; r < M; r += kGjBlock) {
-            const double v = fabs(sw[j * M + r]);
-            if (v > best) { best = v; idx = r; }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(idx, off);
-            if (ob > best || (ob == best && oi < idx)) { best = ob; idx = oi; }
-        }
-        if (lane == 0) { wv[wave] = best; wi[wave] = idx; }
-        __syncthreads();
-        if (t == 0) {
-            double b0 = wv[0];
-            int i0 = wi[0];
-            for (int w = 1; w < kGjBlock / 64; ++w)
-                if (wv[w] > b0 || (wv[w] == b0 && wi[w] < i0)) { b0 = wv[w]; i0 = wi[w]; }
-            piv_s = i0;
-            pvs[j] = i0;
-            if (!(b0 > 0.0)) st->singular = 1;
-        }
-        __syncthreads();
-        const int p = piv_s;
-        if (t < bb && p != c) {  // rows c and p exchanged in every panel column
-            const double v = sw[t * M + c];
-            sw[t * M + c] = sw[t * M + p];
-            sw[t * M + p] = v;
-        }
-        __syncthreads();
-        const double d = sw[j * M + c];
-        if (t < bb) prow[t] = t == j ? 1.0 / d : sw[t * M + c] / d;
-        __syncthreads();
-        for (int r = t; r < M; r += kGjBlock) {
-            if (r == c) {
-                for (int q = 0; q < bb; ++q) sw[q * M + r] = prow[q];
-            } else {
-                const double f = sw[j * M + r];
-                for (int q = 0; q < bb; ++q)
-                    if (q != j) sw[q * M + r] -= f * prow[q];
-                sw[j * M + r] = -f / d;
-            }
-        }
-        __syncthreads();
-    }
-    for (int j = 0; j < bb; ++j)
-        for (int r = t; r < M; r += kGjBlock) T[(long)j * M + r] = sw[j * M + r];
-    for (int r = t; r < M; r += kGjBlock) {  // undo the swaps from the last: the source of row r
-        int src = r;
-        for (int j = bb - 1; j >= 0; --j) {
-            if (src == c0 + j) src = pvs[j];
-            else if (src == pvs[j]) src = c0 + j;
-        }
-        map[r] = src;
-    }
-}
-// Columns of [a trailing columns c0 + bb .. M - 1 | inv] from the in to the
-// out buffers, kBgjCols per thread (row r's transform row is loaded once for
-// them): the panel's swaps (map), then T y_P plus the other rows.
-constexpr int kBgjCols = 8;
-__global__ void __launch_bounds__(kBlock) k_bgj_update(const double* __restrict__ ain, const double* __restrict__ xin,
-                                                       double* __restrict__ aout, double* __restrict__ xout, int M,
-                                                       int c0, int bb, const double* __restrict__ T,
-                                                       const int* __restrict__ map) {
-    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const long na = M - c0 - bb, nq = na + M;
-    if (r >= M) return;
-    double tr[kBgjMaxB];
-#pragma unroll
-    for (int j = 0; j < kBgjMaxB; ++j) tr[j] = j < bb ? T[(long)j * M + r] : 0.0;
-    const int sr = map[r];
-    const bool panel_row = r >= c0 && r < c0 + bb;
-    for (int u = 0; u < kBgjCols; ++u) {
-        const long q = (long)blockIdx.y * kBgjCols + u;
-        if (q >= nq) break;
-        const double* yin = q < na ? ain + (c0 + bb + q) * M : xin + (q - na) * M;
-        double* yout = q < na ? aout + (c0 + bb + q) * M : xout + (q - na) * M;
-        double acc = panel_row ? 0.0 : yin[sr];
-#pragma unroll
-        for (int j = 0; j < kBgjMaxB; ++j)
-            if (j < bb) acc += tr[j] * yin[map[c0 + j]];
-        yout[r] = acc;
-    }
-}
-
 // Block-wide max (op 0) or sum (op 1) for kBlock threads; all threads get it.
 __device__ __forceinline__ double blk_reduce256(double v, int op, double* red) {
 #pragma unroll

@@ -105,6 +105,22 @@ __global__ void __launch_bounds__(kBlock) k_c_hash(const double* __restrict__ C,
     }
     hs[id] = h;
 }
+// Heads of the runs of equal hashes along the node order (a node whose hash
+// differs from the previous node's): their (hash, node) pairs, compacted in any
+// order; *count receives the number of heads, pairs beyond cap are dropped.
+__global__ void __launch_bounds__(kBlock) k_run_heads(const unsigned long long* __restrict__ hs, long N, int cap,
+                                                      unsigned long long* __restrict__ heads,
+                                                      unsigned int* __restrict__ count) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= N) return;
+    const unsigned long long h = hs[id];
+    if (id > 0 && hs[id - 1] == h) return;
+    const unsigned int q = atomicAdd(count, 1u);
+    if ((int)q < cap) {
+        heads[2 * (long)q] = h;
+        heads[2 * (long)q + 1] = (unsigned long long)id;
+    }
+}
 // Row q of the table from node rep[q] (rows padded with zeros).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pal_gather(const double* __restrict__ C, long N, const long* __restrict__ rep,

@@ -124,6 +124,9 @@ def lib():
         L.alamo_amr_patch_info.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.alamo_amr_fillpatch.argtypes = [vp, vp, vp, vp, vp, vp]
         L.alamo_el_tag_cells.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, vp]
+        L.alamo_amr_reflux.argtypes = [vp, vp, vp, vp]
+        i3 = ctypes.POINTER(ctypes.c_int)
+        L.alamo_el_box_copy.argtypes = [vp, i3, i3, vp, i3, i3, i3, i3, ctypes.c_int, vp]
         L.alamo_amr_cluster.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]
@@ -484,6 +487,17 @@ class ElasticOperator:
                 boxes.append((tuple(clo), tuple(chi)))
         return boxes
 
+    def reflux(self, fine_res, parent_res):
+        """Operator::reflux of this patch into its parent (alamo_amr_reflux):
+        parent_res (a field of the parent) at the covered and coarse-fine
+        nodes becomes the full weighting of fine_res (ghost rows read as 0)."""
+        if self.parent is None:
+            raise AlamoError("reflux: this operator is not a patch")
+        self._check_field(fine_res, self.level_shape(0), "fine_res")
+        self._check_field(parent_res, self.parent.level_shape(0), "parent_res")
+        _check(lib().alamo_amr_reflux(self._h, _ptr(fine_res), _ptr(parent_res), _stream()), "alamo_amr_reflux")
+        return parent_res
+
     def fill_patch(self, parent_u, old=None, old_u=None):
         """This patch's field after a regrid (alamo_amr_fillpatch;
         Integrator::FillPatch / FillCoarsePatch, src/Integrator/Integrator.cpp:
@@ -696,3 +710,21 @@ class Newton(Linear):
         res = Linear.solve(self, du, rhs, tol_rel, tol_abs)
         u += du
         return res
+
+
+def box_copy(dst, dst_origin, src, src_origin, lo, hi):
+    """alamo_el_box_copy: dst[:, nodes lo..hi] = src[:, same nodes], the
+    arrays (ncomp, nz, ny, nx) float64 on the GPU with their node 0 at the
+    global (x, y, z) origins given; lo, hi global node indices (x, y, z),
+    inclusive.  Gathers the FABs of a multi-box field into a level's single box,
+    or scatters them back."""
+    if dst.shape[0] != src.shape[0]:
+        raise AlamoError("box_copy: the arrays must hold the same number of components")
+    for t in (dst, src):
+        if t.dim() != 4 or t.dtype != _torch().float64 or not t.is_contiguous() or not t.is_cuda:
+            raise AlamoError("box_copy: expected contiguous float64 CUDA tensors (ncomp, nz, ny, nx)")
+    c3 = lambda v: (ctypes.c_int * 3)(*[int(a) for a in v])
+    _check(lib().alamo_el_box_copy(ctypes.c_void_p(dst.data_ptr()), c3(dst.shape[1:][::-1]), c3(dst_origin),
+                                   ctypes.c_void_p(src.data_ptr()), c3(src.shape[1:][::-1]), c3(src_origin), c3(lo),
+                                   c3(hi), int(dst.shape[0]), _stream()), "alamo_el_box_copy")
+    return dst

@@ -326,8 +326,24 @@ int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost
  * may have another parent (the regridded one). */
 int alamo_amr_fillpatch(alamo_el_t dst, const double* parent_u, alamo_el_t src, const double* src_u, double* dst_u,
                         hipStream_t stream);
+/* Operator::reflux (src/Operator/Operator.cpp:526-633) of one patch into its
+ * parent: parent_res (a field of the parent handle) at the parent nodes
+ * clo..chi becomes the nodal full weighting of fine_res (a field of the patch
+ * handle; its ghost-layer values are taken as 0), with the face, edge and
+ * corner rules of the MG restriction on domain faces.  fine_res is not
+ * modified. */
+int alamo_amr_reflux(alamo_el_t patch, const double* fine_res, double* parent_res, hipStream_t stream);
+/* Gather or scatter one box of a multi-box field (an amrex::MultiFab's FABs on
+ * one AMR level) to or from the single-box layout of a level: copies the nodes
+ * [lo, hi] (global node indices, inclusive) of ncomp planes from src (src_n
+ * nodes per axis, its node 0 at global src_origin) into dst (dst_n, dst_origin).
+ * The box must lie inside both arrays. */
+int alamo_el_box_copy(double* dst, const int dst_n[3], const int dst_origin[3], const double* src, const int src_n[3],
+                      const int src_origin[3], const int lo[3], const int hi[3], int ncomp, hipStream_t stream);
 /* Solve the composite system over levels[0] (the base handle) and its nested
- * patches levels[1..nlev-1]: x[l], b[l] are fields of each handle.  One
+ * patches levels[1..nlev-1], each patch's parent listed before it (a level may
+ * hold several patches that share no nodes, as alamo_amr_cluster makes them):
+ * x[l], b[l] are fields of each handle.  One
  * iteration pre-smooths every patch (nu1 Fsmooth calls), refluxes the residual
  * into its parent, runs one cycle (opts->cycle, opts->precision) on the base
  * level, interpolates the corrections up and post-smooths (nu2); then the

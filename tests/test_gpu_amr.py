@@ -450,3 +450,27 @@ def test_patches_of_one_level_must_not_share_nodes():
     xs = [torch.zeros((3,) + sh, dtype=torch.float64, device="cuda") for sh in shape]
     with pytest.raises(el.AlamoError, match="share nodes"):
         el.amr_solve([base, a, b], xs, [x.clone() for x in xs], 1e-8, 0.0)
+
+
+def test_reflux_entry_point_matches_oracle():
+    """alamo_amr_reflux (Operator::reflux, src/Operator/Operator.cpp:526-633)
+    against the oracle on random residuals: a patch inside the domain and one
+    on two domain faces; the caller's fine residual is not modified."""
+    rng = np.random.default_rng(17)
+    base = el.ElasticOperator(16, 1 / 16, M.ISOTROPIC)
+    for clo, chi in (((3, 4, 2), (11, 12, 9)), ((0, 5, 6), (7, 12, 16))):
+        p = base.refine(clo, chi)
+        rf = rng.standard_normal((3,) + p.level_shape(0))
+        rc = rng.standard_normal((3,) + base.level_shape(0))
+        rf_d, rc_d = torch.from_numpy(rf).cuda(), torch.from_numpy(rc.copy()).cuda()
+        p.reflux(rf_d, rc_d)
+        torch.cuda.synchronize()
+        lvp = AO.Level(np.ones((2,) + p.level_shape(0)), M.ISOTROPIC, p._bc, p.dx[0], origin=p.origin, clo=p.clo,
+                       chi=p.chi, gl=p.ghost_lo, gh=p.ghost_hi, dom=p.dom)
+        lvb = AO.Level(np.ones((2,) + base.level_shape(0)), M.ISOTROPIC, base._bc, base.dx[0], dom=base.dom)
+        rz = rf.copy()
+        rz[:, AO.ghost_mask(rz.shape[1:], p.ghost_lo, p.ghost_hi)] = 0.0
+        want = rc.copy()
+        AO.reflux_loops(rz, lvp, want, lvb)
+        assert np.abs(rc_d.cpu().numpy() - want).max() <= 1e-14 * np.abs(want).max()
+        assert np.array_equal(rf_d.cpu().numpy(), rf)

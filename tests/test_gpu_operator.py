@@ -433,3 +433,48 @@ def test_psi_on_a_tiled_palette_level_matches_oracle():
     xd = dev(u)
     op.Fsmooth(xd, dev(b), nsmooth=1)
     assert relerr(host(xd), O.fsmooth_psi(u, b, C, sym, bc, dx, psi, 1e-4, nsmooth=1)) < REL
+
+
+def test_multibox_field_gathered_into_the_single_box_layout():
+    """A field held as several nodal boxes (an amrex::MultiFab's FABs on one
+    level: neighbouring boxes share their face nodes) is gathered into the
+    level's single-box layout by alamo_el_box_copy and scattered back, bit for
+    bit; the solve of the gathered right-hand side is the solve of the
+    original."""
+    el = _el()
+    from alamo_amd import problems as P
+    rng = np.random.default_rng(23)
+    p = P.polycrystal(32, ngrains=8, seed=2)
+    op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
+    op.SetBC(p["bc"])
+    op.SetModel(dev(p["C"]))
+    shape = op.level_shape(0)
+    full = dev(rng.standard_normal((3,) + shape))
+    # four FABs: x split at node 16, y at node 12 (cells 0..15 | 16..31, 0..11 | 12..31)
+    boxes = [((0, 0, 0), (16, 12, 32)), ((16, 0, 0), (32, 12, 32)), ((0, 12, 0), (16, 32, 32)),
+             ((16, 12, 0), (32, 32, 32))]
+    fabs = []
+    for lo, hi in boxes:
+        fab = torch.empty((3, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1), dtype=torch.float64,
+                          device="cuda")
+        el.box_copy(fab, lo, full, (0, 0, 0), lo, hi)                  # scatter
+        fabs.append(fab)
+    gathered = torch.full_like(full, float("nan"))
+    for (lo, hi), fab in zip(boxes, fabs):
+        el.box_copy(gathered, (0, 0, 0), fab, lo, lo, hi)              # gather
+    assert torch.equal(gathered, full)
+    for (lo, hi), fab in zip(boxes, fabs):
+        assert torch.equal(fab, full[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+    b = op.bc_init(torch.zeros_like(full))
+    x1, x2 = torch.zeros_like(b), torch.zeros_like(b)
+    bg = torch.full_like(b, float("nan"))
+    for lo, hi in boxes:
+        fab = torch.empty((3, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1), dtype=torch.float64,
+                          device="cuda")
+        el.box_copy(fab, lo, b, (0, 0, 0), lo, hi)
+        el.box_copy(bg, (0, 0, 0), fab, lo, lo, hi)
+    el.Linear(op).solve(x1, b, 1e-10, 0.0)
+    el.Linear(op).solve(x2, bg, 1e-10, 0.0)
+    assert torch.equal(x1, x2)
+    with pytest.raises(el.AlamoError):
+        el.box_copy(gathered, (0, 0, 0), fabs[0], (0, 0, 0), (0, 0, 0), (17, 12, 32))   # beyond the FAB

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1416,6 +1417,66 @@ int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double*
     }
     launch_newton_dw(h, u, Cs, L.N, 1, F0, 0, L.nz, h->gu9, h->dw9, s);
     ael::k_newton_rhs<<<nblk(L.N), kBlock, 0, s>>>(u, b, h->gu9, h->dw9, 0, L.nz, h->bct_dev, L, 0, L.nz, rhs);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// --- The Newton right-hand side that allocates nothing --------------------
+// Scratch of alamo_el_newton_rhs_noalloc: DW of a slab of planes.  One copy
+// per GPU, held until the process ends; host code never passes the array's
+// name to a kernel, only the device address hipGetSymbolAddress returns.
+constexpr long kNoAllocScratchBytes = 32l << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch, looked up once per device.
+int scratch_of_current_device(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ALAMO_EL_EINVAL, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(ALAMO_EL_EHIP, "hipGetSymbolAddress returned no address for the scratch");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+int alamo_el_newton_rhs_noalloc(alamo_el_t h, const double* C, int layout, const double* F0, const double* u,
+                                const double* b, double* rhs, hipStream_t s) {
+    if (!h || !C || !u || !b || !rhs) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (layout != ALAMO_EL_SOA && layout != ALAMO_EL_AOS) return fail(ALAMO_EL_EINVAL, "bad layout");
+    const Lvl L = h->lv[0].L;
+    const long P = (long)L.nx * L.ny;
+    // planes of DW (9 doubles a node) the scratch holds; a slab of rows needs
+    // its own planes and one more on each side
+    const long cap = (kNoAllocScratchBytes / (long)sizeof(double)) / (9 * P);
+    if (cap < 3)
+        return fail(ALAMO_EL_EINVAL, "alamo_el_newton_rhs_noalloc: a plane of " + std::to_string(P) +
+                                         " nodes exceeds the scratch (at most " +
+                                         std::to_string(kNoAllocScratchBytes / (long)sizeof(double) / 27) +
+                                         " nodes per x-y plane)");
+    double* dw = nullptr;
+    int rc = scratch_of_current_device(&dw);
+    if (rc) return rc;
+    const long cs = layout == ALAMO_EL_SOA ? L.N : 1, cn = layout == ALAMO_EL_SOA ? 1 : h->nc;
+    const int rows = (int)std::min<long>(cap - 2, L.nz);
+    for (int k0 = 0; k0 < L.nz; k0 += rows) {
+        const int k1 = std::min(L.nz, k0 + rows);
+        const int kb = std::max(0, k0 - 1), kt = std::min(L.nz - 1, k1);
+        const int np = kt - kb + 1;
+        launch_newton_dw(h, u, C, cs, cn, F0, kb, np, nullptr, dw, s);
+        ael::k_newton_rhs<<<nblk((long)(k1 - k0) * P), kBlock, 0, s>>>(u, b, nullptr, dw, kb, np, h->bct_dev, L, k0,
+                                                                        k1, rhs);
+    }
     HIPCHK(hipGetLastError());
     return 0;
 }

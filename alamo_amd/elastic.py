@@ -62,7 +62,7 @@ EXPORTS = [
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
     "alamo_el_tag_bbox", "alamo_amr_fillpatch", "alamo_el_set_psi", "alamo_el_palette_size",
-    "alamo_el_gs_smooth", "alamo_el_tag_cells", "alamo_amr_cluster",
+    "alamo_el_newton_rhs_noalloc", "alamo_el_gs_smooth", "alamo_el_tag_cells", "alamo_amr_cluster",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -100,6 +100,7 @@ def lib():
         L.alamo_el_interp_add.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_bc_init.argtypes = [vp, vp, ctypes.c_int, vp]
         L.alamo_el_newton_rhs.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.alamo_el_newton_rhs_noalloc.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
         L.alamo_el_stress.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, vp]
         L.alamo_el_tag_bbox.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -327,17 +328,20 @@ class ElasticOperator:
         _check(lib().alamo_el_bc_init(self._h, self._v(rhs, 0), int(bool(homogeneous)), _stream()), "alamo_el_bc_init")
         return rhs
 
-    def newton_rhs(self, C, u, b, F0=None, layout: int = SOA, out=None):
+    def newton_rhs(self, C, u, b, F0=None, layout: int = SOA, out=None, noalloc: bool = False):
         """Newton<T>::prepareForSolve right-hand side, b - N(u)
-        (src/Solver/Nonlocal/Newton.H:151-231)."""
+        (src/Solver/Nonlocal/Newton.H:151-231).  noalloc selects
+        alamo_el_newton_rhs_noalloc (the library allocates nothing; give `out`
+        so that this call does not allocate either)."""
         r = self._vec(0) if out is None else out
         f0 = self._v(F0, 0, 9) if F0 is not None else ctypes.c_void_p()
         shape = self.level_shape(0)
         want = (self.ncomp,) + shape if layout == SOA else shape + (self.ncomp,)
         if tuple(C.shape) != want:
             raise AlamoError(f"newton_rhs: expected C of shape {want}, got {tuple(C.shape)}")
-        _check(lib().alamo_el_newton_rhs(self._h, _ptr(C), int(layout), f0, self._v(u, 0), self._v(b, 0),
-                                         self._v(r, 0), _stream()), "alamo_el_newton_rhs")
+        fn = lib().alamo_el_newton_rhs_noalloc if noalloc else lib().alamo_el_newton_rhs
+        _check(fn(self._h, _ptr(C), int(layout), f0, self._v(u, 0), self._v(b, 0), self._v(r, 0), _stream()),
+               "alamo_el_newton_rhs_noalloc" if noalloc else "alamo_el_newton_rhs")
         return r
 
     def mix_models(self, eta, table, out=None):

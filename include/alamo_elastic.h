@@ -237,6 +237,18 @@ int alamo_el_mix_models(alamo_el_t h, const double* eta, int nmodels, const doub
 int alamo_el_newton_rhs(alamo_el_t h, const double* C, int layout, const double* F0, const double* u,
                         const double* b, double* rhs, hipStream_t stream);
 
+/* alamo_el_newton_rhs without allocation.  No counterpart in the reference:
+ * it is for callers that cannot call hipMalloc (code under HIP graph capture,
+ * time-stepping or serving loops, harnesses that forbid allocation).  Same
+ * arguments and result as alamo_el_newton_rhs (bit for bit); instead of the
+ * handle's on-demand 18 N doubles it marches the box in slabs of z planes
+ * through a fixed 32 MiB device array of the library (one per GPU, held until
+ * the process ends), so a plane may hold at most 155,344 nodes (e.g. 394 x 394);
+ * larger planes return ALAMO_EL_EINVAL.  Calls that overlap on one GPU (other
+ * streams or threads) share that array and race. */
+int alamo_el_newton_rhs_noalloc(alamo_el_t h, const double* C, int layout, const double* F0, const double* u,
+                                const double* b, double* rhs, hipStream_t stream);
+
 /* Solver::Nonlocal::Linear::solve(sol, rhs, tol_rel, tol_abs)
  * (src/Solver/Nonlocal/Linear.H:131-147): solves A x = b on the finest level
  * to |b - A x|_inf <= max(tol_abs, tol_rel * max(|b|_inf, |b - A x0|_inf)).

@@ -130,6 +130,11 @@ struct alamo_el_s {
     int* direct_map = nullptr;      // a block's row map, then the inverse's column sources (M)
     double* direct_part = nullptr;  // partial sums of the bottom product
     ael::InvState* direct_st = nullptr;
+    double* direct_A = nullptr;     // the operator (M x M), kept for the check and the pivoted inversion
+    double* direct_X = nullptr;     // D^-1 times a block's rows (kInvBlock x M)
+    double* direct_D = nullptr;     // D^-1 of a diagonal block (kInvBlock x kInvBlock)
+    double* direct_chk = nullptr;   // x, Ainv x and A Ainv x of the check (3 M)
+    int direct_pivoted = 0;         // the last inverse came from the pivoted panels
     // FGMRES storage, allocated on demand
     double* kh = nullptr;          // Gram-Schmidt coefficients on the device
     double* kpart = nullptr;       // multi-dot partials
@@ -578,12 +583,96 @@ int direct_level_of(alamo_el_t h) {
 
 void free_direct(alamo_el_t h) {
     for (void* p : {(void*)h->direct_W, (void*)h->direct_Ainv, (void*)h->direct_perm, (void*)h->direct_map,
-                    (void*)h->direct_part, (void*)h->direct_st})
+                    (void*)h->direct_part, (void*)h->direct_st, (void*)h->direct_A, (void*)h->direct_X,
+                    (void*)h->direct_D, (void*)h->direct_chk})
         if (p) (void)hipFree(p);
-    h->direct_W = h->direct_Ainv = h->direct_part = nullptr;
+    h->direct_W = h->direct_Ainv = h->direct_part = h->direct_A = h->direct_X = h->direct_D = h->direct_chk = nullptr;
     h->direct_perm = h->direct_map = nullptr;
     h->direct_st = nullptr;
     h->direct_M = 0;
+}
+
+// Inverse of the M x M operator in direct_W (destroyed) by block
+// Gauss-Jordan without pivoting across blocks (inverse_kernels.h); *inv is
+// the buffer holding it (direct_W or direct_Ainv), or null when a block
+// pivot vanished or the check of the inverse against the kept operator
+// (max |A Ainv x - x| / max |x| <= kInvCheckTol) failed.
+constexpr double kInvCheckTol = 1e-8;
+int invert_blocked(alamo_el_t h, int M, hipStream_t s, double** inv) {
+    const ael::InvState zero{0, 0.0};
+    HIPCHK(hipMemcpyAsync(h->direct_st, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    double *W = h->direct_W, *W2 = h->direct_Ainv;
+    const unsigned gt = (unsigned)((M + ael::kInvTile - 1) / ael::kInvTile);
+    for (int k0 = 0; k0 < M; k0 += ael::kInvBlock) {
+        const int nb = std::min(ael::kInvBlock, M - k0);
+        ael::k_inv_diag<<<1, kBlock, 0, s>>>(W, M, k0, nb, h->direct_D, h->direct_st);
+        ael::k_inv_xrow<<<gt, kBlock, 0, s>>>(W, M, k0, nb, h->direct_D, h->direct_X);
+        ael::k_inv_gemm_blk<<<dim3(gt, gt), kBlock, 0, s>>>(W, W2, M, k0, nb, h->direct_X);
+        std::swap(W, W2);
+    }
+    // the check: z = A (W x)
+    double *x = h->direct_chk, *y = x + M, *z = y + M;
+    const dim3 grid(nblk(M), (M + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk);
+    ael::k_inv_check_x<<<nblk(M), kBlock, 0, s>>>(x, M);
+    ael::k_dense_mv_partial<double, double><<<grid, kBlock, 0, s>>>(W, x, M, h->direct_part);
+    ael::k_dense_mv_final<double><<<nblk(M), kBlock, 0, s>>>(h->direct_part, (int)grid.y, M, y);
+    ael::k_dense_mv_partial<double, double><<<grid, kBlock, 0, s>>>(h->direct_A, y, M, h->direct_part);
+    ael::k_dense_mv_final<double><<<nblk(M), kBlock, 0, s>>>(h->direct_part, (int)grid.y, M, z);
+    ael::k_inv_check<<<1, kBlock, 0, s>>>(z, M, h->direct_st);
+    HIPCHK(hipGetLastError());
+    ael::InvState st{};
+    HIPCHK(hipMemcpyAsync(&st, h->direct_st, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *inv = (!st.singular && st.resid <= kInvCheckTol) ? W : nullptr;
+    return 0;
+}
+
+// Inverse of the M x M operator in direct_W (destroyed) by blocked
+// Gauss-Jordan with partial pivoting over all rows (inverse_kernels.h); *inv
+// as above, null when the operator is singular.
+int invert_pivoted(alamo_el_t h, int M, hipStream_t s, double** inv) {
+    const ael::InvState zero{0, 0.0};
+    HIPCHK(hipMemcpyAsync(h->direct_st, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    // blocks of kInvBlock columns: panels of pw pivot steps, each updating
+    // the rest of its block, then the block's update of every other column,
+    // out of place (W and the inverse's buffer alternate)
+    double *W = h->direct_W, *W2 = h->direct_Ainv;
+    const unsigned gt = (unsigned)((M + ael::kInvTile - 1) / ael::kInvTile);
+    const int pw = ael::inv_panel_width(M);
+    for (int k0 = 0; k0 < M; k0 += ael::kInvBlock) {
+        const int nbk = std::min(ael::kInvBlock, M - k0), b1 = k0 + nbk;
+        for (int c0 = k0; c0 < b1; c0 += pw) {
+            const int nb = std::min(pw, b1 - c0);
+            ael::launch_inv_panel(W, M, c0, nb, h->direct_perm, h->direct_st, s);
+            if (nbk > nb)
+                ael::k_inv_block<<<(unsigned)(nbk - nb), kBlock, 0, s>>>(W, M, k0, b1, c0, nb, h->direct_perm);
+        }
+        if (M > nbk) {
+            ael::k_inv_rowmap<<<1, kBlock, 0, s>>>(h->direct_perm, M, k0, nbk, h->direct_map);
+            ael::k_inv_gemm<<<dim3(gt, gt), kBlock, 0, s>>>(W, W2, M, k0, nbk, h->direct_map);
+            std::swap(W, W2);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    ael::InvState st{};
+    std::vector<int> perm(M), src(M);
+    HIPCHK(hipMemcpyAsync(&st, h->direct_st, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(perm.data(), h->direct_perm, sizeof(int) * M, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st.singular) {
+        *inv = nullptr;
+        return 0;
+    }
+    // W holds the inverse with its columns permuted: undo the swaps from the
+    // last step to the first (column c of the inverse is column src[c] of W)
+    for (int c = 0; c < M; ++c) src[c] = c;
+    for (int k = M - 1; k >= 0; --k) std::swap(src[k], src[perm[k]]);
+    HIPCHK(hipMemcpyAsync(h->direct_map, src.data(), sizeof(int) * M, hipMemcpyHostToDevice, s));
+    ael::k_inv_unpermute<<<nblk((long)M * M), kBlock, 0, s>>>(W, M, h->direct_map, W2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));  // src lives on the host stack
+    *inv = W2;
+    return 0;
 }
 
 // Operator of the direct level (A e_q for every unit vector) and its FP64
@@ -607,6 +696,10 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
         ALLOC(h->direct_map, M);
         ALLOC(h->direct_part, (long)((M + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk) * M);
         ALLOC(h->direct_st, 1);
+        ALLOC(h->direct_A, (long)M * M);
+        ALLOC(h->direct_X, (long)ael::kInvBlock * M);
+        ALLOC(h->direct_D, (long)ael::kInvBlock * ael::kInvBlock);
+        ALLOC(h->direct_chk, 3L * M);
         h->direct_M = M;
     }
     // the unit vectors (in the buffer the inverse will take), the operator's columns into W
@@ -620,48 +713,24 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
     }
     psi_post_columns(h, lb, h->direct_Ainv, h->direct_W, M, s);
     HIPCHK(hipGetLastError());
-    const ael::InvState zero{0};
-    HIPCHK(hipMemcpyAsync(h->direct_st, &zero, sizeof(zero), hipMemcpyHostToDevice, s));
-    // blocks of kInvBlock columns: panels of kInvPanel pivot steps, each
-    // updating the rest of its block, then the block's update of every other
-    // column, out of place (W and the inverse's buffer alternate)
-    double *W = h->direct_W, *W2 = h->direct_Ainv;
-    const unsigned gt = (unsigned)((M + ael::kInvTile - 1) / ael::kInvTile);
-    for (int k0 = 0; k0 < M; k0 += ael::kInvBlock) {
-        const int nbk = std::min(ael::kInvBlock, M - k0), b1 = k0 + nbk;
-        for (int c0 = k0; c0 < b1; c0 += ael::kInvPanel) {
-            const int nb = std::min(ael::kInvPanel, b1 - c0);
-            ael::k_inv_panel<<<1, ael::kInvThreads, 0, s>>>(W, M, c0, nb, h->direct_perm, h->direct_st);
-            if (nbk > nb)
-                ael::k_inv_block<<<(unsigned)(nbk - nb), kBlock, 0, s>>>(W, M, k0, b1, c0, nb, h->direct_perm);
-        }
-        if (M > nbk) {
-            ael::k_inv_rowmap<<<1, kBlock, 0, s>>>(h->direct_perm, M, k0, nbk, h->direct_map);
-            ael::k_inv_gemm<<<dim3(gt, gt), kBlock, 0, s>>>(W, W2, M, k0, nbk, h->direct_map);
-            std::swap(W, W2);
-        }
+    HIPCHK(hipMemcpyAsync(h->direct_A, h->direct_W, sizeof(double) * MM, hipMemcpyDeviceToDevice, s));
+    double* inv = nullptr;
+    const char* how = std::getenv("ALAMO_EL_INVERSE");  // "pivoted": skip the block path
+    int rc = how && std::strcmp(how, "pivoted") == 0 ? 0 : invert_blocked(h, M, s, &inv);
+    if (rc) return rc;
+    h->direct_pivoted = 0;
+    if (!inv) {  // a vanishing or inaccurate block pivot: the pivoted panels, from the kept operator
+        HIPCHK(hipMemcpyAsync(h->direct_W, h->direct_A, sizeof(double) * MM, hipMemcpyDeviceToDevice, s));
+        rc = invert_pivoted(h, M, s, &inv);
+        if (rc) return rc;
+        h->direct_pivoted = 1;
     }
-    HIPCHK(hipGetLastError());
-    ael::InvState st{};
-    std::vector<int> perm(M), src(M);
-    HIPCHK(hipMemcpyAsync(&st, h->direct_st, sizeof(st), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(perm.data(), h->direct_perm, sizeof(int) * M, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (st.singular) {
+    if (!inv) {  // singular
         (void)hipFree(h->direct_Ainv);
         h->direct_Ainv = nullptr;
         return 0;
     }
-    // W holds the inverse with its columns permuted: undo the swaps from the
-    // last step to the first (column c of the inverse is column src[c] of W)
-    for (int c = 0; c < M; ++c) src[c] = c;
-    for (int k = M - 1; k >= 0; --k) std::swap(src[k], src[perm[k]]);
-    HIPCHK(hipMemcpyAsync(h->direct_map, src.data(), sizeof(int) * M, hipMemcpyHostToDevice, s));
-    ael::k_inv_unpermute<<<nblk(MM), kBlock, 0, s>>>(W, M, h->direct_map, W2);
-    HIPCHK(hipGetLastError());
-    h->direct_Ainv = W2;
-    h->direct_W = W;
-    HIPCHK(hipStreamSynchronize(s));  // src lives on the host stack
+    if (inv != h->direct_Ainv) std::swap(h->direct_W, h->direct_Ainv);
     return 0;
 }
 
@@ -1285,6 +1354,13 @@ int alamo_el_palette_size(alamo_el_t h, int mglev, int* npal) {
 int alamo_el_prepare(alamo_el_t h, hipStream_t s) {
     if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
     return prepare(h, s);
+}
+
+int alamo_el_prepare_fp32(alamo_el_t h, hipStream_t s) {
+    if (!h) return fail(ALAMO_EL_EINVAL, "null handle");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    int rc = h->prepared ? 0 : prepare(h, s);
+    return rc ? rc : prepare32(h, s);
 }
 
 int alamo_el_fapply(alamo_el_t h, int mglev, const double* u, double* f, hipStream_t s) {

@@ -56,7 +56,8 @@ class Info(ctypes.Structure):
 EXPORTS = [
     "alamo_el_last_error", "alamo_el_default_opts", "alamo_el_create", "alamo_el_destroy",
     "alamo_el_num_levels", "alamo_el_level_nodes", "alamo_el_set_bc", "alamo_el_set_uniform",
-    "alamo_el_set_omega", "alamo_el_set_model", "alamo_el_prepare", "alamo_el_fapply", "alamo_el_diagonal",
+    "alamo_el_set_omega", "alamo_el_set_model", "alamo_el_prepare", "alamo_el_prepare_fp32", "alamo_el_fapply",
+    "alamo_el_diagonal",
     "alamo_el_fsmooth", "alamo_el_residual", "alamo_el_restrict", "alamo_el_interp_add", "alamo_el_bc_init",
     "alamo_el_newton_rhs", "alamo_el_solve", "alamo_el_norm_inf", "alamo_el_level_model", "alamo_el_coarse_rhs",
     "alamo_el_learned_input", "alamo_el_set_learned", "alamo_el_learned_guess", "alamo_amr_patch_create",
@@ -91,6 +92,7 @@ def lib():
         L.alamo_el_set_psi.argtypes = [vp, vp, ctypes.c_double, vp]
         L.alamo_el_palette_size.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.alamo_el_prepare.argtypes = [vp, vp]
+        L.alamo_el_prepare_fp32.argtypes = [vp, vp]
         L.alamo_el_fapply.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.alamo_el_diagonal.argtypes = [vp, ctypes.c_int, vp, vp]
         L.alamo_el_fsmooth.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
@@ -263,6 +265,11 @@ class ElasticOperator:
 
     def prepare(self):
         _check(lib().alamo_el_prepare(self._h, _stream()), "alamo_el_prepare")
+
+    def prepare_fp32(self):
+        """The FP32 copies of the FP32 cycles (alamo_el_prepare_fp32); the
+        first FP32 solve after a model change makes them otherwise."""
+        _check(lib().alamo_el_prepare_fp32(self._h, _stream()), "alamo_el_prepare_fp32")
 
     def palette_size(self, mglev: int = 0) -> int:
         """Rows of the level's C palette (0: the level reads its C copy)."""

@@ -296,6 +296,23 @@ def main():
     log(f"learned + FP32 path: {fast_t:.3f}s per solve, {head['fgmres_steps']} FGMRES steps, "
         f"{head['learned_steps']} learned steps taken, rel res {head['rel_res']:.2e}")
 
+    # --- the set-up of a model update on the same handle (one time step) ------
+    resetup = {}
+    for _ in range(2):  # the second, warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.SetModel(C)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        op.prepare()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        op.prepare_fp32()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        resetup = dict(set_model=t1 - t0, prepare=t2 - t1, prepare_fp32=t3 - t2, total=t3 - t0)
+    log("set-up of a model update: " + ", ".join(f"{k} {v * 1e3:.1f} ms" for k, v in resetup.items()))
+
     traffic = None if args.no_traffic else measure_traffic(args)
     traffic_bytes = palette_traffic = None
     if traffic:
@@ -363,7 +380,8 @@ def main():
         "speedup_vs_plain_fp64_V": (plain["V"]["seconds"] / fast_t) if "V" in plain else None,
         "network_gain_vs_fp32_only": fp32_only["seconds"] / fast_t if learned else None,
         "speedup_vs_plain_fp64_F": (plain["F"]["seconds"] / fast_t) if "F" in plain else None,
-        "setup_once_per_model_s": {"set_model": set_model_s, "prepare": prepare_s},
+        "setup_once_per_model_s": dict(resetup, first_model_on_a_new_handle={"set_model": set_model_s,
+                                                                              "prepare": prepare_s}),
         "fapply": {"us_per_launch": fapply_ms * 1e3, "gbs": fapply_gbs, "bytes_per_node": FAPPLY_BYTES_PER_NODE[p["sym"]],
                    "c": "per-node copy (21 values)"},
         "fapply_palette": {"us_per_launch": palette_ms * 1e3, "gbs_algorithmic": palette_gbs,

@@ -173,6 +173,13 @@ int alamo_el_palette_size(alamo_el_t h, int mglev, int* npal);
  * Diagonal(true) (src/Operator/Operator.cpp:253-260,
  * src/Operator/Elastic.cpp:605-626,746-910). */
 int alamo_el_prepare(alamo_el_t h, hipStream_t stream);
+/* The FP32 copies the FP32 cycles read (C, its tile and boundary copies, the
+ * palette table, the compact grad-C matrices and the diagonal of every MG
+ * level), after alamo_el_prepare.  No counterpart in the reference, which has
+ * no FP32 path; the first ALAMO_EL_PREC_FP32_IR solve after a model change
+ * makes them otherwise.  A caller that updates the model every time step
+ * calls it with alamo_el_prepare to keep the set-up out of the solve. */
+int alamo_el_prepare_fp32(alamo_el_t h, hipStream_t stream);
 
 /* Operator::Elastic::Fapply, f = A u on MG level mglev
  * (src/Operator/Elastic.cpp:139-292). */

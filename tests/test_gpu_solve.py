@@ -95,10 +95,17 @@ def test_polycrystal_fp64_matches_oracle(n_cell):
 # run to 1e-14 on those 2187 unknowns (a direct solve to rounding): the two
 # agree to 1e-10.
 # ALAMO_EL_DIRECT_MAX=375 moves the direct bottom to the 5^3 level (375 unknowns).
-@pytest.mark.parametrize("n_cell,coarsenings,direct_max", [(16, 1, None), (32, 2, None), (16, 2, "375")])
-def test_direct_bottom_cycles_match_oracle(n_cell, coarsenings, direct_max, monkeypatch):
+# The inverse comes from block Gauss-Jordan (pivoting inside 64-row blocks,
+# checked against the operator) or, with ALAMO_EL_INVERSE=pivoted, from the
+# panels pivoted over all rows.
+@pytest.mark.parametrize("n_cell,coarsenings,direct_max,inverse", [(16, 1, None, "blocked"), (32, 2, None, "blocked"),
+                                                                    (16, 2, "375", "blocked"),
+                                                                    (16, 1, None, "pivoted"), (16, 2, "375", "pivoted")])
+def test_direct_bottom_cycles_match_oracle(n_cell, coarsenings, direct_max, inverse, monkeypatch):
     if direct_max:
         monkeypatch.setenv("ALAMO_EL_DIRECT_MAX", direct_max)
+    if inverse == "pivoted":
+        monkeypatch.setenv("ALAMO_EL_INVERSE", "pivoted")
     p = P.polycrystal(n_cell, ngrains=8, seed=4)
     el, op, rhs = gpu_setup(p)
     rhs_o = oracle_rhs(p)
@@ -109,6 +116,41 @@ def test_direct_bottom_cycles_match_oracle(n_cell, coarsenings, direct_max, monk
                             bottom_tol_rel=1e-14, bottom_max_iter=4000, max_coarsening_level=coarsenings)
     assert s.info.iters == 3 and info["iters"] == 3
     assert rel_l2(host(x), xo) < 1e-10
+
+
+# Direct levels beyond the cubic hierarchies, for the other shapes of the
+# device inversion: 9 x 9 x 11 nodes (2673 unknowns; pivoted: 8-column panels
+# of 6 rows a thread) and 9 x 9 x 13 (3159; a partial last block of 23 rows;
+# pivoted: 8 rows a thread).  Three FP64
+# V-cycles with the direct bottom against the same cycles with the
+# hierarchy stopped at that level and BiCGStab run to 1e-14 there (a direct
+# solve to rounding), both on the device: they agree to 1e-10.
+@pytest.mark.parametrize("inverse", ["blocked", "pivoted"])
+@pytest.mark.parametrize("nz", [20, 24])
+def test_direct_bottom_on_non_cubic_levels(nz, inverse, monkeypatch):
+    if inverse == "pivoted":
+        monkeypatch.setenv("ALAMO_EL_INVERSE", "pivoted")
+    from alamo_amd import elastic as el
+    shape = (nz + 1, 17, 17)
+    z, y, xg = np.meshgrid(*[np.linspace(0.0, 1.0, s) for s in shape], indexing="ij")
+    E = 1.0 + 0.5 * np.sin(3.0 * xg + 2.0 * y) * np.cos(2.0 * z)
+    lam, mu = M.lame_from_E_nu(1.0, 0.3)
+    C = dev(np.ascontiguousarray(np.stack([lam * E, mu * E])))
+    bc = P.polycrystal(16, ngrains=2, seed=0)["bc"]
+    xs = []
+    for levels, bottom in ((2, "direct"), (1, "bicgstab")):
+        op = el.ElasticOperator((16, 16, nz), 1.0 / 16, M.ISOTROPIC, max_coarsening_level=levels)
+        op.SetBC(bc)
+        op.SetModel(C)
+        b = op.bc_init(torch.zeros((3,) + shape, dtype=torch.float64, device="cuda"))
+        rhs = op.newton_rhs(C, torch.zeros_like(b), b)
+        x = torch.zeros_like(rhs)
+        s = el.Linear(op, cycle="V", bottom_solver=bottom, max_iter=3, abort_on_fail=False, bottom_tol_rel=1e-14,
+                      bottom_max_iter=4000)
+        s.solve(x, rhs, 1e-14, 0.0)
+        assert s.info.iters == 3
+        xs.append(host(x))
+    assert rel_l2(xs[0], xs[1]) < 1e-10
 
 
 @pytest.mark.parametrize("precision,outer,n_cell", [("fp32", "mlmg", 32), ("fp64", "fgmres", 32),
@@ -221,3 +263,21 @@ def test_psi_weighted_solve_solves_the_weighted_system(precision, outer):
     with pytest.raises(el.AlamoError):
         el.Linear(big).solve(torch.zeros((3, 19, 19, 19), dtype=torch.float64, device="cuda"),
                              torch.ones((3, 19, 19, 19), dtype=torch.float64, device="cuda"), 1e-8, 0.0)
+
+
+def test_prepare_fp32_ahead_of_the_solve_gives_the_same_bits():
+    """alamo_el_prepare_fp32 makes the FP32 copies the first FP32 solve would
+    make; a solve after it is the same, bit for bit, after a model update too."""
+    p = P.polycrystal(32, ngrains=8, seed=2)
+    el, op, rhs = gpu_setup(p)
+    C = dev(p["C"])
+    xs = []
+    for ahead in (False, True, True):
+        op.SetModel(C)
+        op.prepare()
+        if ahead:
+            op.prepare_fp32()
+        x = torch.zeros_like(rhs)
+        el.Linear(op, precision="fp32", outer="fgmres", bottom_solver="direct").solve(x, rhs, 1e-8, 0.0)
+        xs.append(x)
+    assert torch.equal(xs[0], xs[1]) and torch.equal(xs[1], xs[2])

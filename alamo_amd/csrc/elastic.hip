@@ -125,7 +125,7 @@ struct alamo_el_s {
     int direct_level = -1;
     int direct_M = 0;
     double* direct_W = nullptr;     // work matrix of the inversion (M x M)
-    double* direct_Ainv = nullptr;  // inverse, column-major (null when singular)
+    double* direct_Ainv = nullptr;  // inverse, row-major (null when singular)
     int* direct_perm = nullptr;     // row swapped with row k at pivot step k (M)
     int* direct_map = nullptr;      // a block's row map, then the inverse's column sources (M)
     double* direct_part = nullptr;  // partial sums of the bottom product
@@ -420,13 +420,10 @@ struct Cycle {
             // exact FP64 solve with the inverse of the direct level's operator
             const int l = bot();
             const int Mb = (int)M(l);
-            const dim3 grid(nblk(Mb), (Mb + ael::kDenseMvChunk - 1) / ael::kDenseMvChunk);
-            // the FP32 cycles keep this solve in FP64 too: the FP64 inverse times
-            // the residual widened to FP64, the sums in FP64, the result rounded
-            // to the cycle's precision
-            ael::k_dense_mv_partial<T, double><<<grid, kBlock, 0, s>>>(h->direct_Ainv, V(l).res, Mb,
-                                                                       h->direct_part);
-            ael::k_dense_mv_final<T><<<nblk(Mb), kBlock, 0, s>>>(h->direct_part, (int)grid.y, Mb, V(l).cor);
+            // the FP32 cycles keep this solve in FP64 too: the FP64 inverse
+            // (row-major) times the residual widened to FP64, the sums in FP64,
+            // the result rounded to the cycle's precision; one wave a row
+            ael::k_dense_mv_rows<T><<<nblk(64L * Mb), kBlock, 0, s>>>(h->direct_Ainv, V(l).res, Mb, V(l).cor);
             return;
         }
         const int l = nlev() - 1;
@@ -491,8 +488,20 @@ struct Cycle {
     // (MLLinOp::avgDownResMG -> amrex::average_down on nodal data).
     void fcycle() {
         const int bot = this->bot();
-        for (int l = 1; l <= bot; ++l)
-            ael::k_inject<T><<<nblk(L(l).L.N), kBlock, 0, s>>>(V(l - 1).res, L(l - 1).L, V(l).res, L(l).L, 3);
+        if (bot + 1 > ael::kInjectMaxLevels) {
+            for (int l = 1; l <= bot; ++l)
+                ael::k_inject<T><<<nblk(L(l).L.N), kBlock, 0, s>>>(V(l - 1).res, L(l - 1).L, V(l).res, L(l).L, 3);
+        } else if (bot >= 1) {  // every level's residual from level 0 in one launch
+            ael::InjectLevels<T> il{};
+            il.n = bot + 1;
+            il.off[1] = 0;
+            for (int l = 0; l <= bot; ++l) {
+                il.f[l] = V(l).res;
+                il.L[l] = L(l).L;
+                if (l >= 1) il.off[l + 1] = il.off[l] + L(l).L.N;
+            }
+            ael::k_inject_all<T><<<nblk(il.off[bot + 1]), kBlock, 0, s>>>(il, 3);
+        }
         bottom();
         for (int l = bot - 1; l >= 0; --l) {
             Vecs<T>& v = V(l);
@@ -730,7 +739,13 @@ int assemble_direct(alamo_el_t h, hipStream_t s) {
         h->direct_Ainv = nullptr;
         return 0;
     }
-    if (inv != h->direct_Ainv) std::swap(h->direct_W, h->direct_Ainv);
+    // the bottom reads the inverse by rows: store it row-major in the other buffer
+    double* other = inv == h->direct_W ? h->direct_Ainv : h->direct_W;
+    const unsigned gt = (unsigned)((M + 63) / 64);
+    ael::k_transpose<<<dim3(gt, gt), kBlock, 0, s>>>(inv, M, other);
+    HIPCHK(hipGetLastError());
+    h->direct_Ainv = other;
+    h->direct_W = inv;
     return 0;
 }
 
@@ -1817,37 +1832,57 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             }
             // w = A Z_j
             launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);
-            // classical Gram-Schmidt with the DGKS criterion ("twice is
-            // enough"): h = V^T w; w -= V h, and a second pass only when the
-            // first cancelled most of w (|w'| < 0.7 |w|).  Coefficients and
-            // norms stay on the device and come back in one copy per pass.
+            // classical Gram-Schmidt, twice ("twice is enough"): h = V^T w;
+            // w -= V h; h' = V^T w; w -= V h'.  Up to kMultiDotMax basis
+            // vectors the passes are fused: the multi-dot, then one pass that
+            // projects and forms the next dots, then one that projects and
+            // forms |w|^2 (V read three times); the coefficients and the norm
+            // come back in one copy.  Beyond that: the DGKS criterion decides
+            // the second pass, with separate dot, projection and norm passes.
             const int nb = (int)std::min<long>(kRedBlocks, std::max<long>(1, nblk(M0)));
             auto sq_norm = [&](double* dst) {
                 ael::k_reduce_partial<double><<<nb, kBlock, 0, s>>>(h->kw, h->kw, M0, 1, h->red_partial);
                 ael::k_reduce_final<<<1, kBlock, 0, s>>>(h->red_partial, nb, 1, dst);
             };
             double* nrm_after = h->kh + 2 * (m + 1);
-            sq_norm(nrm_after + 1);  // |w|^2 before
-            double w2_before = 0.0, hn2 = 0.0;
+            double hn2 = 0.0;
             for (int i = 0; i <= j; ++i) H[i * m + j] = 0.0;
-            for (int pass = 0; pass < 2; ++pass) {
-                double* hp = h->kh + (long)pass * (m + 1);
-                for (int c0 = 0; c0 <= j; c0 += ael::kMultiDotMax) {
-                    const int kc = std::min(ael::kMultiDotMax, j + 1 - c0);
-                    ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv + (long)c0 * MP, kc, M0, MP, h->kw,
-                                                                          h->kpart + (long)c0 * kDotBlocks);
-                }
-                ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, hp);
-                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, MP, hp, -1.0);
-                sq_norm(nrm_after);
+            if (j + 1 <= ael::kMultiDotMax) {
+                double *h0 = h->kh, *h1 = h->kh + (m + 1);
+                ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, j + 1, M0, MP, h->kw, h->kpart);
+                ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h0);
+                ael::k_multiaxpy_dot<true><<<kDotBlocks, kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, MP, h0, h->kpart);
+                ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h1);
+                ael::k_multiaxpy_dot<false><<<kDotBlocks, kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, MP, h1, h->kpart);
+                ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, nrm_after);
                 HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (2 * (m + 1) + 2),
                                       hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
                 const double* hv = h->red_host_vec;
-                for (int i = 0; i <= j; ++i) H[i * m + j] += hv[(long)pass * (m + 1) + i];
-                if (pass == 0) w2_before = hv[2 * (m + 1) + 1];
+                for (int i = 0; i <= j; ++i) H[i * m + j] = hv[i] + hv[(m + 1) + i];
                 hn2 = hv[2 * (m + 1)];
-                if (!(hn2 < 0.49 * w2_before)) break;
+            } else {
+                sq_norm(nrm_after + 1);  // |w|^2 before
+                double w2_before = 0.0;
+                for (int pass = 0; pass < 2; ++pass) {
+                    double* hp = h->kh + (long)pass * (m + 1);
+                    for (int c0 = 0; c0 <= j; c0 += ael::kMultiDotMax) {
+                        const int kc = std::min(ael::kMultiDotMax, j + 1 - c0);
+                        ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv + (long)c0 * MP, kc, M0, MP,
+                                                                              h->kw, h->kpart + (long)c0 * kDotBlocks);
+                    }
+                    ael::k_multidot_final<<<j + 1, kBlock, 0, s>>>(h->kpart, kDotBlocks, hp);
+                    ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(h->kw, h->kv, j + 1, M0, MP, hp, -1.0);
+                    sq_norm(nrm_after);
+                    HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (2 * (m + 1) + 2),
+                                          hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                    const double* hv = h->red_host_vec;
+                    for (int i = 0; i <= j; ++i) H[i * m + j] += hv[(long)pass * (m + 1) + i];
+                    if (pass == 0) w2_before = hv[2 * (m + 1) + 1];
+                    hn2 = hv[2 * (m + 1)];
+                    if (!(hn2 < 0.49 * w2_before)) break;
+                }
             }
             const double hn = std::sqrt(hn2);
             H[(j + 1) * m + j] = hn;

@@ -642,6 +642,33 @@ __global__ void __launch_bounds__(kBlock) k_inject(const T* __restrict__ fine, L
     for (int n = 0; n < ncomp; ++n) crse[n * Lc.N + cid] = fine[n * Lf.N + f0];
 }
 
+// Nodal injection from level 0 to every coarser level 1 .. n-1 at once:
+// node (I, J, K) of level l takes node (2^l I, 2^l J, 2^l K) of level 0,
+// which is what l successive injections give (each copies a node of the
+// level above it).  Launch over the sum of the coarse levels' nodes.
+constexpr int kInjectMaxLevels = 16;
+template <typename T>
+struct InjectLevels {
+    int n;  // levels, the finest included
+    T* f[kInjectMaxLevels];
+    Lvl L[kInjectMaxLevels];
+    long off[kInjectMaxLevels + 1];  // off[l]: first flat index of level l (off[1] = 0)
+};
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_inject_all(InjectLevels<T> lv, int ncomp) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= lv.off[lv.n]) return;
+    int l = 1;
+    while (e >= lv.off[l + 1]) ++l;
+    const Lvl& Lc = lv.L[l];
+    const Lvl& L0 = lv.L[0];
+    const long cid = e - lv.off[l];
+    int I, J, K;
+    node_ijk(cid, Lc, I, J, K);
+    const long f0 = ((long)K << l) * ((long)L0.nx * L0.ny) + ((long)J << l) * L0.nx + ((long)I << l);
+    for (int n = 0; n < ncomp; ++n) lv.f[l][n * Lc.N + cid] = lv.f[0][n * L0.N + f0];
+}
+
 // Trilinear prolongation, fine += P crse (src/Operator/Operator.cpp:399-428).
 // With zero_first the fine field is overwritten instead (cor = P cor_crse).
 template <typename T>
@@ -1021,6 +1048,46 @@ __global__ void __launch_bounds__(kBlock) k_dense_mv_final(const double* __restr
     y[r] = (T)acc;
 }
 
+// Direct bottom solve with the inverse stored row-major (the transpose of
+// the column-major inverse): one wave per row, a dot product of contiguous
+// FP64 values with x widened to FP64, summed in FP64 and rounded to T.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_dense_mv_rows(const double* __restrict__ At, const T* __restrict__ x,
+                                                          int M, T* __restrict__ y) {
+    const int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
+    if (r >= M) return;  // whole waves
+    const double* __restrict__ row = At + (long)r * M;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    int c = lane;
+    for (; c + 192 < M; c += 256) {
+        acc0 += row[c] * (double)x[c];
+        acc1 += row[c + 64] * (double)x[c + 64];
+        acc2 += row[c + 128] * (double)x[c + 128];
+        acc3 += row[c + 192] * (double)x[c + 192];
+    }
+    for (; c < M; c += 64) acc0 += row[c] * (double)x[c];
+    double acc = (acc0 + acc1) + (acc2 + acc3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) y[r] = (T)acc;
+}
+
+// Out-of-place transpose of an M x M matrix (column-major in, row-major out:
+// the same bytes read as the other order), 64 x 64 tiles through LDS.
+__global__ void __launch_bounds__(kBlock) k_transpose(const double* __restrict__ A, int M, double* __restrict__ At) {
+    __shared__ double tile[64][65];
+    const int r0 = (int)blockIdx.x * 64, c0 = (int)blockIdx.y * 64;
+    for (int e = (int)threadIdx.x; e < 64 * 64; e += kBlock) {
+        const int c = e >> 6, r = e & 63;
+        tile[c][r] = (r0 + r < M && c0 + c < M) ? A[(long)(c0 + c) * M + r0 + r] : 0.0;
+    }
+    __syncthreads();
+    for (int e = (int)threadIdx.x; e < 64 * 64; e += kBlock) {
+        const int r = e >> 6, c = e & 63;
+        if (r0 + r < M && c0 + c < M) At[(long)(r0 + r) * M + c0 + c] = tile[c][r];
+    }
+}
+
 // Block-wide max (op 0) or sum (op 1) for kBlock threads; all threads get it.
 __device__ __forceinline__ double blk_reduce256(double v, int op, double* red) {
 #pragma unroll
@@ -1193,6 +1260,84 @@ __global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __res
     }
     __syncthreads();
     if ((int)threadIdx.x < k) {
+        double t = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < kBlock / 64; ++wv) t += sh[wv][threadIdx.x];
+        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// One pass of w <- w - sum_{i<k} h[i] V_i (k <= kMultiDotMax, h on the
+// device) that also forms, from the updated w, the partials of V_i . w
+// (slots 0 .. k-1, when DOTS) and of w . w (slot k when DOTS, else slot 0),
+// each slot holding gridDim.x block partials: the projection and the dots
+// or the norm of classical Gram-Schmidt's next step read V once instead of
+// twice.  Grid-stride, four strides of independent loads per step.
+template <bool DOTS>
+__global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w, const double* __restrict__ V, int k,
+                                                          long M, long ld, const double* __restrict__ h,
+                                                          double* __restrict__ partial) {
+    __shared__ double sh[kBlock / 64][kMultiDotMax + 1];
+    double hc[kMultiDotMax];
+#pragma unroll
+    for (int i = 0; i < kMultiDotMax; ++i) hc[i] = i < k ? h[i] : 0.0;
+    double acc[kMultiDotMax], nrm = 0.0;
+#pragma unroll
+    for (int i = 0; i < kMultiDotMax; ++i) acc[i] = 0.0;
+    const long st = (long)gridDim.x * blockDim.x;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q + 3 * st < M; q += 4 * st) {
+        double wq[4], vq[kMultiDotMax][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) wq[u] = w[q + u * st];
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax; ++i)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            double a = wq[u];
+#pragma unroll
+            for (int i = 0; i < kMultiDotMax; ++i)
+                if (i < k) a += -hc[i] * vq[i][u];
+            w[q + u * st] = a;
+            if constexpr (DOTS) {
+#pragma unroll
+                for (int i = 0; i < kMultiDotMax; ++i)
+                    if (i < k) acc[i] += vq[i][u] * a;
+            }
+            nrm += a * a;
+        }
+    }
+    for (; q < M; q += st) {
+        double a = w[q];
+        double vq[kMultiDotMax];
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax; ++i) vq[i] = i < k ? V[(long)i * ld + q] : 0.0;
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax; ++i)
+            if (i < k) a += -hc[i] * vq[i];
+        w[q] = a;
+        if constexpr (DOTS) {
+#pragma unroll
+            for (int i = 0; i < kMultiDotMax; ++i)
+                if (i < k) acc[i] += vq[i] * a;
+        }
+        nrm += a * a;
+    }
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+    const int nd = DOTS ? k : 0;
+#pragma unroll
+    for (int i = 0; i <= kMultiDotMax; ++i) {
+        if (i <= nd) {
+            double v = i < nd ? acc[i < kMultiDotMax ? i : 0] : nrm;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) sh[wave][i] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x <= nd) {
         double t = 0.0;
 #pragma unroll
         for (int wv = 0; wv < kBlock / 64; ++wv) t += sh[wv][threadIdx.x];

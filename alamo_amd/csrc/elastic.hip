@@ -392,21 +392,31 @@ struct Cycle {
         }
     }
     // x = 0, then Fsmooth x n; the first Jacobi sweep from zero is w b / D,
-    // one elementwise kernel instead of the fill and a stencil sweep.
-    void smooth_from_zero(int l, T* x, const T* b, int n) {
+    // one elementwise kernel instead of the fill and a stencil sweep (with
+    // first_ready, the restriction into this level has already written it
+    // to v.tmp: fz_ready).
+    bool fz_ready() const { return smoother == ALAMO_EL_SMOOTH_JACOBI && nu1 > 0; }
+    void smooth_from_zero(int l, T* x, const T* b, int n, bool first_ready = false) {
         Vecs<T>& v = V(l);
         if (n <= 0 || smoother == ALAMO_EL_SMOOTH_RBGS) {
             launch_fill<T>(x, T(0), M(l), s);
             smooth(l, x, b, n);
             return;
         }
-        ael::k_jacobi_from_zero<T><<<nblk(M(l)), kBlock, 0, s>>>(b, v.diag, (T)h->omega, v.tmp, M(l));
+        if (!first_ready) ael::k_jacobi_from_zero<T><<<nblk(M(l)), kBlock, 0, s>>>(b, v.diag, (T)h->omega, v.tmp, M(l));
         launch_jacobi<T>(h, L(l), v.tmp, b, v.diag, v.C, x, s);
         smooth(l, x, b, n - 1);
     }
     void corr_res(int l, T* r, const T* x, const T* b) { launch_residual<T>(h, L(l), x, b, V(l).C, r, s); }
     void restrict_res(int lf, const T* fine, T* crse) {
         ael::k_restrict<T><<<nblk8(L(lf + 1).L.N), kBlock, 0, s>>>(fine, L(lf).L, crse, L(lf + 1).L, 3);
+    }
+    // restriction into V(lf + 1).res that also writes that level's first
+    // Jacobi sweep from zero into its tmp
+    void restrict_res_fz(int lf, const T* fine) {
+        Vecs<T>& c = V(lf + 1);
+        ael::k_restrict_fz<T><<<nblk8(L(lf + 1).L.N), kBlock, 0, s>>>(fine, L(lf).L, c.res, L(lf + 1).L, 3, c.diag,
+                                                                      (T)h->omega, c.tmp);
     }
     void interp(int lf, T* fine, const T* crse, int zero_first) {
         ael::k_interp<T><<<nblk8(L(lf).L.N), kBlock, 0, s>>>(crse, L(lf + 1).L, fine, L(lf).L, 3, zero_first);
@@ -470,11 +480,13 @@ struct Cycle {
     // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.
     void vcycle(int top) {
         const int bot = this->bot();
+        const bool fz = fz_ready();
         for (int l = top; l < bot; ++l) {
             Vecs<T>& v = V(l);
-            smooth_from_zero(l, v.cor, v.res, nu1);
+            smooth_from_zero(l, v.cor, v.res, nu1, fz && l > top);
             corr_res(l, v.rescor, v.cor, v.res);
-            restrict_res(l, v.rescor, V(l + 1).res);
+            if (fz && l + 1 < bot) restrict_res_fz(l, v.rescor);
+            else restrict_res(l, v.rescor, V(l + 1).res);
         }
         bottom();
         for (int l = bot - 1; l >= top; --l) {
@@ -505,12 +517,18 @@ struct Cycle {
         bottom();
         for (int l = bot - 1; l >= 0; --l) {
             Vecs<T>& v = V(l);
-            interp(l, v.cor, V(l + 1).cor, 1);
+            // the level below ran a V-cycle on top of its held correction:
+            // the interpolation reads their sum (the sum is not needed after)
+            if (l + 1 < bot)
+                ael::k_interp_sum<T><<<nblk8(L(l).L.N), kBlock, 0, s>>>(V(l + 1).cor, V(l + 1).cor_hold, L(l + 1).L,
+                                                                        v.cor, L(l).L, 3, 1);
+            else
+                interp(l, v.cor, V(l + 1).cor, 1);
             corr_res(l, v.rescor, v.cor, v.res);
             std::swap(v.res, v.rescor);
             std::swap(v.cor, v.cor_hold);
             vcycle(l);
-            launch_axpy<T>(v.cor, T(1), v.cor_hold, M(l), s);
+            if (l == 0) launch_axpy<T>(v.cor, T(1), v.cor_hold, M(l), s);
         }
     }
 

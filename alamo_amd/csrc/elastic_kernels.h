@@ -629,6 +629,22 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const T* __restrict__ fine,
     if (cid < Lc.N) restrict_node<T>(fine, Lf, crse, Lc, ncomp, cid);
 }
 
+// Restriction followed by the first Jacobi sweep from zero on the coarse
+// level, tmp = w crse / D (k_jacobi_from_zero's arithmetic on the restricted
+// values), in one launch.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_restrict_fz(const T* __restrict__ fine, Lvl Lf, T* __restrict__ crse,
+                                                        Lvl Lc, int ncomp, const T* __restrict__ diag, T omega,
+                                                        T* __restrict__ tmp) {
+    const long cid = xcd_node();
+    if (cid >= Lc.N) return;
+    restrict_node<T>(fine, Lf, crse, Lc, ncomp, cid);
+    for (int n = 0; n < ncomp; ++n) {
+        const long q = n * Lc.N + cid;
+        tmp[q] = omega * crse[q] / diag[q];
+    }
+}
+
 // Nodal injection crse(I,J,K) = fine(2I,2J,2K): amrex::average_down on nodal
 // data, which MLLinOp::avgDownResMG applies to the residual in the F-cycle.
 template <typename T>
@@ -671,9 +687,9 @@ __global__ void __launch_bounds__(kBlock) k_inject_all(InjectLevels<T> lv, int n
 
 // Trilinear prolongation, fine += P crse (src/Operator/Operator.cpp:399-428).
 // With zero_first the fine field is overwritten instead (cor = P cor_crse).
-template <typename T>
-__device__ __forceinline__ void interp_node(const T* __restrict__ crse, const Lvl& Lc, T* __restrict__ fine,
-                                            const Lvl& Lf, int ncomp, int zero_first, long id) {
+template <typename T, typename F>
+__device__ __forceinline__ void interp_node_of(F crse_at, const Lvl& Lc, T* __restrict__ fine, const Lvl& Lf,
+                                               int ncomp, int zero_first, long id) {
     int i, j, k;
     node_ijk(id, Lf, i, j, k);
     const int I = i >> 1, J = j >> 1, K = k >> 1;
@@ -681,26 +697,43 @@ __device__ __forceinline__ void interp_node(const T* __restrict__ crse, const Lv
     const long sy = Lc.nx, sz = (long)Lc.nx * Lc.ny;
     const long c0 = (long)K * sz + (long)J * sy + I;
     for (int n = 0; n < ncomp; ++n) {
-        const T* Cc = crse + n * Lc.N + c0;
+        const long q = n * Lc.N + c0;
+        auto Cc = [&](long o) { return crse_at(q + o); };
         T v;
-        if (!oi && !oj && !ok) v = Cc[0];
-        else if (!oj && !ok) v = T(0.5) * (Cc[0] + Cc[1]);
-        else if (!ok && !oi) v = T(0.5) * (Cc[0] + Cc[sy]);
-        else if (!oi && !oj) v = T(0.5) * (Cc[0] + Cc[sz]);
-        else if (!oi) v = T(0.25) * (Cc[0] + Cc[sy] + Cc[sz] + Cc[sy + sz]);
-        else if (!oj) v = T(0.25) * (Cc[0] + Cc[sz] + Cc[1] + Cc[1 + sz]);
-        else if (!ok) v = T(0.25) * (Cc[0] + Cc[1] + Cc[sy] + Cc[1 + sy]);
+        if (!oi && !oj && !ok) v = Cc(0);
+        else if (!oj && !ok) v = T(0.5) * (Cc(0) + Cc(1));
+        else if (!ok && !oi) v = T(0.5) * (Cc(0) + Cc(sy));
+        else if (!oi && !oj) v = T(0.5) * (Cc(0) + Cc(sz));
+        else if (!oi) v = T(0.25) * (Cc(0) + Cc(sy) + Cc(sz) + Cc(sy + sz));
+        else if (!oj) v = T(0.25) * (Cc(0) + Cc(sz) + Cc(1) + Cc(1 + sz));
+        else if (!ok) v = T(0.25) * (Cc(0) + Cc(1) + Cc(sy) + Cc(1 + sy));
         else
-            v = T(0.125) * (Cc[0] + Cc[1] + Cc[sy] + Cc[sz] + Cc[sy + sz] + Cc[1 + sz] + Cc[1 + sy] + Cc[1 + sy + sz]);
+            v = T(0.125) * (Cc(0) + Cc(1) + Cc(sy) + Cc(sz) + Cc(sy + sz) + Cc(1 + sz) + Cc(1 + sy) + Cc(1 + sy + sz));
         T* out = fine + n * Lf.N + id;
         *out = zero_first ? v : *out + v;
     }
+}
+template <typename T>
+__device__ __forceinline__ void interp_node(const T* __restrict__ crse, const Lvl& Lc, T* __restrict__ fine,
+                                            const Lvl& Lf, int ncomp, int zero_first, long id) {
+    interp_node_of<T>([crse](long q) { return crse[q]; }, Lc, fine, Lf, ncomp, zero_first, id);
 }
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_interp(const T* __restrict__ crse, Lvl Lc, T* __restrict__ fine, Lvl Lf,
                                                    int ncomp, int zero_first) {
     const long id = xcd_node();
     if (id < Lf.N) interp_node<T>(crse, Lc, fine, Lf, ncomp, zero_first, id);
+}
+
+// k_interp of the coarse sum crse + held, without forming it: the value
+// k_axpy(crse, 1, held) would store, rounded the same way, at every read.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_interp_sum(const T* __restrict__ crse, const T* __restrict__ held, Lvl Lc,
+                                                       T* __restrict__ fine, Lvl Lf, int ncomp, int zero_first) {
+    const long id = xcd_node();
+    if (id < Lf.N)
+        interp_node_of<T>([crse, held](long q) { return crse[q] + T(1) * held[q]; }, Lc, fine, Lf, ncomp, zero_first,
+                          id);
 }
 
 template <typename T>

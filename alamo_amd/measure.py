@@ -1,22 +1,23 @@
 """Measurements of the secondary configurations (the headline one is bench.py).
 
-  python -m alamo_amd.measure --config 5 [--fields 4]   257^3 polycrystals, 256 grains
-  python -m alamo_amd.measure --config 3                 3-level AMR, 65^3 base
+  python -m alamo_amd.measure --workload polycrystal257 [--fields 32]   257^3 polycrystals, 256 grains
+  python -m alamo_amd.measure --workload amr3                            3-level AMR, 65^3 base
 
-Config 5: unseen seeded Voronoi fields (256 grains, cubic C11=1.7, C12=1.2,
+polycrystal257: unseen seeded Voronoi fields (256 grains, cubic C11=1.7, C12=1.2,
 C44=0.75, Haar orientations, uniaxial 1e-3 along z) at 257^3 nodes solved to
 1e-8 with the plain FP64 MLMG (V and F cycles), the FP32 path (FP32 F-cycle
 inside FP64 FGMRES) and the learned + FP32 path (the trained network on the
 33^3 level, judged initial correction); times, iterations and the relative
 L2 error against an FP64 solve to 1e-10.
 
-Config 3: three seeded spherical inclusions (r ~ U[0.05, 0.15], E ~
+amr3: three seeded spherical inclusions (r ~ U[0.05, 0.15], E ~
 U[2, 20], dilatational eigenstrain 1e-3, interfaces two base cells wide),
 zero-displacement faces, base 65^3 nodes, two patch levels from the
-indicator-gradient tags; composite V-cycle solve to 1e-8 in FP64 and with
-FP32 base cycles.
+indicator-gradient tags (clustered into several patches per level);
+composite V-cycle solve to 1e-8 in FP64 and with FP32 base cycles.
 
-Results go to stdout as one JSON object per configuration.
+Results go to stdout as one JSON object per workload; with --fields the
+polycrystal run also prints a summary over the fields (medians and ranges).
 """
 from __future__ import annotations
 
@@ -42,7 +43,7 @@ def _timed(fn):
     return out, time.perf_counter() - t
 
 
-def config5(fields=4, n_cell=256, weights=None):
+def polycrystal257(fields=4, n_cell=256, weights=None):
     import torch
 
     from . import bc as B
@@ -87,7 +88,29 @@ def config5(fields=4, n_cell=256, weights=None):
         rows.append(row)
         del op, C
         torch.cuda.empty_cache()
-    return dict(config=5, workload=f"polycrystal-{n_cell + 1}^3-256g uniaxial 1e-3 z, to 1e-8", fields=rows)
+    return dict(workload=f"polycrystal-{n_cell + 1}^3-256g uniaxial 1e-3 z, to 1e-8", fields=rows,
+                summary=_summary(rows))
+
+
+def _summary(rows):
+    """Median, min and max over the fields of each path's time, iterations and
+    error against the FP64 solve."""
+    out = {}
+    for name in ("plain_fp64_V", "plain_fp64_F", "fp32", "learned_fp32"):
+        got = [r[name] for r in rows if name in r]
+        if not got:
+            continue
+        out[name] = {k: dict(median=float(np.median([g[k] for g in got])), min=float(min(g[k] for g in got)),
+                             max=float(max(g[k] for g in got)))
+                     for k in ("seconds", "iters", "rel_l2_err_vs_fp64")}
+        out[name]["converged"] = sum(bool(g["converged"]) for g in got)
+        out[name]["fields"] = len(got)
+    if "plain_fp64_V" in out:
+        for name in ("fp32", "learned_fp32"):
+            if name in out:
+                out[name]["speedup_vs_plain_fp64_V_median"] = (out["plain_fp64_V"]["seconds"]["median"] /
+                                                               out[name]["seconds"]["median"])
+    return out
 
 
 def _amr_models(ops, spheres, width):
@@ -106,11 +129,13 @@ def _amr_models(ops, spheres, width):
     return bs
 
 
-def config3(n0=64, seed=0, shift=0.03):
-    """3-level AMR with patches from the device refinement tags, solved in FP64
-    and with FP32 base cycles; then the inclusions move by `shift` along x and
-    the hierarchy is regridded (new tags, FillPatch of the FP64 solution) and
-    solved again from the filled fields and from zero."""
+def amr3(n0=64, seed=0, shift=0.03, grid_eff=0.7, blocking_factor=8):
+    """3-level AMR with patches from the device refinement tags, clustered
+    into boxes (Berger-Rigoutsos with grid_eff and blocking_factor; None: one
+    box per level), solved in FP64 and with FP32 base cycles; then the
+    inclusions move by `shift` along x and the hierarchy is regridded (new
+    tags, FillPatch of the FP64 solution) and solved again from the filled
+    fields and from zero."""
     import torch
 
     from . import bc as B
@@ -126,12 +151,18 @@ def config3(n0=64, seed=0, shift=0.03):
 
     base = el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC)
     base.SetBC(B.Constant())
-    ops, dt_tag = _timed(lambda: el.amr_levels(base, eta_of(spheres), levels=3))
+    kw = dict(levels=3, blocking_factor=blocking_factor if grid_eff is not None else 0, grid_eff=grid_eff)
+    ops, dt_tag = _timed(lambda: el.amr_levels(base, eta_of(spheres), **kw))
     bs = _amr_models(ops, spheres, width)
-    out = dict(config=3, workload=f"3-level AMR, base {n0 + 1}^3, 3 inclusions (seed {seed})",
+    out = dict(workload=f"3-level AMR, base {n0 + 1}^3, 3 inclusions (seed {seed})", grid_eff=grid_eff,
+               blocking_factor=kw["blocking_factor"],
                spheres=[dict(center=c, radius=r, E=E) for c, r, E in spheres],
                patches=[dict(clo=list(op.clo), chi=list(op.chi)) for op in ops[1:]],
                level_nodes=[int(np.prod(op.level_shape(0))) for op in ops], tag_and_refine_seconds=dt_tag)
+    depth = [0] * len(ops)
+    for q, op in enumerate(ops[1:], 1):
+        depth[q] = depth[ops.index(op.parent)] + 1
+    out["nodes_per_level"] = [sum(n for n, d in zip(out["level_nodes"], depth) if d == lev) for lev in range(3)]
     sols = {}
     for prec in ("fp64", "fp32"):
         xs = [torch.zeros_like(b) for b in bs]
@@ -139,12 +170,12 @@ def config3(n0=64, seed=0, shift=0.03):
                                                abort_on_fail=False))
         sols[prec] = xs
         out[prec] = dict(seconds=dt, iters=info.iters, converged=info.status == 0, res=info.res)
-        _log(f"config 3 {prec}: {dt:.3f}s, {info.iters} composite V-cycles, converged={info.status == 0}")
+        _log(f"3-level AMR {prec}: {dt:.3f}s, {info.iters} composite V-cycles, converged={info.status == 0}")
     out["fp32_vs_fp64_rel_l2"] = [float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
                                   for a, b in zip(sols["fp32"], sols["fp64"])]
     # the inclusions move: regrid from new tags, fill from the old solution
     moved = [((c[0] + shift, c[1], c[2]), r, E) for c, r, E in spheres]
-    (ops2, xs2), dt_regrid = _timed(lambda: el.amr_regrid(ops, sols["fp64"], eta_of(moved), levels=3))
+    (ops2, xs2), dt_regrid = _timed(lambda: el.amr_regrid(ops, sols["fp64"], eta_of(moved), **kw))
     bs2 = _amr_models(ops2, moved, width)
     reg = dict(shift=shift, patches=[dict(clo=list(op.clo), chi=list(op.chi)) for op in ops2[1:]],
                regrid_seconds=dt_regrid)
@@ -154,7 +185,7 @@ def config3(n0=64, seed=0, shift=0.03):
                                                abort_on_fail=False))
         reg[name] = dict(seconds=dt, iters=info.iters, converged=info.status == 0, res=info.res, res0=info.res0,
                          rhsnorm=info.rhsnorm)
-        _log(f"config 3 regrid, from {name}: {dt:.3f}s, {info.iters} composite V-cycles, "
+        _log(f"3-level AMR regrid, from {name}: {dt:.3f}s, {info.iters} composite V-cycles, "
              f"|r0| {info.res0:.3e}, |b| {info.rhsnorm:.3e}")
     out["regrid"] = reg
     return out
@@ -162,12 +193,14 @@ def config3(n0=64, seed=0, shift=0.03):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--config", type=int, choices=(3, 5), required=True)
+    ap.add_argument("--workload", choices=("polycrystal257", "amr3"), required=True)
     ap.add_argument("--fields", type=int, default=4)
+    ap.add_argument("--one-box", action="store_true", help="amr3: one box per level (no clustering)")
     ap.add_argument("--weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
                                                       "unet_polycrystal129_level2.npz"))
     a = ap.parse_args(argv)
-    r = config5(a.fields, weights=a.weights) if a.config == 5 else config3()
+    r = (polycrystal257(a.fields, weights=a.weights) if a.workload == "polycrystal257"
+         else amr3(grid_eff=None if a.one_box else 0.7))
     print(json.dumps(r), flush=True)
     return 0
 

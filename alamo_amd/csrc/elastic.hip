@@ -367,6 +367,9 @@ struct Cycle {
     double bottom_tol;
     int bottom_solver = ALAMO_EL_BOTTOM_BICGSTAB;
     int smoother = ALAMO_EL_SMOOTH_JACOBI;
+    // F-cycle: leave the finest level's correction as cor + cor_hold for the
+    // caller to sum (it reads the two in one pass)
+    bool leave_held = false;
 
     int nlev() const { return (int)h->lv.size(); }
     Level& L(int l) { return h->lv[l]; }
@@ -528,7 +531,7 @@ struct Cycle {
             std::swap(v.res, v.rescor);
             std::swap(v.cor, v.cor_hold);
             vcycle(l);
-            if (l == 0) launch_axpy<T>(v.cor, T(1), v.cor_hold, M(l), s);
+            if (l == 0 && !leave_held) launch_axpy<T>(v.cor, T(1), v.cor_hold, M(l), s);
         }
     }
 
@@ -1811,6 +1814,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                       o.smoother};
     Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
                      o.smoother};
+    c64.leave_held = c32.leave_held = true;  // Z_j takes cor + cor_hold in its copy
     std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
     int rc;
     iters = 0;
@@ -1829,24 +1833,31 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 1, &beta2, s))) return rc;
         const double beta = std::sqrt(beta2);
         if (beta == 0.0) { resnorm = 0.0; return 0; }
-        // V0 = r / beta
-        ael::k_scale_copy<double><<<nblk(M0), kBlock, 0, s>>>(l0.d.res, 1.0 / beta, h->kv, M0);
         const double inf_at_restart = resnorm;
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int j = 0;
+        double hn_prev = 0.0;
         for (; j < m && iters < o.max_iter; ++j) {
             double* Vj = h->kv + (long)j * MP;
             double* Zj = h->kz + (long)j * MP;
-            // Z_j = cycle(V_j)
+            // V_j = r / beta (j = 0) or w / |w| of the last step, written with
+            // its copy in the cycle's precision as the cycle's right-hand side
+            const double* src = j == 0 ? l0.d.res : h->kw;
+            const double scale = j == 0 ? 1.0 / beta : (hn_prev > 0 ? 1.0 / hn_prev : 0.0);
+            // Z_j = cycle(V_j); an F-cycle's last correction sum is formed by
+            // the copy into Z_j
+            const bool held = o.cycle == ALAMO_EL_CYCLE_F;
             if (fp32) {
-                ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(Vj, l0.f.res, M0);
+                ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0);
                 c32.run(o.cycle);
-                ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, Zj, M0);
+                if (held) ael::k_cast_sum<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, l0.f.cor_hold, Zj, M0);
+                else ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, Zj, M0);
             } else {
-                ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(Vj, l0.d.res, M0);
+                ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0);
                 c64.run(o.cycle);
-                ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, Zj, M0);
+                if (held) ael::k_cast_sum<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, l0.d.cor_hold, Zj, M0);
+                else ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, Zj, M0);
             }
             // w = A Z_j
             launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);
@@ -1904,8 +1915,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             }
             const double hn = std::sqrt(hn2);
             H[(j + 1) * m + j] = hn;
-            double* Vn = h->kv + (long)(j + 1) * MP;
-            ael::k_scale_copy<double><<<nblk(M0), kBlock, 0, s>>>(h->kw, hn > 0 ? 1.0 / hn : 0.0, Vn, M0);
+            hn_prev = hn;  // V_{j+1} = w / hn is written by the next step
             for (int i = 0; i < j; ++i) {
                 const double a = H[i * m + j], c = H[(i + 1) * m + j];
                 H[i * m + j] = cs[i] * a + sn[i] * c;

@@ -795,6 +795,27 @@ __global__ void __launch_bounds__(kBlock) k_cast(const Tin* __restrict__ a, Tout
     if (id < n) b[id] = (Tout)a[id];
 }
 
+// y = a x (FP64) and yc = y in the cycle's precision, one pass (k_scale_copy
+// then k_cast); yc may be x itself.
+template <typename Tc>
+__global__ void __launch_bounds__(kBlock) k_scale_copy_cast(const double* x, double a, double* __restrict__ y, Tc* yc,
+                                                            long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) {
+        const double v = a * x[id];
+        y[id] = v;
+        yc[id] = (Tc)v;
+    }
+}
+
+// out = (Tout)(a + 1 b): k_axpy(a, 1, b) then k_cast, one pass.
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(kBlock) k_cast_sum(const Tin* __restrict__ a, const Tin* __restrict__ b,
+                                                     Tout* __restrict__ out, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) out[id] = (Tout)(a[id] + Tin(1) * b[id]);
+}
+
 // y += (double) x  (FP64 correction accumulation of an FP32 correction)
 __global__ void __launch_bounds__(kBlock) k_add_f32_to_f64(double* __restrict__ y, const float* __restrict__ x,
                                                            long n) {

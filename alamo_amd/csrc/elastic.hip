@@ -541,6 +541,16 @@ struct Cycle {
     }
 };
 
+// The cycle of outer iteration it: MLMG runs F-cycles while it < max_fmg_iter
+// (every iteration when max_fmg_iter < 0), V-cycles after; a V-cycle solve
+// stays on V-cycles.
+int cycle_of(const alamo_el_opts& o, int it) {
+    return o.cycle == ALAMO_EL_CYCLE_F && (o.max_fmg_iter < 0 || it < o.max_fmg_iter) ? ALAMO_EL_CYCLE_F
+                                                                                       : ALAMO_EL_CYCLE_V;
+}
+// Iteration limit: fixed_iter > 0 caps max_iter.
+int iter_limit(const alamo_el_opts& o) { return o.fixed_iter > 0 ? std::min(o.max_iter, o.fixed_iter) : o.max_iter; }
+
 int alloc(void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return fail(ALAMO_EL_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -1198,6 +1208,8 @@ void alamo_el_default_opts(alamo_el_opts* o) {
     o->verbose = 0;
     o->bottom_solver = ALAMO_EL_BOTTOM_BICGSTAB;
     o->smoother = ALAMO_EL_SMOOTH_JACOBI;
+    o->max_fmg_iter = -1;
+    o->fixed_iter = -1;
 }
 
 int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_coarsening_level, alamo_el_t* out) {
@@ -1741,7 +1753,7 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
                           o.smoother};
         Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
                          o.smoother};
-        for (int it = 0; it < o.max_iter; ++it) {
+        for (int it = 0; it < iter_limit(o); ++it) {
             if (learned_pre && it > 0) {
                 // learned coarse correction between cycles, judged like the guess
                 int taken;
@@ -1752,10 +1764,10 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
             if (o.precision == ALAMO_EL_PREC_FP32_IR) {
                 // FP32 cycle on the FP64 residual; FP64 correction accumulation
                 ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(l0.d.res, l0.f.res, M0);
-                c32.run(o.cycle);
+                c32.run(cycle_of(o, it));
                 ael::k_add_f32_to_f64<<<nblk(M0), kBlock, 0, s>>>(x, l0.f.cor, M0);
             } else {
-                c64.run(o.cycle);
+                c64.run(cycle_of(o, it));
                 launch_axpy<double>(x, 1.0, l0.d.cor, M0, s);
             }
             launch_residual<double>(h, l0, x, b, l0.d.C, l0.d.res, s);
@@ -1767,6 +1779,7 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
         }
     }
     HIPCHK(hipGetLastError());
+    if (status == ALAMO_EL_NOT_CONVERGED && o.fixed_iter > 0 && iters >= iter_limit(o)) status = 0;
     if (info) {
         info->status = status;
         info->learned_steps = learned_taken;
@@ -1820,7 +1833,8 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
     iters = 0;
     // r = b - A x is in l0.d.res on entry
     bool first = true;
-    while (iters < o.max_iter) {
+    const int limit = iter_limit(o);
+    while (iters < limit) {
         if ((o.learned & ALAMO_EL_LEARNED_PRECOND) && h->nn && !first) {
             // learned coarse correction at each restart, judged like the guess
             int taken;
@@ -1838,7 +1852,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         g[0] = beta;
         int j = 0;
         double hn_prev = 0.0;
-        for (; j < m && iters < o.max_iter; ++j) {
+        for (; j < m && iters < limit; ++j) {
             double* Vj = h->kv + (long)j * MP;
             double* Zj = h->kz + (long)j * MP;
             // V_j = r / beta (j = 0) or w / |w| of the last step, written with
@@ -1847,15 +1861,16 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             const double scale = j == 0 ? 1.0 / beta : (hn_prev > 0 ? 1.0 / hn_prev : 0.0);
             // Z_j = cycle(V_j); an F-cycle's last correction sum is formed by
             // the copy into Z_j
-            const bool held = o.cycle == ALAMO_EL_CYCLE_F;
+            const int cyc = cycle_of(o, iters);
+            const bool held = cyc == ALAMO_EL_CYCLE_F;
             if (fp32) {
                 ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0);
-                c32.run(o.cycle);
+                c32.run(cyc);
                 if (held) ael::k_cast_sum<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, l0.f.cor_hold, Zj, M0);
                 else ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, Zj, M0);
             } else {
                 ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0);
-                c64.run(o.cycle);
+                c64.run(cyc);
                 if (held) ael::k_cast_sum<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, l0.d.cor_hold, Zj, M0);
                 else ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, Zj, M0);
             }
@@ -2317,7 +2332,7 @@ int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const doub
             launch_jacobi<double>(hs[i], lv, src, lv.d.res, lv.d.diag, lv.d.C, dst, s);
         }
     };
-    for (int it = 0; it < o.max_iter && status != 0; ++it) {
+    for (int it = 0; it < iter_limit(o) && status != 0; ++it) {
         // down: pre-smooth each patch on its composite residual (deepest
         // first), reflux the remaining residual into its parent
         for (int i : A.order) {
@@ -2334,10 +2349,10 @@ int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const doub
         // base level: one multigrid cycle on its composite residual
         if (fp32) {
             ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(A.L(0).d.res, A.L(0).f.res, M0);
-            c32.run(o.cycle);
+            c32.run(cycle_of(o, it));
             ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(A.L(0).f.cor, A.L(0).d.cor, M0);
         } else {
-            c64.run(o.cycle);
+            c64.run(cycle_of(o, it));
         }
         // up: add the interpolated parent correction, post-smooth (parents first)
         for (int u = nlev - 1; u >= 0; --u) {
@@ -2356,6 +2371,7 @@ int alamo_amr_solve(int nlev, const alamo_el_t* hs, double* const* x, const doub
         if (nrm <= target) status = 0;
     }
     HIPCHK(hipGetLastError());
+    if (status == ALAMO_EL_NOT_CONVERGED && o.fixed_iter > 0 && iters >= iter_limit(o)) status = 0;
     if (info) {
         info->status = status;
         info->iters = iters;

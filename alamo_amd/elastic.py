@@ -44,7 +44,8 @@ class Opts(ctypes.Structure):
                 ("cycle", ctypes.c_int), ("bottom_max_iter", ctypes.c_int), ("final_smooth", ctypes.c_int),
                 ("omega", ctypes.c_double), ("bottom_tol_rel", ctypes.c_double), ("precision", ctypes.c_int),
                 ("outer", ctypes.c_int), ("restart", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("bottom_solver", ctypes.c_int), ("learned", ctypes.c_int), ("smoother", ctypes.c_int)]
+                ("bottom_solver", ctypes.c_int), ("learned", ctypes.c_int), ("smoother", ctypes.c_int),
+                ("max_fmg_iter", ctypes.c_int), ("fixed_iter", ctypes.c_int)]
 
 
 class Info(ctypes.Structure):
@@ -600,7 +601,8 @@ class Linear:
 
     def __init__(self, op: ElasticOperator, max_iter=200, pre_smooth=2, post_smooth=2, bottom_max_iter=200,
                  final_smooth=8, bottom_tol_rel=1e-4, omega=2.0 / 3.0, cycle="F", precision="fp64", outer="mlmg",
-                 restart=30, verbose=0, abort_on_fail=True, bottom_solver="bicgstab", learned="", smoother="jacobi"):
+                 restart=30, verbose=0, abort_on_fail=True, bottom_solver="bicgstab", learned="", smoother="jacobi",
+                 max_fmg_iter=-1, fixed_iter=-1):
         self.op = op
         o = default_opts()
         o.max_iter, o.nu1, o.nu2 = int(max_iter), int(pre_smooth), int(post_smooth)
@@ -612,6 +614,7 @@ class Linear:
         o.restart, o.verbose = int(restart), int(verbose)
         o.bottom_solver = {"bicgstab": 0, "direct": 1}[bottom_solver]
         o.smoother = {"jacobi": 0, "rbgs": 1}[smoother]
+        o.max_fmg_iter, o.fixed_iter = int(max_fmg_iter), int(fixed_iter)
         o.learned = {"": 0, "guess": LEARNED_GUESS, "precond": LEARNED_PRECOND,
                      "guess+precond": LEARNED_GUESS | LEARNED_PRECOND}[learned]
         self.opts = o

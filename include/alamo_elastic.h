@@ -111,6 +111,12 @@ typedef struct {
     int bottom_solver;     /* ALAMO_EL_BOTTOM_BICGSTAB */
     int learned;           /* 0, or ALAMO_EL_LEARNED_* bits (needs alamo_el_set_learned) */
     int smoother;          /* ALAMO_EL_SMOOTH_JACOBI */
+    int max_fmg_iter;      /* -1: every cycle an F-cycle (with cycle F; setMaxFmgIter(1e8),
+                              Linear.H:261); k >= 0: F-cycles for the first k iterations,
+                              V-cycles after (max_fmg_iter, Linear.H:266,305) */
+    int fixed_iter;        /* -1; k > 0: at most k iterations, and a solve that has not
+                              converged by then returns 0 with info.res its residual
+                              (fixed_iter, "exiting gracefully", Linear.H:267,312) */
 } alamo_el_opts;
 
 typedef struct {

@@ -730,6 +730,11 @@ int ora_mlmg_solve(const int n[3], const double dx[3], int sym, const double* C,
     mg.nuf = opts[5];
     const int max_coarsening = opts[6];
     mg.fcycle_inject = opts[7];
+    // MLMG::setMaxFmgIter / setFixedIter (src/Solver/Nonlocal/Linear.H:266-267):
+    // F-cycles while iter < max_fmg_iter (always when < 0, the reference's
+    // 1e8), V-cycles after; fixed_iter > 0 caps the iterations and a solve
+    // that has not converged by then ends without failure
+    const int max_fmg_iter = opts[8], fixed_iter = opts[9];
     const double tol_rel = ropts[0], tol_abs = ropts[1];
     mg.omega = ropts[2];
     mg.bottom_tol_rel = ropts[3];
@@ -777,8 +782,9 @@ int ora_mlmg_solve(const int n[3], const double dx[3], int sym, const double* C,
     if (out_hist) out_hist[0] = resnorm0;
     if (out_iters) *out_iters = 0;
     if (resnorm0 <= res_target) return 0;
-    for (int iter = 0; iter < max_iter; ++iter) {
-        if (mg.cycle == 0) mg.fcycle();
+    const int limit = fixed_iter > 0 ? std::min(max_iter, fixed_iter) : max_iter;
+    for (int iter = 0; iter < limit; ++iter) {
+        if (mg.cycle == 0 && (max_fmg_iter < 0 || iter < max_fmg_iter)) mg.fcycle();
         else mg.vcycle(0);
         for (long q = 0; q < m0; ++q) x[q] += mg.cor[0][q];
         residual(mg.op(0), x, b, mg.res[0].data());
@@ -788,7 +794,7 @@ int ora_mlmg_solve(const int n[3], const double dx[3], int sym, const double* C,
         if (out_res) out_res[0] = nrm;
         if (nrm <= res_target) return 0;
     }
-    return 1;
+    return fixed_iter > 0 ? 0 : 1;
 }
 
 }  // extern "C"

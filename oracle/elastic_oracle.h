@@ -75,6 +75,9 @@ int ora_stress(const int n[3], const double dx[3], int sym, const double* C,
  *   opts[6] max_coarsening_level (-1 = until 2 cells)
  *   opts[7] F-cycle residual transfer: 1 nodal injection (amrex::average_down,
  *           the MLLinOp::avgDownResMG default), 0 full-weighting restriction
+ *   opts[8] max_fmg_iter (-1: F-cycles throughout; k: F for the first k
+ *           iterations, V after)      opts[9] fixed_iter (-1; k > 0: at most
+ *           k iterations, not converged by then returns 0)
  *   ropts[0] tol_rel  ropts[1] tol_abs  ropts[2] omega (2/3)
  *   ropts[3] bottom_tol_rel (1e-4)
  * out: out_iters, out_res[0] final residual inf-norm, out_res[1] initial,

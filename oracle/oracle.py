@@ -153,12 +153,12 @@ def stress(u, C, sym, dx, F0=None):
 
 def mlmg_solve(b, C, sym, bc, dx, x0=None, tol_rel=1e-10, tol_abs=0.0, max_iter=200, nu1=2, nu2=2,
                cycle="F", bottom_max_iter=200, final_smooth=8, max_coarsening_level=-1, omega=2.0 / 3.0,
-               bottom_tol_rel=1e-4, uniform=False, fcycle_inject=True):
+               bottom_tol_rel=1e-4, uniform=False, fcycle_inject=True, max_fmg_iter=-1, fixed_iter=-1):
     """Returns (x, info) with info = dict(status, iters, res, res0, rhsnorm, hist)."""
     b, C = _f64(b), _f64(C)
     x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
     opts = np.array([max_iter, nu1, nu2, 0 if cycle == "F" else 1, bottom_max_iter, final_smooth,
-                     max_coarsening_level, int(fcycle_inject)], dtype=np.int32)
+                     max_coarsening_level, int(fcycle_inject), max_fmg_iter, fixed_iter], dtype=np.int32)
     ropts = np.array([tol_rel, tol_abs, omega, bottom_tol_rel], dtype=np.float64)
     iters = ctypes.c_int(0)
     res = np.zeros(3)

@@ -281,3 +281,33 @@ def test_prepare_fp32_ahead_of_the_solve_gives_the_same_bits():
         el.Linear(op, precision="fp32", outer="fgmres", bottom_solver="direct").solve(x, rhs, 1e-8, 0.0)
         xs.append(x)
     assert torch.equal(xs[0], xs[1]) and torch.equal(xs[1], xs[2])
+
+
+@pytest.mark.parametrize("outer", ["mlmg", "fgmres"])
+def test_max_fmg_iter_and_fixed_iter_match_oracle(outer):
+    """Linear::max_fmg_iter (F-cycles for the first k iterations, V-cycles
+    after) and fixed_iter (at most k iterations, no failure;
+    src/Solver/Nonlocal/Linear.H:266-267): the FP64 MLMG solve takes the
+    oracle's iteration count and solution (1e-10); the FGMRES solve with the
+    switching preconditioner reaches the target; fixed_iter ends a solve whose
+    target is out of reach after k iterations, at the oracle's residual."""
+    p = P.polycrystal(16, ngrains=8, seed=4)
+    el, op, rhs = gpu_setup(p)
+    rhs_o = oracle_rhs(p)
+    x = torch.zeros_like(rhs)
+    s = el.Linear(op, outer=outer, max_fmg_iter=2)
+    s.solve(x, rhs, 1e-10, 0.0)
+    xo, info = O.mlmg_solve(rhs_o, p["C"], p["sym"], p["bc"], p["dx"], tol_rel=1e-10, max_fmg_iter=2)
+    if outer == "mlmg":
+        assert s.info.iters == info["iters"]
+        assert rel_l2(host(x), xo) < 1e-10
+    else:
+        assert s.info.status == 0
+        assert op.norm_inf(op.residual(x, rhs)) <= 1e-10 * max(s.info.rhsnorm, s.info.res0) * (1 + 1e-9)
+    if outer == "mlmg":
+        x = torch.zeros_like(rhs)
+        s = el.Linear(op, fixed_iter=3)  # abort_on_fail: a failure would raise
+        s.solve(x, rhs, 1e-300, 0.0)
+        _, info3 = O.mlmg_solve(rhs_o, p["C"], p["sym"], p["bc"], p["dx"], tol_rel=1e-300, fixed_iter=3)
+        assert s.info.status == 0 and s.info.iters == 3 and info3["iters"] == 3
+        assert abs(s.info.res - info3["res"]) <= 1e-6 * info3["res"]

@@ -562,3 +562,38 @@ def test_psi_weight_pins():
     spike[0, 0, 0] = 1.0                      # the corner cell: coarse (0, 0, 0) only, by copy
     r = O.psi_restrict(spike)
     assert r[0, 0, 0] == 1.0 and np.count_nonzero(r) == 1
+
+
+def test_max_fmg_iter_and_fixed_iter_compose_the_cycles():
+    """Linear::max_fmg_iter and fixed_iter (src/Solver/Nonlocal/Linear.H:266-267,
+    305-312) on the oracle: a solve with max_fmg_iter = k is k F-cycles and then
+    the V-cycle solve started from that iterate, with the same absolute target
+    (the same iteration count and residual history); fixed_iter = k stops after
+    k iterations without failure, on the first k steps of the unlimited
+    history.  The oracle's threaded BiCGStab bottom sums in a varying order,
+    so histories agree to 1e-6 relative, not bit for bit."""
+    from alamo_amd import problems
+    p = problems.polycrystal(16, ngrains=8, seed=4)
+    shape = (17,) * 3
+    b = O.bc_init(shape, p["bc"])
+    rhs = O.newton_rhs(np.zeros((3,) + shape), b, p["C"], p["sym"], p["bc"], p["dx"])
+    args = (rhs, p["C"], p["sym"], p["bc"], p["dx"])
+    T = 1e-9
+    k = 2
+    x, info = O.mlmg_solve(*args, tol_rel=0.0, tol_abs=T, max_fmg_iter=k)
+    xf, infof = O.mlmg_solve(*args, tol_rel=0.0, tol_abs=1e-300, cycle="F", fixed_iter=k)
+    assert infof["status"] == 0 and infof["iters"] == k
+    xv, infov = O.mlmg_solve(*args, x0=xf, tol_rel=0.0, tol_abs=T, cycle="V")
+    assert info["status"] == 0 and infov["status"] == 0
+    assert info["iters"] == k + infov["iters"]
+    np.testing.assert_allclose(info["hist"], np.concatenate([infof["hist"], infov["hist"][1:]]), rtol=1e-6)
+    assert np.linalg.norm(x - xv) <= 1e-6 * np.linalg.norm(x)
+    # more F-cycles than needed: the same as F-cycles throughout
+    xa, infoa = O.mlmg_solve(*args, tol_rel=0.0, tol_abs=T, max_fmg_iter=10**6)
+    xb, infob = O.mlmg_solve(*args, tol_rel=0.0, tol_abs=T)
+    assert infoa["iters"] == infob["iters"]
+    assert np.linalg.norm(xa - xb) <= 1e-6 * np.linalg.norm(xb)
+    # fixed_iter on a target out of reach
+    xc, infoc = O.mlmg_solve(*args, tol_rel=0.0, tol_abs=1e-300, fixed_iter=3)
+    assert infoc["status"] == 0 and infoc["iters"] == 3
+    np.testing.assert_allclose(infoc["hist"], infob["hist"][:4], rtol=1e-6)

@@ -167,9 +167,12 @@ __global__ void __launch_bounds__(kBlock) k_patch_inject(const double* __restric
 // stencil falls back to the face, edge and corner weights of the MG
 // restriction (Operator.cpp:280-353), since no fine node exists outside the
 // domain; inside the domain the fine ghost layer supplies the outer nodes.
+// The same restriction of the ncomp planes of a patch's C onto its parent's
+// is Elastic::averageDownCoeffsDifferentAmrLevels (Elastic.cpp:628-745,
+// whose corner, edge, face and interior weights are these).
 __global__ void __launch_bounds__(kBlock) k_patch_reflux(const double* __restrict__ rf, Lvl Lf, Patch P,
                                                          double* __restrict__ rc, Lvl Lc, int pox, int poy, int poz,
-                                                         int dnx, int dny, int dnz) {
+                                                         int dnx, int dny, int dnz, int ncomp) {
     const int nx = P.chi[0] - P.clo[0] + 1, ny = P.chi[1] - P.clo[1] + 1, nz = P.chi[2] - P.clo[2] + 1;
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long)nx * ny * nz) return;
@@ -180,8 +183,7 @@ __global__ void __launch_bounds__(kBlock) k_patch_reflux(const double* __restric
     const long sy = Lf.nx, sz = (long)Lf.nx * Lf.ny;
     const long f0 = ((long)(2 * K - P.oz) * Lf.ny + (2 * J - P.oy)) * Lf.nx + (2 * I - P.ox);
     const long ci = ((long)(K - poz) * Lc.ny + (J - poy)) * Lc.nx + (I - pox);
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
+    for (int p = 0; p < ncomp; ++p) {
         const double* F = rf + p * Lf.N + f0;
         double v;
         if (bi && bj && bk) v = F[0];

@@ -2043,7 +2043,7 @@ struct AmrCtx {
         const long m = (long)(Pi.chi[0] - Pi.clo[0] + 1) * (Pi.chi[1] - Pi.clo[1] + 1) * (Pi.chi[2] - Pi.clo[2] + 1);
         const int* dom = h[q]->amr_dom;
         ael::k_patch_reflux<<<nblk(m), kBlock, 0, s>>>(rf, L(i).L, Pi, rc, L(q).L, P(q).ox, P(q).oy, P(q).oz, dom[0],
-                                                        dom[1], dom[2]);
+                                                        dom[1], dom[2], 3);
     }
     void interp(int i, const double* c, double* f, int mode) {
         const int q = par[i];
@@ -2220,7 +2220,22 @@ int alamo_amr_reflux(alamo_el_t patch, const double* fine_res, double* parent_re
     const long m = (long)(P.chi[0] - P.clo[0] + 1) * (P.chi[1] - P.clo[1] + 1) * (P.chi[2] - P.clo[2] + 1);
     ael::k_patch_reflux<<<nblk(m), kBlock, 0, s>>>(lv.d.rescor, lv.L, P, parent_res, par->lv[0].L, par->amr.ox,
                                                     par->amr.oy, par->amr.oz, par->amr_dom[0], par->amr_dom[1],
-                                                    par->amr_dom[2]);
+                                                    par->amr_dom[2], 3);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int alamo_amr_average_down_coeffs(alamo_el_t patch, const double* patch_C, double* parent_C, hipStream_t s) {
+    if (!patch || !patch_C || !parent_C) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (!patch->amr_parent) return fail(ALAMO_EL_EINVAL, "averaging down needs a patch (alamo_amr_patch_create)");
+    const alamo_el_t par = patch->amr_parent;
+    const ael::Patch& P = patch->amr;
+    const long m = (long)(P.chi[0] - P.clo[0] + 1) * (P.chi[1] - P.clo[1] + 1) * (P.chi[2] - P.clo[2] + 1);
+    // the patch's C planes, ghost layer included, restricted onto the parent's
+    // nodes clo..chi with the reflux weights
+    ael::k_patch_reflux<<<nblk(m), kBlock, 0, s>>>(patch_C, patch->lv[0].L, P, parent_C, par->lv[0].L, par->amr.ox,
+                                                    par->amr.oy, par->amr.oz, par->amr_dom[0], par->amr_dom[1],
+                                                    par->amr_dom[2], patch->nc);
     HIPCHK(hipGetLastError());
     return 0;
 }

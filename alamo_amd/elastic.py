@@ -65,6 +65,7 @@ EXPORTS = [
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
     "alamo_el_tag_bbox", "alamo_amr_fillpatch", "alamo_el_set_psi", "alamo_el_palette_size",
     "alamo_el_newton_rhs_noalloc", "alamo_el_gs_smooth", "alamo_el_tag_cells", "alamo_amr_cluster",
+    "alamo_amr_average_down_coeffs",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -129,6 +130,7 @@ def lib():
         L.alamo_amr_fillpatch.argtypes = [vp, vp, vp, vp, vp, vp]
         L.alamo_el_tag_cells.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, vp]
         L.alamo_amr_reflux.argtypes = [vp, vp, vp, vp]
+        L.alamo_amr_average_down_coeffs.argtypes = [vp, vp, vp, vp]
         i3 = ctypes.POINTER(ctypes.c_int)
         L.alamo_el_box_copy.argtypes = [vp, i3, i3, vp, i3, i3, i3, i3, ctypes.c_int, vp]
         L.alamo_amr_cluster.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
@@ -510,6 +512,22 @@ class ElasticOperator:
         _check(lib().alamo_amr_reflux(self._h, _ptr(fine_res), _ptr(parent_res), _stream()), "alamo_amr_reflux")
         return parent_res
 
+    def average_down_coeffs(self, C, parent_C):
+        """Elastic::averageDownCoeffsDifferentAmrLevels (src/Operator/
+        Elastic.cpp:628-745) of this patch (alamo_amr_average_down_coeffs):
+        parent_C (SoA over the parent's nodes) at the covered and coarse-fine
+        nodes becomes the full weighting of C (SoA over this patch's nodes,
+        ghost layer included).  Set the parent's model from parent_C after."""
+        if self.parent is None:
+            raise AlamoError("average_down_coeffs: this operator is not a patch")
+        for v, shape, name in ((C, self.level_shape(0), "C"), (parent_C, self.parent.level_shape(0), "parent_C")):
+            if tuple(v.shape) != (self.ncomp,) + tuple(shape) or v.dtype != _torch().float64 or not v.is_contiguous():
+                raise AlamoError(f"average_down_coeffs: expected {name} as a contiguous float64 tensor of shape "
+                                 f"{(self.ncomp,) + tuple(shape)}, got {tuple(v.shape)} {v.dtype}")
+        _check(lib().alamo_amr_average_down_coeffs(self._h, _ptr(C), _ptr(parent_C), _stream()),
+               "alamo_amr_average_down_coeffs")
+        return parent_C
+
     def fill_patch(self, parent_u, old=None, old_u=None):
         """This patch's field after a regrid (alamo_amr_fillpatch;
         Integrator::FillPatch / FillCoarsePatch, src/Integrator/Integrator.cpp:
@@ -724,6 +742,20 @@ class Newton(Linear):
         res = Linear.solve(self, du, rhs, tol_rel, tol_abs)
         u += du
         return res
+
+
+def amr_average_down_coeffs(ops, Cs):
+    """Linear's average_down_coeffs over a hierarchy (Elastic::averageDownCoeffs,
+    src/Operator/Elastic.cpp:605-614): every patch's C restricted onto its
+    parent's, the deepest patches first, then every model set again.  ops as
+    amr_levels returns them (parents before children); Cs their SoA C tensors,
+    changed in place."""
+    idx = {id(op): q for q, op in enumerate(ops)}
+    for q in sorted(range(1, len(ops)), key=lambda q: -_depth(ops[q])):
+        ops[q].average_down_coeffs(Cs[q], Cs[idx[id(ops[q].parent)]])
+    for op, C in zip(ops, Cs):
+        op.SetModel(C)
+    return Cs
 
 
 def box_copy(dst, dst_origin, src, src_origin, lo, hi):

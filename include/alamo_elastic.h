@@ -339,6 +339,16 @@ int alamo_amr_cluster(const unsigned char* tags_host, const int ncell[3], const 
  * its model with alamo_el_set_model over the handle's nodes
  * (alamo_amr_patch_info gives the global node index of node 0). */
 int alamo_amr_patch_create(alamo_el_t parent, const int clo[3], const int chi[3], alamo_el_t* out);
+/* Elastic::averageDownCoeffsDifferentAmrLevels (src/Operator/Elastic.cpp:628-745;
+ * Linear's average_down_coeffs, Linear.H:288): the parent's stiffness at its
+ * nodes clo..chi (covered and coarse-fine nodes) becomes the full-weighting
+ * restriction of the patch's, with the corner, edge and face weights on domain
+ * faces.  patch_C: the patch's C over all its nodes (ghost layer included),
+ * parent_C: the parent's, both SoA in the handles' symmetry (device); the
+ * caller then sets the parent's model from parent_C (alamo_el_set_model), as
+ * the reference's prepareForSolve does before averaging the MG levels.  Call
+ * it for the deepest patches first, as the reference loops amrlev down. */
+int alamo_amr_average_down_coeffs(alamo_el_t patch, const double* patch_C, double* parent_C, hipStream_t stream);
 int alamo_amr_patch_info(alamo_el_t h, int origin[3], int ghost_lo[3], int ghost_hi[3]);
 /* Regrid: fill the field dst_u of the new patch dst (every handle node, ghost
  * layer included) as Integrator::FillPatch does for a nodal field

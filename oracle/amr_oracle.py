@@ -12,7 +12,9 @@ Restates, in numpy over the C++ oracle's Fapply/Jacobi/residual/MG cycle:
   - averageDownSolutionRHS, nodal injection (src/Operator/Operator.cpp:436-448);
   - reflux (Operator.cpp:526-633): the parent residual at the covered and
     coarse-fine nodes is replaced by the full-weighting restriction of the
-    patch residual (domain-face weights at domain faces);
+    patch residual (domain-face weights at domain faces); the same
+    restriction of the patch's C is averageDownCoeffsDifferentAmrLevels
+    (Elastic.cpp:628-745);
   - the composite inf-norm with strictly covered parent nodes masked
     (fixUpResidualMask, Operator.cpp:230-251, keeps coarse-fine nodes);
   - one iteration: pre-smooth patches finest first, reflux, one MG cycle on
@@ -179,6 +181,27 @@ def reflux_loops(rf, lev, rc, parent):
                         for dx in dxs:
                             v = v + w1[dx] * w1[dy] * w1[dz] * rf[:, k + dz, j + dy, i + dx]
                 rc[:, K - pz, J - py, I - px] = v / norm
+
+
+def average_down_coeffs(levels):
+    """Elastic::averageDownCoeffs with average_down_coeffs on
+    (src/Operator/Elastic.cpp:605-614,628-745): for every patch, deepest first,
+    the parent's C at the covered and coarse-fine nodes [clo, chi] becomes the
+    nodal full weighting of the patch's C, ghost layer included, with the
+    corner, edge and face weights on domain faces (the reflux restriction on
+    the C planes).  Returns new Levels (C and diagonal updated)."""
+    import copy
+    out = [copy.copy(lv) for lv in levels]
+    for lv in out:
+        lv.C = lv.C.copy()
+    par, kids, order = _tree(out)
+    for i in order:
+        if par[i] is None:
+            continue
+        reflux(out[i].C, out[i], out[par[i]].C, out[par[i]])
+    for lv in out:
+        lv.diag = O.diagonal(lv.C, lv.sym, lv.bc, lv.dx, lv.shape)
+    return out
 
 
 def covered_mask(parent, child):

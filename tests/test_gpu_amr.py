@@ -474,3 +474,33 @@ def test_reflux_entry_point_matches_oracle():
         AO.reflux_loops(rz, lvp, want, lvb)
         assert np.abs(rc_d.cpu().numpy() - want).max() <= 1e-14 * np.abs(want).max()
         assert np.array_equal(rf_d.cpu().numpy(), rf)
+
+
+@pytest.mark.parametrize("boxes", [
+    [((2, 2, 2), (14, 14, 14)), ((8, 8, 8), (20, 20, 20))],       # three AMR levels
+    [((0, 2, 4), (10, 16, 12))],                                   # on the domain faces x lo and y hi
+])
+def test_average_down_coeffs_matches_oracle(boxes):
+    """alamo_amr_average_down_coeffs (Elastic::averageDownCoeffsDifferentAmrLevels,
+    src/Operator/Elastic.cpp:628-745) against the oracle's restatement: the
+    parents' stiffness after the averaging, deepest patch first, to rounding,
+    and the composite iterations of the averaged hierarchy equal the oracle's."""
+    ops, bs, lv = hierarchy(16, boxes, width=1.0 / 16)
+    Cs = [torch.from_numpy(l.C.copy()).cuda() for l in lv]
+    el.amr_average_down_coeffs(ops, Cs)
+    lv_avg = AO.average_down_coeffs(lv)
+    for l in range(len(ops)):
+        e = np.abs(Cs[l].cpu().numpy() - lv_avg[l].C).max() / np.abs(lv_avg[l].C).max()
+        print(f"level {l}: max |device - oracle| / max |C| = {e:.2e}")
+        assert e < 1e-15
+    changed = np.abs(lv_avg[0].C - lv[0].C).max()
+    assert changed > 1e-3  # the averaging did change the parent
+    xs = [torch.zeros_like(b) for b in bs]
+    iters = 4
+    info = el.amr_solve(ops, xs, bs, 1e-300, 0.0, max_iter=iters, abort_on_fail=False)
+    bh = [b.cpu().numpy() for b in bs]
+    xo, io = AO.amr_solve(lv_avg, [np.zeros_like(b) for b in bh], bh, 1e-300, max_iter=iters)
+    for l in range(len(ops)):
+        e = np.linalg.norm(xs[l].cpu().numpy() - xo[l]) / np.linalg.norm(xo[l])
+        print(f"level {l}: rel L2 device vs oracle {e:.2e}")
+        assert e < 1e-9

@@ -251,3 +251,38 @@ def test_uniaxial_tension_three_levels_closed_form():
         assert err < 1e-8 * 0.1, (l, err)
         sig = O.stress(xs[l], lv.C, M.ISOTROPIC, h)
         assert np.allclose(sig[0], 210.0 * eps, rtol=1e-8, atol=0), l
+
+
+def test_average_down_coeffs_weights():
+    """averageDownCoeffsDifferentAmrLevels (src/Operator/Elastic.cpp:628-745) on
+    the oracle: a stiffness linear in space comes back unchanged at the parent's
+    covered and coarse-fine nodes (full weighting and its domain-face forms
+    reproduce linears), a fine checkerboard (-1)^(i+j+k) averages to zero at
+    interior nodes (1/8 - 6/16 + 12/32 - 8/64), and the other parent nodes keep
+    their values.  A wrong face, edge or corner weight fails one of the two."""
+    n0 = 8
+    dom = (n0,) * 3
+    for clo, chi in [((2, 2, 2), (6, 6, 6)), ((0, 1, 2), (5, 8, 6))]:   # inside; touching x lo, y hi
+        origin, gl, gh, nc = AO.patch_geometry(dom, clo, chi)
+        shape_f = tuple(nc[a] + 1 for a in (2, 1, 0))
+        z, y, x = np.meshgrid(*[origin[a] + np.arange(shape_f[2 - a]) for a in (2, 1, 0)], indexing="ij")
+        lin = np.stack([1.0 + 0.1 * x + 0.2 * y - 0.05 * z, 2.0 - 0.03 * x + 0.07 * z])
+        chk = np.stack([(-1.0) ** (x + y + z), np.zeros_like(x, dtype=float)])
+        Z, Y, X = np.meshgrid(*[np.arange(n0 + 1)] * 3, indexing="ij")
+        lin_c = np.stack([1.0 + 0.2 * X + 0.4 * Y - 0.1 * Z, 2.0 - 0.06 * X + 0.14 * Z])
+        for Cf, expect in ((lin, "linear"), (chk, "checker")):
+            base = AO.Level(lin_c.copy() if expect == "linear" else np.full((2,) + (n0 + 1,) * 3, 7.0), M.ISOTROPIC,
+                            B.Constant(), 1.0 / n0, dom=dom)
+            pat = AO.Level(Cf, M.ISOTROPIC, AO.patch_bc(B.Constant(), gl, gh), 0.5 / n0, origin=origin, clo=clo,
+                           chi=chi, gl=gl, gh=gh, dom=tuple(2 * d for d in dom))
+            out = AO.average_down_coeffs([base, pat])[0].C
+            sl = tuple(slice(clo[a], chi[a] + 1) for a in (2, 1, 0))
+            inside = np.zeros((n0 + 1,) * 3, bool)
+            inside[sl] = True
+            if expect == "linear":
+                np.testing.assert_allclose(out, lin_c, rtol=0, atol=1e-13)
+            else:
+                Kc, Jc, Ic = np.meshgrid(*[np.arange(n0 + 1)] * 3, indexing="ij")
+                interior = inside & (Ic > 0) & (Ic < n0) & (Jc > 0) & (Jc < n0) & (Kc > 0) & (Kc < n0)
+                assert np.abs(out[0][interior]).max() < 1e-15
+                assert np.all(out[:, ~inside] == 7.0)

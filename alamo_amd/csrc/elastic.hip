@@ -1491,6 +1491,31 @@ int alamo_el_interp_add(alamo_el_t h, int fmglev, double* fine, const double* cr
     return 0;
 }
 
+int alamo_el_level_op32(alamo_el_t h, int mglev, int op, const float* x, const float* b, float* out,
+                        hipStream_t s) {
+    if (bad_level(h, mglev) || !x || !out || op < 0 || op > 4) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if ((op == 1 || op == 2) && !b) return fail(ALAMO_EL_EINVAL, "bad argument");
+    if (op >= 3 && mglev + 1 >= (int)h->lv.size()) return fail(ALAMO_EL_EINVAL, "no coarser level");
+    if (!h->model_set) return fail(ALAMO_EL_ESTATE, "Attempting to use operator before calling SetModel");
+    int rc;
+    if (!h->prepared && (rc = prepare(h, s))) return rc;
+    if ((rc = prepare32(h, s))) return rc;
+    Level& l = h->lv[mglev];
+    switch (op) {
+        case 0: launch_fapply<float>(h, l, x, l.f.C, out, s); break;
+        case 1: launch_residual<float>(h, l, x, b, l.f.C, out, s); break;
+        case 2: launch_jacobi<float>(h, l, x, b, l.f.diag, l.f.C, out, s); break;
+        case 3:
+            ael::k_restrict<float><<<nblk8(h->lv[mglev + 1].L.N), kBlock, 0, s>>>(x, l.L, out, h->lv[mglev + 1].L, 3);
+            break;
+        default:
+            ael::k_interp<float><<<nblk8(l.L.N), kBlock, 0, s>>>(x, h->lv[mglev + 1].L, out, l.L, 3, 0);
+            break;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int alamo_el_bc_init(alamo_el_t h, double* rhs, int homogeneous, hipStream_t s) {
     if (!h || !rhs) return fail(ALAMO_EL_EINVAL, "bad argument");
     const Lvl L = h->lv[0].L;

@@ -65,7 +65,7 @@ EXPORTS = [
     "alamo_amr_patch_info", "alamo_amr_solve", "alamo_el_mix_models", "alamo_el_stress", "alamo_el_face_traction",
     "alamo_el_tag_bbox", "alamo_amr_fillpatch", "alamo_el_set_psi", "alamo_el_palette_size",
     "alamo_el_newton_rhs_noalloc", "alamo_el_gs_smooth", "alamo_el_tag_cells", "alamo_amr_cluster",
-    "alamo_amr_average_down_coeffs",
+    "alamo_amr_average_down_coeffs", "alamo_el_level_op32",
 ]
 LEARNED_GUESS, LEARNED_PRECOND = 1, 2
 
@@ -131,6 +131,7 @@ def lib():
         L.alamo_el_tag_cells.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, vp]
         L.alamo_amr_reflux.argtypes = [vp, vp, vp, vp]
         L.alamo_amr_average_down_coeffs.argtypes = [vp, vp, vp, vp]
+        L.alamo_el_level_op32.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         i3 = ctypes.POINTER(ctypes.c_int)
         L.alamo_el_box_copy.argtypes = [vp, i3, i3, vp, i3, i3, i3, i3, ctypes.c_int, vp]
         L.alamo_amr_cluster.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
@@ -553,6 +554,34 @@ class ElasticOperator:
         if tuple(v.shape) != (3,) + tuple(shape) or v.dtype != _torch().float64 or not v.is_contiguous():
             raise AlamoError(f"{name}: expected a contiguous float64 tensor of shape {(3,) + tuple(shape)}, "
                              f"got {tuple(v.shape)} {v.dtype}")
+
+    OP32 = {"fapply": 0, "residual": 1, "jacobi": 2, "restrict": 3, "interp_add": 4}
+
+    def level_op32(self, op: str, mglev: int, x, b=None, out=None):
+        """One FP32 kernel of the FP32 cycles on MG level mglev
+        (alamo_el_level_op32): "fapply", "residual" (b - A x), "jacobi" (one
+        sweep), "restrict" (x onto mglev + 1), "interp_add" (out += P x, x on
+        mglev + 1).  Float32 CUDA tensors (3, nz, ny, nx)."""
+        torch = _torch()
+        code = self.OP32[op]
+        src_lev = mglev + 1 if code == 4 else mglev
+        dst_lev = mglev + 1 if code == 3 else mglev
+        for v, lev, name in ((x, src_lev, "x"), (b, mglev, "b"), (out, dst_lev, "out")):
+            if v is None:
+                continue
+            want = (3,) + tuple(self.level_shape(lev))
+            if tuple(v.shape) != want or v.dtype != torch.float32 or not v.is_contiguous():
+                raise AlamoError(f"level_op32: expected {name} as a contiguous float32 tensor of shape {want}")
+        if out is None:
+            out = torch.zeros((3,) + tuple(self.level_shape(dst_lev)), dtype=torch.float32, device="cuda")
+        def p32(t):
+            return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p()
+
+        if not x.is_cuda or (b is not None and not b.is_cuda) or not out.is_cuda:
+            raise AlamoError("level_op32: expected CUDA tensors")
+        _check(lib().alamo_el_level_op32(self._h, int(mglev), code, p32(x), p32(b), p32(out), _stream()),
+               "alamo_el_level_op32")
+        return out
 
     def norm_inf(self, v, mglev: int = 0) -> float:
         out = ctypes.c_double()

@@ -187,6 +187,17 @@ int alamo_el_prepare(alamo_el_t h, hipStream_t stream);
  * calls it with alamo_el_prepare to keep the set-up out of the solve. */
 int alamo_el_prepare_fp32(alamo_el_t h, hipStream_t stream);
 
+/* The FP32 kernels of the FP32 cycles, one call each, on MG level mglev with
+ * the level's FP32 copies (C, its tile and boundary copies or palette, the
+ * compact grad-C matrices, the diagonal; made here if needed).  No
+ * counterpart in the reference, which has no FP32 path: for parity tests of
+ * those kernels and for callers driving their own FP32 cycles.  op 0: out =
+ * A x (Fapply); 1: out = b - A x; 2: one damped Jacobi sweep of x (Fsmooth is
+ * two); 3: out = the restriction of x onto level mglev + 1; 4: out += the
+ * interpolation of x from level mglev + 1.  Device float arrays of 3 planes. */
+int alamo_el_level_op32(alamo_el_t h, int mglev, int op, const float* x, const float* b, float* out,
+                        hipStream_t stream);
+
 /* Operator::Elastic::Fapply, f = A u on MG level mglev
  * (src/Operator/Elastic.cpp:139-292). */
 int alamo_el_fapply(alamo_el_t h, int mglev, const double* u, double* f, hipStream_t stream);

@@ -5,8 +5,8 @@
 
 polycrystal257: unseen seeded Voronoi fields (256 grains, cubic C11=1.7, C12=1.2,
 C44=0.75, Haar orientations, uniaxial 1e-3 along z) at 257^3 nodes solved to
-1e-8 with the plain FP64 MLMG (V and F cycles), the FP32 path (FP32 F-cycle
-inside FP64 FGMRES) and the learned + FP32 path (the trained network on the
+1e-8 with the plain FP64 MLMG (V and F cycles), the FP32 path (FP32 cycles
+inside FP64 FGMRES, elastic.fast_path) and the learned + FP32 path (the trained network on the
 33^3 level, judged initial correction); times, iterations and the relative
 L2 error against an FP64 solve to 1e-10.
 
@@ -70,13 +70,10 @@ def polycrystal257(fields=4, n_cell=256, weights=None):
         ref = torch.zeros_like(b)
         el.Linear(op, cycle="F", outer="fgmres", abort_on_fail=False).solve(ref, b, 1e-10, 0.0)
         row = dict(field=seed, grains=256, nodes=int(np.prod(op.level_shape(0))), network_level=level)
-        runs = [("plain_fp64_V", dict(cycle="V")), ("plain_fp64_F", dict(cycle="F")),
-                ("fp32", dict(cycle="F", precision="fp32", outer="fgmres", restart=10, bottom_solver="direct", pre_smooth=3,
-                             post_smooth=3))]
+        fast = el.fast_path(row["nodes"])
+        runs = [("plain_fp64_V", dict(cycle="V")), ("plain_fp64_F", dict(cycle="F")), ("fp32", fast)]
         if net is not None:
-            runs.append(("learned_fp32", dict(cycle="F", precision="fp32", outer="fgmres", restart=10, bottom_solver="direct",
-                                              pre_smooth=3, post_smooth=3,
-                                              learned="guess")))
+            runs.append(("learned_fp32", dict(fast, learned="guess")))
         for name, kw in runs:
             x = torch.zeros_like(b)
             s = el.Linear(op, abort_on_fail=False, max_iter=400, **kw)

@@ -66,14 +66,6 @@ def log(msg):
 FAPPLY_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 1>"
 FAPPLY_PALETTE_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 2>"
 COPY_BYTES = 1 << 28  # calibration copy: 256 MiB read and 256 MiB written
-# FGMRES restart of the fast path: the classical Gram-Schmidt passes read
-# every basis vector twice per step, so a short basis is cheaper; 10 took 23
-# steps at 129^3 (as 30 did) in 8% less time.
-FAST_RESTART = 10
-# Fsmooth calls before and after the coarse correction in the FP32 F-cycle
-# that preconditions the fast path (the plain FP64 path keeps the reference's
-# 2 and 2): 3 and 3 took 16 steps at 129^3 instead of 23, 13% less time.
-FAST_NU = 3
 # F-cycles of the FP64 oracle timed for the CPU baseline (about 9 s on 16
 # host threads at 129^3), scaled to the plain FP64 solve's cycle count
 CPU_CYCLES = 10
@@ -269,11 +261,11 @@ def main():
     else:
         log("no trained network for this grid: the learned step is off")
 
+    fast = el.fast_path(nodes)
+
     def fast_solve(mode):
         x = torch.zeros_like(rhs)
-        s = el.Linear(op, cycle="F", precision="fp32", outer="fgmres", restart=FAST_RESTART, bottom_solver="direct",
-                      pre_smooth=FAST_NU, post_smooth=FAST_NU,
-                      learned=mode)
+        s = el.Linear(op, learned=mode, **fast)
         s.solve(x, rhs, args.tol, 0.0)
         return x, s
 
@@ -376,8 +368,10 @@ def main():
         "config": {"workload": f"{p['name']} cubic C11=1.7 C12=1.2 C44=0.75, Haar rotations, uniaxial 1e-3 in z",
                    "n_cell": args.n_cell, "nodes": nodes, "tol_rel": args.tol,
                    "path": ("learned judged initial correction (U-Net, BF16 MFMA, on the 33^3 level) + " if learned
-                            else "") + f"FP32 F-cycle (nu1 = nu2 = {FAST_NU}) preconditioned FGMRES({FAST_RESTART}), FP64 residual, Krylov basis "
-                                       "and direct FP64 solve on the 9^3 level"},
+                            else "") + (f"FP32 cycles (nu1 = {fast['pre_smooth']}, nu2 = {fast['post_smooth']}; "
+                                        f"F-cycles for the first {fast['max_fmg_iter']} steps, V-cycles after) "
+                                        f"preconditioning FGMRES({fast['restart']}), FP64 residual, Krylov basis "
+                                        "and direct FP64 solve on the 9^3 level")},
         "learned_fp32": head if learned else None,
         "fp32_only": fp32_only,
         "plain_fp64": plain,

@@ -1824,10 +1824,12 @@ namespace {
 
 // Flexible GMRES(m), right preconditioned by one multigrid cycle in the
 // chosen precision (the cycle is a nonlinear, changing preconditioner, so
-// the preconditioned directions Z are stored).  Classical Gram-Schmidt, one
-// multi-vector dot kernel per pass, with the DGKS test deciding a second pass.
-// The Krylov basis is FP64, its vectors MP apart (M0 rounded up to whole
-// 128-byte lines, so that every vector starts a line).
+// the preconditioned directions Z are stored).  Classical Gram-Schmidt twice:
+// for m <= kMultiDotMax with the second pass of each vector delayed to the
+// next step (DCGS2, elastic_kernels.h), else fused passes up to kMultiDotMax
+// vectors and the DGKS test deciding a second pass beyond.  The Krylov basis
+// is FP64, its vectors MP apart (M0 rounded up to whole 128-byte lines, so
+// that every vector starts a line).
 int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const alamo_el_opts& o, int& iters,
                  double& resnorm, int& learned_taken, hipStream_t s) {
     Level& l0 = h->lv[0];
@@ -1842,7 +1844,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         ALLOC(h->kz, (long)m * MP);
         ALLOC(h->kw, M0);
         ALLOC(h->kh, 2 * (m + 1) + 2);  // two CGS passes of coefficients, |w|^2 after and before
-        ALLOC(h->kpart, (long)(m + 1) * kDotBlocks);
+        ALLOC(h->kpart, 2L * (m + 1) * kDotBlocks);
         if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
         HIPCHK(hipHostMalloc((void**)&h->red_host_vec, sizeof(double) * (2 * (m + 1) + 2)));
         h->krylov_m = m;
@@ -1853,7 +1855,38 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
     Cycle<float> c32{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
                      o.smoother};
     c64.leave_held = c32.leave_held = true;  // Z_j takes cor + cor_hold in its copy
-    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    // H: the Hessenberg columns as projected; R: the final columns after the
+    // Givens rotations (R[i * m + j], row i, column j)
+    std::vector<double> H((m + 1) * m), R((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    auto finalize = [&](int c) {
+        for (int i = 0; i <= c + 1; ++i) R[i * m + c] = H[i * m + c];
+        for (int i = 0; i < c; ++i) {
+            const double a = R[i * m + c], d = R[(i + 1) * m + c];
+            R[i * m + c] = cs[i] * a + sn[i] * d;
+            R[(i + 1) * m + c] = -sn[i] * a + cs[i] * d;
+        }
+        const double a = R[c * m + c], d = R[(c + 1) * m + c];
+        const double rr = std::hypot(a, d);
+        cs[c] = rr > 0 ? a / rr : 1.0;
+        sn[c] = rr > 0 ? d / rr : 0.0;
+        R[c * m + c] = rr;
+        R[(c + 1) * m + c] = 0.0;
+        g[c + 1] = -sn[c] * g[c];
+        g[c] = cs[c] * g[c];
+    };
+    // |g_{c+1}| if column c were final as it stands (columns < c final)
+    auto estimate = [&](int c) {
+        // row c of the rotated column: rotation i mixes rows i and i+1, so
+        // row i+1 after it depends only on row i after rotation i-1
+        double a = H[c];
+        for (int i = 0; i < c; ++i) a = -sn[i] * a + cs[i] * H[(i + 1) * m + c];
+        const double d = H[(c + 1) * m + c];
+        const double rr = std::hypot(a, d);
+        return rr > 0 ? std::fabs(d / rr * g[c]) : std::fabs(g[c]);
+    };
+    // ALAMO_EL_GS=cgs2 keeps the undelayed passes (for comparison)
+    const char* gs_env = std::getenv("ALAMO_EL_GS");
+    const bool lag = m <= ael::kMultiDotMax && !(gs_env && std::strcmp(gs_env, "cgs2") == 0);
     int rc;
     iters = 0;
     // r = b - A x is in l0.d.res on entry
@@ -1909,6 +1942,51 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             // come back in one copy.  Beyond that: the DGKS criterion decides
             // the second pass, with separate dot, projection and norm passes.
             const int nb = (int)std::min<long>(kRedBlocks, std::max<long>(1, nblk(M0)));
+            double* nrm_slot = h->kh + 2 * (m + 1);
+            if (lag) {
+                // DCGS2: w's first projection and V_j's second in two passes
+                // over V; V_0 = r / beta needs no second projection
+                const int n = j + 1;
+                if (j == 0) {
+                    ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, 1, M0, MP, h->kw, h->kpart);
+                    ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h->kh);
+                    ael::k_multiaxpy_dot<false><<<kDotBlocks, kBlock, 0, s>>>(h->kw, h->kv, 1, M0, MP, h->kh, h->kpart);
+                } else {
+                    ael::k_multidot2_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, h->kpart);
+                    ael::k_dcgs2_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, n, h->kh);
+                    ael::k_dcgs2_update<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, h->kh, h->kpart);
+                }
+                ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, nrm_slot);
+                HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (2 * (m + 1) + 2),
+                                      hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                const double* hv = h->red_host_vec;
+                for (int i = 0; i <= m; ++i) H[i * m + j] = 0.0;
+                if (j == 0) {
+                    H[0] = hv[0];
+                } else {
+                    // V_j = (t - V P) / alpha completes column j-1:
+                    // w_{j-1} = V (h + beta P) + beta alpha V_j
+                    const double alpha = hv[2 * n];
+                    for (int i = 0; i < j; ++i) H[i * m + (j - 1)] += hn_prev * hv[i];
+                    H[j * m + (j - 1)] = hn_prev * alpha;
+                    finalize(j - 1);
+                    if (!(alpha > 0.0)) break;  // t in the span of V: column j-1 ends the cycle exactly
+                    for (int i = 0; i <= j; ++i) H[i * m + j] = hv[n + i];
+                }
+                const double hn = std::sqrt(hv[2 * (m + 1)]);
+                H[(j + 1) * m + j] = hn;
+                hn_prev = hn;  // V_{j+1} = w / hn is written by the next step
+                ++iters;
+                const double est = estimate(j);
+                if (o.verbose) std::fprintf(stderr, "alamo_el: fgmres %d |r|_2 est = %.6e\n", iters, est);
+                if (est / beta * inf_at_restart <= target || hn == 0.0 || j + 1 == m || iters >= limit) {
+                    finalize(j);  // as it stands: its second projection is not needed
+                    ++j;
+                    break;
+                }
+                continue;
+            }
             auto sq_norm = [&](double* dst) {
                 ael::k_reduce_partial<double><<<nb, kBlock, 0, s>>>(h->kw, h->kw, M0, 1, h->red_partial);
                 ael::k_reduce_final<<<1, kBlock, 0, s>>>(h->red_partial, nb, 1, dst);
@@ -1956,19 +2034,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             const double hn = std::sqrt(hn2);
             H[(j + 1) * m + j] = hn;
             hn_prev = hn;  // V_{j+1} = w / hn is written by the next step
-            for (int i = 0; i < j; ++i) {
-                const double a = H[i * m + j], c = H[(i + 1) * m + j];
-                H[i * m + j] = cs[i] * a + sn[i] * c;
-                H[(i + 1) * m + j] = -sn[i] * a + cs[i] * c;
-            }
-            const double a = H[j * m + j], c = H[(j + 1) * m + j];
-            const double rr = std::hypot(a, c);
-            cs[j] = rr > 0 ? a / rr : 1.0;
-            sn[j] = rr > 0 ? c / rr : 0.0;
-            H[j * m + j] = rr;
-            H[(j + 1) * m + j] = 0.0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
+            finalize(j);
             ++iters;
             if (o.verbose) std::fprintf(stderr, "alamo_el: fgmres %d |r|_2 est = %.6e\n", iters, std::fabs(g[j + 1]));
             // Test the true inf-norm residual once the 2-norm reduction since
@@ -1980,8 +2046,8 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         const int k = j;
         for (int i = k - 1; i >= 0; --i) {
             double acc = g[i];
-            for (int q = i + 1; q < k; ++q) acc -= H[i * m + q] * y[q];
-            y[i] = acc / H[i * m + i];
+            for (int q = i + 1; q < k; ++q) acc -= R[i * m + q] * y[q];
+            y[i] = acc / R[i * m + i];
         }
         // one pass over x and Z (the accumulation order of k axpys: x + y_0 z_0 + ...)
         if (k > 0) {

@@ -1399,6 +1399,213 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w
     }
 }
 
+// Classical Gram-Schmidt twice with the second pass delayed by one step
+// (DCGS2: Bielich, Langou, Thomas, Swirydowicz, Yamazaki and Boman, "Low-synch
+// Gram-Schmidt with delayed reorthogonalization for Krylov solvers", Parallel
+// Computing 112, 2022).  Of the n vectors in V, V_0 .. V_{n-2} are
+// orthonormal and t = V_{n-1} is the last step's vector after one projection,
+// normalized.  Its second projection shares the two passes over V with the
+// first projection of the new w:
+//   pass 1 (k_multidot2_partial): Q_i = V_i . w and R_i = V_i . t, i < n;
+//   k_dcgs2_final: P_i = R_i (i < n-1), alpha = sqrt(R_{n-1} - |P|^2),
+//     c_i = Q_i (i < n-1), c_{n-1} = (Q_{n-1} - P . Q) / alpha;
+//   pass 2 (k_dcgs2_update): V_{n-1} <- (t - sum P_i V_i) / alpha and
+//     w <- w - sum_{i<n-1} c_i V_i - c_{n-1} V_{n-1}, with |w|^2.
+// V is read twice a step where CGS2 reads it three times.  n <= kMultiDotMax.
+//
+// Pass 1: slot i < n of the partials holds Q_i, slot n + i holds R_i.
+__global__ void __launch_bounds__(kBlock) k_multidot2_partial(const double* __restrict__ V, int n, long M, long ld,
+                                                              const double* __restrict__ w,
+                                                              double* __restrict__ partial) {
+    __shared__ double sh[kBlock / 64][2 * kMultiDotMax];
+    const double* __restrict__ t = V + (long)(n - 1) * ld;
+    const int k = n - 1;  // vectors before t
+    double aw[kMultiDotMax - 1], at[kMultiDotMax - 1], awl = 0.0, atl = 0.0;
+#pragma unroll
+    for (int i = 0; i < kMultiDotMax - 1; ++i) aw[i] = at[i] = 0.0;
+    // two grid strides per step with independent loads (four held 256 VGPRs)
+    constexpr int kU = 2;
+    const long st = (long)gridDim.x * blockDim.x;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q + (kU - 1) * st < M; q += kU * st) {
+        double wq[kU], tq[kU], vq[kMultiDotMax - 1][kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            wq[u] = w[q + u * st];
+            tq[u] = t[q + u * st];
+        }
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax - 1; ++i)
+#pragma unroll
+            for (int u = 0; u < kU; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax - 1; ++i)
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (i < k) {
+                    aw[i] += vq[i][u] * wq[u];
+                    at[i] += vq[i][u] * tq[u];
+                }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            awl += tq[u] * wq[u];
+            atl += tq[u] * tq[u];
+        }
+    }
+    for (; q < M; q += st) {
+        const double wq = w[q], tq = t[q];
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax - 1; ++i)
+            if (i < k) {
+                const double v = V[(long)i * ld + q];
+                aw[i] += v * wq;
+                at[i] += v * tq;
+            }
+        awl += tq * wq;
+        atl += tq * tq;
+    }
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+    auto wave_sum = [](double v) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        return v;
+    };
+#pragma unroll
+    for (int i = 0; i < kMultiDotMax - 1; ++i) {
+        if (i < k) {
+            const double a = wave_sum(aw[i]), b = wave_sum(at[i]);
+            if (lane == 0) {
+                sh[wave][i] = a;
+                sh[wave][n + i] = b;
+            }
+        }
+    }
+    {
+        const double a = wave_sum(awl), b = wave_sum(atl);
+        if (lane == 0) {
+            sh[wave][k] = a;
+            sh[wave][n + k] = b;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * n) {
+        double s = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < kBlock / 64; ++wv) s += sh[wv][threadIdx.x];
+        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// Sums the 2n slots of nb partials (one wave per slot), then forms the
+// coefficients of pass 2: coef[0 .. n-2] = P, coef[n-1] = 1/alpha,
+// coef[n .. 2n-1] = c, coef[2n] = alpha (0 when t lies in the span of the
+// other vectors).  One workgroup.
+__global__ void __launch_bounds__(kBlock) k_dcgs2_final(const double* __restrict__ partial, int nb, int n,
+                                                        double* __restrict__ coef) {
+    __shared__ double d[2 * kMultiDotMax];
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+    for (int s = wave; s < 2 * n; s += kBlock / 64) {
+        double a = 0.0;
+        for (int q = lane; q < nb; q += 64) a += partial[(long)s * nb + q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) d[s] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int k = n - 1;
+        double pp = 0.0, pq = 0.0;
+        for (int i = 0; i < k; ++i) {
+            pp += d[n + i] * d[n + i];
+            pq += d[n + i] * d[i];
+        }
+        const double a2 = d[n + k] - pp;
+        const double alpha = a2 > 0.0 ? sqrt(a2) : 0.0;
+        const double inv = alpha > 0.0 ? 1.0 / alpha : 0.0;
+        for (int i = 0; i < k; ++i) {
+            coef[i] = d[n + i];
+            coef[n + i] = d[i];
+        }
+        coef[k] = inv;
+        coef[n + k] = (d[k] - pq) * inv;
+        coef[2 * n] = alpha;
+    }
+}
+
+// Pass 2: V_{n-1} and w updated in place, |w|^2 partials in slot 0.
+__global__ void __launch_bounds__(kBlock) k_dcgs2_update(double* __restrict__ V, int n, long M, long ld,
+                                                         double* __restrict__ w, const double* __restrict__ coef,
+                                                         double* __restrict__ partial) {
+    __shared__ double sh[kBlock / 64];
+    double* __restrict__ t = V + (long)(n - 1) * ld;
+    const int k = n - 1;
+    double p[kMultiDotMax - 1], c[kMultiDotMax - 1];
+#pragma unroll
+    for (int i = 0; i < kMultiDotMax - 1; ++i) {
+        p[i] = i < k ? coef[i] : 0.0;
+        c[i] = i < k ? coef[n + i] : 0.0;
+    }
+    const double inv = coef[k], cl = coef[n + k];
+    double nrm = 0.0;
+    // two grid strides per step with independent loads (four held 256 VGPRs)
+    constexpr int kU = 2;
+    const long st = (long)gridDim.x * blockDim.x;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q + (kU - 1) * st < M; q += kU * st) {
+        double wq[kU], tq[kU], vq[kMultiDotMax - 1][kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            wq[u] = w[q + u * st];
+            tq[u] = t[q + u * st];
+        }
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax - 1; ++i)
+#pragma unroll
+            for (int u = 0; u < kU; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            double v = tq[u], a = wq[u];
+#pragma unroll
+            for (int i = 0; i < kMultiDotMax - 1; ++i)
+                if (i < k) {
+                    v += -p[i] * vq[i][u];
+                    a += -c[i] * vq[i][u];
+                }
+            v *= inv;
+            a += -cl * v;
+            t[q + u * st] = v;
+            w[q + u * st] = a;
+            nrm += a * a;
+        }
+    }
+    for (; q < M; q += st) {
+        double v = t[q], a = w[q];
+#pragma unroll
+        for (int i = 0; i < kMultiDotMax - 1; ++i)
+            if (i < k) {
+                const double vi = V[(long)i * ld + q];
+                v += -p[i] * vi;
+                a += -c[i] * vi;
+            }
+        v *= inv;
+        a += -cl * v;
+        t[q] = v;
+        w[q] = a;
+        nrm += a * a;
+    }
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off, 64);
+    if (lane == 0) sh[wave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < kBlock / 64; ++wv) s += sh[wv];
+        partial[blockIdx.x] = s;
+    }
+}
+
 // h[i] = sum of partials; one block per i.
 __global__ void __launch_bounds__(kBlock) k_multidot_final(const double* __restrict__ partial, int nb,
                                                            double* __restrict__ h) {

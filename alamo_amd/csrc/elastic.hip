@@ -213,6 +213,46 @@ int stencil_variant() {
     return v;
 }
 
+// The LDS-tiled kernel on level l, when it serves the level: MajorMinor, the
+// masked stencil variant, a tiled level with its compact D and C copies, and C
+// the level's own.  Returns false otherwise.  x may be an FP32 field for the
+// FP64 Fapply (TX = float, T = double, OP = 0).
+template <typename T, int OP, typename TX>
+bool launch_tiled(alamo_el_t h, const Level& l, const TX* x, const T* b, const T* diag, const T* C, T* out,
+                  hipStream_t s) {
+    const int var = stencil_variant();
+    if (!(var == kVariantSlab || var == kVariantSlabOnly) || h->sym != 1 || !l.tiled) return false;
+    Level& lv = const_cast<Level&>(l);
+    const bool have_d = l.dwbase && (l.nflag == 0 || vecs<T>(lv).D);
+    if (!have_d || !vecs<T>(lv).Ct || !vecs<T>(lv).Cb || C != vecs<T>(lv).C) return false;
+    const Lvl L = l.L;
+    const T w = (T)h->omega;
+    const int u = h->uniform;
+    const unsigned nbb = ael::slab_boundary_blocks(L, kBlock);
+    const unsigned char* gm = l.gmask;
+    // its compact D is in tile order.  C streamed with the non-temporal hint
+    // on levels whose C exceeds kNtMinBytes: read once per launch, it then
+    // leaves L2 to the u halos (129^3: FP64 Fapply 124 -> 108 us, FP32 sweep
+    // 65 -> 62 us).  A smaller level's C stays in the 256 MB last-level cache
+    // between sweeps, which the hint gives up (65^3 FP32 sweep: 11.9 us
+    // without, 12.6 us with).
+    const ael::TileGrid& G = sizeof(T) == 8 ? l.tg : l.tg32;
+    const unsigned grid = nbb + ael::tile_blocks(G);
+    if (l.pidx && vecs<T>(lv).pal)
+        ael::k_stencil_lds<TileT, T, OP, true, 2, TX><<<grid, TileT::NT, 0, s>>>(
+            x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
+            l.nflag, out, l.pidx);
+    else if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
+        ael::k_stencil_lds<TileT, T, OP, true, 1, TX><<<grid, TileT::NT, 0, s>>>(
+            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
+            l.nflag, out, nullptr);
+    else
+        ael::k_stencil_lds<TileT, T, OP, true, 0, TX><<<grid, TileT::NT, 0, s>>>(
+            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
+            l.nflag, out, nullptr);
+    return true;
+}
+
 template <typename T, int OP>
 void launch_stencil_raw(alamo_el_t h, const Level& l, const T* x, const T* b, const T* diag, const T* C, T* out,
                         hipStream_t s) {
@@ -229,33 +269,12 @@ void launch_stencil_raw(alamo_el_t h, const Level& l, const T* x, const T* b, co
         }
         return;
     }
+    if (launch_tiled<T, OP, T>(h, l, x, b, diag, C, out, s)) return;
     const unsigned g = ael::slab_blocks(L, kBlock), nbb = ael::slab_boundary_blocks(L, kBlock);
     const bool masked = var == kVariantSlab || var == kVariantSlabOnly;
     const unsigned char* gm = masked ? l.gmask : nullptr;
     Level& lv = const_cast<Level&>(l);
     const bool have_d = l.dwbase && (l.nflag == 0 || vecs<T>(lv).D);
-    if (masked && h->sym == 1 && l.tiled && have_d && vecs<T>(lv).Ct && vecs<T>(lv).Cb && C == vecs<T>(lv).C) {
-        // LDS-tiled kernel on its own copies of this level's C; its compact D is in tile order
-        // C streamed with the non-temporal hint on levels whose C exceeds
-        // kNtMinBytes: read once per launch, it then leaves L2 to the u halos
-        // (129^3: FP64 Fapply 124 -> 108 us, FP32 sweep 65 -> 62 us).  A smaller
-        // level's C stays in the 256 MB last-level cache between sweeps, which
-        // the hint gives up (65^3 FP32 sweep: 11.9 us without, 12.6 us with).
-        const ael::TileGrid& G = sizeof(T) == 8 ? l.tg : l.tg32;
-        if (l.pidx && vecs<T>(lv).pal)
-            ael::k_stencil_lds<TileT, T, OP, true, 2><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
-                x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out, l.pidx);
-        else if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
-            ael::k_stencil_lds<TileT, T, OP, true, 1><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
-                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out, nullptr);
-        else
-            ael::k_stencil_lds<TileT, T, OP, true, 0><<<nbb + ael::tile_blocks(G), TileT::NT, 0, s>>>(
-                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out, nullptr);
-        return;
-    }
     // compact grad-C matrices in slab order: MajorMinor, masked variant, built for this level
     const bool dc = gm && h->sym == 1 && !l.tiled && have_d;
     const int* dwb = dc ? l.dwbase : nullptr;
@@ -480,13 +499,15 @@ struct Cycle {
         ael::k_cast<float, double><<<nblk(m), kBlock, 0, s>>>(lv.f.res, lv.d.res, m);
     }
 
-    // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.
-    void vcycle(int top) {
+    // amrex::MLMG::mgVcycle on AMR level 0 starting at MG level top.  With
+    // top_fz (and fz_ready) the caller has written the first sweep from zero
+    // of level top into its tmp.
+    void vcycle(int top, bool top_fz = false) {
         const int bot = this->bot();
         const bool fz = fz_ready();
         for (int l = top; l < bot; ++l) {
             Vecs<T>& v = V(l);
-            smooth_from_zero(l, v.cor, v.res, nu1, fz && l > top);
+            smooth_from_zero(l, v.cor, v.res, nu1, fz && (l > top || top_fz));
             corr_res(l, v.rescor, v.cor, v.res);
             if (fz && l + 1 < bot) restrict_res_fz(l, v.rescor);
             else restrict_res(l, v.rescor, V(l + 1).res);
@@ -535,9 +556,9 @@ struct Cycle {
         }
     }
 
-    void run(int cycle) {
+    void run(int cycle, bool top_fz = false) {
         if (cycle == ALAMO_EL_CYCLE_F) fcycle();
-        else vcycle(0);
+        else vcycle(0, top_fz);
     }
 };
 
@@ -1835,6 +1856,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
     Level& l0 = h->lv[0];
     const long M0 = 3 * l0.L.N;
     const long MP = (M0 + 15) / 16 * 16;
+    const long MPf = (M0 + 31) / 32 * 32;  // FP32 Z: whole 128-byte lines too
     const int m = std::max(2, std::min(o.restart, 100));
     if (h->krylov_m < m) {
         for (double* p : {h->kv, h->kz, h->kw, h->kh, h->kpart})
@@ -1887,6 +1909,9 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
     // ALAMO_EL_GS=cgs2 keeps the undelayed passes (for comparison)
     const char* gs_env = std::getenv("ALAMO_EL_GS");
     const bool lag = m <= ael::kMultiDotMax && !(gs_env && std::strcmp(gs_env, "cgs2") == 0);
+    // ALAMO_EL_Z=fp64 keeps the FP32 path's Z basis in FP64 (for comparison)
+    const char* z_env = std::getenv("ALAMO_EL_Z");
+    const bool z32 = !(z_env && std::strcmp(z_env, "fp64") == 0);
     int rc;
     iters = 0;
     // r = b - A x is in l0.d.res on entry
@@ -1921,19 +1946,52 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             // the copy into Z_j
             const int cyc = cycle_of(o, iters);
             const bool held = cyc == ALAMO_EL_CYCLE_F;
-            if (fp32) {
+            // a V-cycle's first sweep from zero on level 0 comes with the copy
+            const bool fz = cyc == ALAMO_EL_CYCLE_V && c32.fz_ready() && c32.bot() > 0;
+            if (fp32 && !z32) {  // Z_j widened to FP64 (ALAMO_EL_Z=fp64)
                 ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0);
                 c32.run(cyc);
                 if (held) ael::k_cast_sum<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, l0.f.cor_hold, Zj, M0);
                 else ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, Zj, M0);
+                launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);
+            } else if (fp32) {
+                // Z_j is kept in FP32: the FP32 cycle's correction, whose FP64
+                // widening is exact, so every product with it is the one with
+                // its FP64 copy
+                float* Zf = reinterpret_cast<float*>(h->kz) + (long)j * MPf;
+                if (fz)
+                    ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0, l0.f.diag,
+                                                                             (float)h->omega, l0.f.tmp);
+                else
+                    ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0);
+                if (held) {
+                    c32.run(cyc, fz);
+                    ael::k_cast_sum<float, float><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, l0.f.cor_hold, Zf, M0);
+                } else {
+                    // a V-cycle writes its level-0 correction straight into Z_j
+                    float* cor = l0.f.cor;
+                    l0.f.cor = Zf;
+                    c32.run(cyc, fz);
+                    l0.f.cor = cor;
+                }
+                // w = A Z_j: the FP64 tiled Fapply reads the FP32 field;
+                // elsewhere an FP64 copy in l0.d.rescor (free during a step)
+                if ((h->psi_set && l0.psi) ||
+                    !launch_tiled<double, 0, float>(h, l0, Zf, nullptr, nullptr, l0.d.C, h->kw, s)) {
+                    ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(Zf, l0.d.rescor, M0);
+                    launch_fapply<double>(h, l0, l0.d.rescor, l0.d.C, h->kw, s);
+                }
             } else {
-                ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0);
-                c64.run(cyc);
+                if (fz)
+                    ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0,
+                                                                              l0.d.diag, h->omega, l0.d.tmp);
+                else
+                    ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0);
+                c64.run(cyc, fz);
                 if (held) ael::k_cast_sum<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, l0.d.cor_hold, Zj, M0);
                 else ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, Zj, M0);
+                launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);  // w = A Z_j
             }
-            // w = A Z_j
-            launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);
             // classical Gram-Schmidt, twice ("twice is enough"): h = V^T w;
             // w -= V h; h' = V^T w; w -= V h'.  Up to kMultiDotMax basis
             // vectors the passes are fused: the multi-dot, then one pass that
@@ -2052,7 +2110,11 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         // one pass over x and Z (the accumulation order of k axpys: x + y_0 z_0 + ...)
         if (k > 0) {
             HIPCHK(hipMemcpyAsync(h->kh, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-            ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(x, h->kz, k, M0, MP, h->kh, 1.0);
+            if (fp32 && z32)
+                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(x, reinterpret_cast<const float*>(h->kz), k, M0, MPf,
+                                                             h->kh, 1.0);
+            else
+                ael::k_multiaxpy<<<nblk(M0), kBlock, 0, s>>>(x, h->kz, k, M0, MP, h->kh, 1.0);
         }
         launch_residual<double>(h, l0, x, b, l0.d.C, l0.d.res, s);
         if ((rc = reduce_host<double>(h, l0.d.res, l0.d.res, M0, 0, &resnorm, s))) return rc;

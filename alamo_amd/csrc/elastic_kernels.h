@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ael {
 
 constexpr int kBlock = 256;
@@ -198,10 +200,11 @@ __device__ __forceinline__ void node_ijk(long id, const Lvl& L, int& i, int& j, 
 // wave with lanes on different faces waits on memory once, not once per case;
 // the one-sided factor 1 is exact, so each case rounds as (a - b) / h or
 // (a - b) * 0.5 / h.
-template <typename T>
-__device__ __forceinline__ T d1(const T* __restrict__ f, long id, long stride, int pos, int n, T h) {
+// f may be narrower than T (an FP32 field in an FP64 row): widened exactly.
+template <typename T, typename TU>
+__device__ __forceinline__ T d1(const TU* __restrict__ f, long id, long stride, int pos, int n, T h) {
     const bool lo = pos == 0, hi = pos == n - 1;
-    const T a = f[hi ? id : id + stride], b = f[lo ? id : id - stride];
+    const T a = (T)f[hi ? id : id + stride], b = (T)f[lo ? id : id - stride];
     return (a - b) * (lo || hi ? T(1) : T(0.5)) / h;
 }
 
@@ -412,9 +415,10 @@ __device__ __forceinline__ void contract2_mm_stream(const T* __restrict__ C, lon
 }
 
 // Boundary row of Fapply at node (i, j, k) of region reg with normal nrm:
-// one-sided gradients, sigma = C : grad u, and the Constant BC row.
-template <typename T, int SYM>
-__device__ __forceinline__ void apply_boundary(const T* __restrict__ u, const T* __restrict__ C,
+// one-sided gradients, sigma = C : grad u, and the Constant BC row.  u may be
+// narrower than T (widened exactly).
+template <typename T, int SYM, typename TU = T>
+__device__ __forceinline__ void apply_boundary(const TU* __restrict__ u, const T* __restrict__ C,
                                                const int* __restrict__ bct, const Lvl& L, int i, int j, int k,
                                                long id, int reg, const T nrm[3], T out[3], long cN = -1,
                                                long cid = -1) {
@@ -429,8 +433,8 @@ __device__ __forceinline__ void apply_boundary(const T* __restrict__ u, const T*
     T uu[3], gu[3][3], sg[3][3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-        const T* f = u + p * N;
-        uu[p] = f[id];
+        const TU* f = u + p * N;
+        uu[p] = (T)f[id];
         gu[p][0] = d1(f, id, 1, i, L.nx, hx);
         gu[p][1] = d1(f, id, sy, j, L.ny, hy);
         gu[p][2] = d1(f, id, sz, k, L.nz, hz);
@@ -796,15 +800,19 @@ __global__ void __launch_bounds__(kBlock) k_cast(const Tin* __restrict__ a, Tout
 }
 
 // y = a x (FP64) and yc = y in the cycle's precision, one pass (k_scale_copy
-// then k_cast); yc may be x itself.
+// then k_cast); yc may be x itself.  With fz, also the first Jacobi sweep
+// from zero on yc, fz = w yc / D (k_jacobi_from_zero's arithmetic).
 template <typename Tc>
 __global__ void __launch_bounds__(kBlock) k_scale_copy_cast(const double* x, double a, double* __restrict__ y, Tc* yc,
-                                                            long n) {
+                                                            long n, const Tc* __restrict__ diag = nullptr,
+                                                            Tc omega = Tc(0), Tc* __restrict__ fz = nullptr) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id < n) {
         const double v = a * x[id];
         y[id] = v;
-        yc[id] = (Tc)v;
+        const Tc c = (Tc)v;
+        yc[id] = c;
+        if (fz) fz[id] = omega * c / diag[id];
     }
 }
 
@@ -1621,13 +1629,15 @@ __global__ void __launch_bounds__(kBlock) k_multidot_final(const double* __restr
     if (threadIdx.x == 0) h[blockIdx.x] = sh[0];
 }
 
-// w -= sum_i h[i] V_i  (h on the device), then optionally y += sum_i g[i] Z_i.
-__global__ void __launch_bounds__(kBlock) k_multiaxpy(double* __restrict__ w, const double* __restrict__ V, int k,
-                                                      long M, long ld, const double* __restrict__ h, double sign) {
+// w += sign sum_i h[i] V_i  (h on the device); V may be FP32 vectors
+// (widened exactly, so the sums are those of their FP64 copies).
+template <typename TV>
+__global__ void __launch_bounds__(kBlock) k_multiaxpy(double* __restrict__ w, const TV* __restrict__ V, int k, long M,
+                                                      long ld, const double* __restrict__ h, double sign) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= M) return;
     double acc = w[q];
-    for (int i = 0; i < k; ++i) acc += sign * h[i] * V[(long)i * ld + q];
+    for (int i = 0; i < k; ++i) acc += sign * h[i] * (double)V[(long)i * ld + q];
     w[q] = acc;
 }
 

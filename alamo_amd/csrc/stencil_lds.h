@@ -283,8 +283,8 @@ __global__ void __launch_bounds__(TS::NT) k_gradc_fill_tiles(const double* __res
 
 // One plane of the u halo tile: this thread's PRE values (in range of the
 // box; the others are never read by a node of the tile).
-template <class TS, typename T>
-__device__ __forceinline__ void tile_fetch(const T* __restrict__ u, const Lvl& L, int it, int jt, int k,
+template <class TS, typename T, typename TX>
+__device__ __forceinline__ void tile_fetch(const TX* __restrict__ u, const Lvl& L, int it, int jt, int k,
                                            T pre[TS::PRE]) {
     const unsigned N = (unsigned)L.N;
     const unsigned plane = (unsigned)k * (unsigned)(L.nx * L.ny);
@@ -298,7 +298,7 @@ __device__ __forceinline__ void tile_fetch(const T* __restrict__ u, const Lvl& L
         const int c = q / TS::HP, pos = q - c * TS::HP;
         const int hy = pos / TS::HX, hx = pos - hy * TS::HX;
         const int gi = min(it * TS::TX + hx, L.nx - 1), gj = min(jt * TS::TY + hy, L.ny - 1);
-        pre[m] = ld32(u, (unsigned)c * N + plane + (unsigned)(gj * L.nx + gi));
+        pre[m] = (T)ld32(u, (unsigned)c * N + plane + (unsigned)(gj * L.nx + gi));
     }
 }
 template <class TS, typename T>
@@ -311,14 +311,16 @@ __device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PR
 
 // CM: how the interior nodes read C -- 0 the tile-blocked copy Ct, 1 the same
 // with the non-temporal hint, 2 the palette (Ct is then the table, pidx the
-// tile-blocked row indices).
-template <class TS, typename T, int OP, bool DC, int CM>
+// tile-blocked row indices).  TX: the type of x, narrower than T only for
+// Fapply (an FP32 field in FP64 rows, widened exactly on load).
+template <class TS, typename T, int OP, bool DC, int CM, typename TX = T>
 __global__ void __launch_bounds__(TS::NT)
-    k_stencil_lds(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
+    k_stencil_lds(const TX* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
                   const T* __restrict__ Ct, const T* __restrict__ Cb, const int* __restrict__ bct, int uniform,
                   Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
                   const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out,
                   const PalIdx* __restrict__ pidx) {
+    static_assert(OP == 0 || std::is_same<T, TX>::value, "a narrower x is for Fapply only");
     __shared__ T su[3][TS::RING];  // [ring slot][component][hy][hx], padded to RING
     // The nbb boundary workgroups (a multiple of kXcd, so that tile ib keeps
     // ib % kXcd == XCD) come after the tile ones in FP64 and before them in
@@ -338,12 +340,12 @@ __global__ void __launch_bounds__(TS::NT)
         const long id = ((long)k * L.ny + j) * L.nx + i;
         T nrm[3], o[3];
         const int reg = region_of<T>(i, j, k, L, nrm);
-        apply_boundary<T, 1>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
+        apply_boundary<T, 1, TX>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
         const unsigned n32 = (unsigned)id;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
             const size_t off = (size_t)p * L.N;
-            const T xv = (OP == 2) ? ld32(x + off, n32) : T(0);
+            const T xv = (OP == 2) ? (T)ld32(x + off, n32) : T(0);
             const T bv = (OP >= 1) ? ld32(b + off, n32) : T(0);
             const T dv = (OP == 2) ? ld32(diag + off, n32) : T(1);
             st32(out + off, n32, finish_op<T, OP>(o[p], xv, bv, dv, omega));
@@ -376,7 +378,7 @@ __global__ void __launch_bounds__(TS::NT)
         T pre[TS::PRE];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-            tile_fetch<TS, T>(x, L, it, jt, kfirst + d * (s - 1), pre);
+            tile_fetch<TS, T, TX>(x, L, it, jt, kfirst + d * (s - 1), pre);
             tile_store<TS, T>(su[s], pre);
         }
     }
@@ -446,7 +448,7 @@ __global__ void __launch_bounds__(TS::NT)
                 for (int q = 0; q < 18; ++q) dv[q] = ld32(Dc + (size_t)q * nflag, pidx);
             }
         }
-        if (more) tile_fetch<TS, T>(x, L, it, jt, k + 2 * d, pre);
+        if (more) tile_fetch<TS, T, TX>(x, L, it, jt, k + 2 * d, pre);
         __builtin_amdgcn_sched_barrier(0);
 
         if (valid) {

@@ -313,8 +313,9 @@ def test_max_fmg_iter_and_fixed_iter_match_oracle(outer):
         assert abs(s.info.res - info3["res"]) <= 1e-6 * info3["res"]
 
 
-@pytest.mark.parametrize("precision,restart", [("fp64", 10), ("fp32", 10), ("fp32", 3), ("fp32", 12)])
-def test_delayed_reorthogonalization_matches_cgs2(precision, restart, monkeypatch):
+@pytest.mark.parametrize("precision,restart,n_cell", [("fp64", 10, 32), ("fp32", 10, 40), ("fp32", 3, 32),
+                                                      ("fp32", 12, 40)])
+def test_delayed_reorthogonalization_matches_cgs2(precision, restart, n_cell, monkeypatch):
     """FGMRES with restart <= 12 delays each basis vector's second
     Gram-Schmidt projection to the next step (DCGS2: two passes over the basis
     a step instead of three).  Against the undelayed passes (ALAMO_EL_GS=cgs2)
@@ -323,7 +324,7 @@ def test_delayed_reorthogonalization_matches_cgs2(precision, restart, monkeypatc
     to 1e-11 (the solves stop at 1e-8 relative residual).  Restart 3 runs
     several restart cycles (the last column of each is kept as projected once);
     12 fills the kernels' vector limit."""
-    p = P.polycrystal(32, ngrains=16, seed=3)
+    p = P.polycrystal(n_cell, ngrains=16, seed=3)
     el, op, rhs = gpu_setup(p)
     x64 = torch.zeros_like(rhs)
     el.Linear(op, outer="fgmres").solve(x64, rhs, 1e-11, 0.0)
@@ -343,3 +344,31 @@ def test_delayed_reorthogonalization_matches_cgs2(precision, restart, monkeypatc
         assert rel_l2(host(x), host(x64)) < 1e-5
         got[gs] = s.info.iters
     assert abs(got["cgs2"] - got["dcgs2"]) <= 1, got
+
+
+@pytest.mark.parametrize("n_cell,cycle", [(32, "F"), (40, "F"), (40, "V")])
+def test_fp32_z_basis_gives_the_bits_of_the_fp64_copy(n_cell, cycle, monkeypatch):
+    """The FP32 path keeps its preconditioned directions Z in FP32 (the FP32
+    cycle's output, whose widening to FP64 is exact): the FP64 Fapply reads
+    them directly (tiled levels, 40^3 here) or through an FP64 copy (slab
+    levels, 32^3), a V-cycle writes its correction straight into Z, and the
+    solution update widens them.  The solve is the one with Z stored in FP64
+    (ALAMO_EL_Z=fp64), bit for bit: same iterations, same solution."""
+    from alamo_amd import elastic as el
+    p = P.polycrystal(n_cell, ngrains=16, seed=5)
+    _, op, rhs = gpu_setup(p)
+    kw = dict(el.FAST_PATH, cycle=cycle, max_fmg_iter=2)
+    xs, its = [], []
+    for z in ("fp64", "fp32"):
+        if z == "fp64":
+            monkeypatch.setenv("ALAMO_EL_Z", "fp64")
+        else:
+            monkeypatch.delenv("ALAMO_EL_Z", raising=False)
+        x = torch.zeros_like(rhs)
+        s = el.Linear(op, max_iter=400, **kw)
+        s.solve(x, rhs, 1e-8, 0.0)
+        assert s.info.status == 0
+        xs.append(x)
+        its.append(s.info.iters)
+    assert its[0] == its[1]
+    assert torch.equal(xs[0], xs[1])

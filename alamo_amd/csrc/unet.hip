@@ -396,6 +396,33 @@ int alamo_nn_backward(alamo_nn_t nn, const float* dout, float* dparams, float* d
     return backward(nn, dout, dparams, din, s);
 }
 
+int alamo_nn_tape(alamo_nn_t nn, int conv, float* x, float* y, hipStream_t s) {
+    if (!nn || !x) return fail(ALAMO_EL_EINVAL, "null argument");
+    const int ncv = (int)nn->cv.size();
+    if (conv < 0 || conv >= ncv) return fail(ALAMO_EL_EINVAL, "alamo_nn_tape: no such convolution");
+    if (y && conv == ncv - 1) return fail(ALAMO_EL_EINVAL, "alamo_nn_tape: the output layer keeps no bf16 output");
+    if (!nn->have_fwd) return fail(ALAMO_EL_ESTATE, "alamo_nn_tape needs a preceding alamo_nn_forward");
+    // the buffers forward() reads and writes for each layer
+    const bf16_t *in = nullptr, *out = nullptr;
+    if (conv < 2 * kLevels) {
+        const int l = conv / 2;
+        in = (conv & 1) ? nn->A[l] : (l == 0 ? nn->X0 : nn->D[l]);
+        out = (conv & 1) ? nn->E[l] : nn->A[l];
+    } else if (conv < ncv - 1) {
+        const int q = conv - 2 * kLevels, l = kLevels - 2 - q / 2;
+        in = (q & 1) ? nn->B[l] : nn->K[l];
+        out = (q & 1) ? nn->G[l] : nn->B[l];
+    } else {
+        in = nn->G[0];
+    }
+    const ConvL& c = nn->cv[conv];
+    const int N = nn->g[c.level].N;
+    aunet::k_unpack_soa<<<nb256((long)N * c.cin), 256, 0, s>>>(in, c.cin, c.cinp, N, x);
+    if (y) aunet::k_unpack_soa<<<nb256((long)N * c.cout), 256, 0, s>>>(out, c.cout, c.coutp, N, y);
+    NNCHK(hipGetLastError());
+    return 0;
+}
+
 int alamo_nn_mse(alamo_nn_t nn, const float* out, const float* target, float* dout, double* loss_host, hipStream_t s) {
     if (!nn || !out || !target) return fail(ALAMO_EL_EINVAL, "null argument");
     return mse(nn, out, target, dout, loss_host, s);

@@ -18,7 +18,7 @@ from .elastic import AlamoError
 NN_EXPORTS = [
     "alamo_nn_create", "alamo_nn_destroy", "alamo_nn_num_params", "alamo_nn_shape", "alamo_nn_init_params",
     "alamo_nn_set_params", "alamo_nn_get_params", "alamo_nn_forward", "alamo_nn_backward", "alamo_nn_mse",
-    "alamo_nn_adam", "alamo_nn_train_step",
+    "alamo_nn_adam", "alamo_nn_train_step", "alamo_nn_tape",
 ]
 
 _bound = False
@@ -43,6 +43,7 @@ def lib():
         L.alamo_nn_mse.argtypes = [vp, vp, vp, vp, ctypes.POINTER(cd), vp]
         L.alamo_nn_adam.argtypes = [vp, vp, ctypes.c_float, vp]
         L.alamo_nn_train_step.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.POINTER(cd), vp]
+        L.alamo_nn_tape.argtypes = [vp, ci, vp, vp, vp]
         _bound = True
     return L
 
@@ -87,6 +88,39 @@ class UNet:
     @property
     def field_shape(self):
         return (self.nodes[2], self.nodes[1], self.nodes[0])
+
+    CHANNELS = (16, 32, 64, 128)
+
+    def conv_table(self):
+        """(cin, cout, level) of the 15 convolutions, in parameter order:
+        two per encoder level, two per decoder level (2, 1, 0), the output."""
+        ch = self.CHANNELS
+        t = []
+        for lev in range(4):
+            t += [(self.n_in if lev == 0 else ch[lev - 1], ch[lev], lev), (ch[lev], ch[lev], lev)]
+        for lev in (2, 1, 0):
+            t += [(ch[lev + 1] + ch[lev], ch[lev], lev), (ch[lev], ch[lev], lev)]
+        return t + [(ch[0], self.n_out, 0)]
+
+    def level_shape(self, lev):
+        n = list(self.nodes)
+        for _ in range(lev):
+            n = [(v - 1) // 2 + 1 for v in n]
+        return (n[2], n[1], n[0])
+
+    def tape(self, conv):
+        """After forward(): the input (cin, nz, ny, nx) of convolution `conv`
+        and its output after the ReLU (cout, nz, ny, nx; None for the output
+        layer), as the forward pass kept them (alamo_nn_tape)."""
+        torch = _el._torch()
+        cin, cout, lev = self.conv_table()[conv]
+        shape = self.level_shape(lev)
+        x = torch.empty((cin,) + shape, dtype=torch.float32, device="cuda")
+        y = None if conv == len(self.conv_table()) - 1 else torch.empty((cout,) + shape, dtype=torch.float32,
+                                                                            device="cuda")
+        _el._check(lib().alamo_nn_tape(self.h, int(conv), _f32(x), _f32(y) if y is not None else None,
+                                       _el._stream()), "alamo_nn_tape")
+        return x, y
 
     def params(self):
         torch = _el._torch()

@@ -50,6 +50,15 @@ int alamo_nn_forward(alamo_nn_t nn, const float* in, float* out, hipStream_t str
  * parameters (dparams, num_params floats) and, when din is not NULL, the input
  * (din[n_in][N]). */
 int alamo_nn_backward(alamo_nn_t nn, const float* dout, float* dparams, float* din, hipStream_t stream);
+/* After alamo_nn_forward: the activations it kept for convolution `conv`
+ * (0 .. 14, in parameter order), bf16 values widened to float32 -- its input
+ * x[cin][nodes of its level] and, when y is not NULL, its output after the
+ * ReLU y[cout][nodes] (ALAMO_EL_EINVAL for the output layer, which has no
+ * ReLU and keeps no bf16 output).  No counterpart in the reference (the
+ * network is this build's own): it lets a test run a backward pass on the
+ * device's own forward tape, so that the comparison is not blurred by ReLU
+ * masks that bf16 rounding flipped in the forward pass. */
+int alamo_nn_tape(alamo_nn_t nn, int conv, float* x, float* y, hipStream_t stream);
 /* loss = mean((out - target)^2) over n_out * N values; dout (may be NULL)
  * receives its gradient 2 (out - target) / (n_out N). */
 int alamo_nn_mse(alamo_nn_t nn, const float* out, const float* target, float* dout, double* loss_host,

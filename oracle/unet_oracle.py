@@ -198,12 +198,17 @@ def forward(params, x, n_out, bf16=False, keep=False):
     return y
 
 
-def backward(params, x, n_out, dy, bf16=False):
-    """Gradient of <y, dy> w.r.t. the flat parameters and the input x."""
+def backward(params, x, n_out, dy, bf16=False, tape_in=None):
+    """Gradient of <y, dy> w.r.t. the flat parameters and the input x.
+    tape_in: optional list, per convolution, of (X, Y) -- its input and its
+    output after the ReLU (Y None for the output layer) -- that replaces the
+    activations of this forward pass (a device's own forward tape)."""
     n_in = x.shape[-1]
     r = round_bf16 if bf16 else (lambda v: v)
     y, st = forward(params, x, n_out, bf16=bf16, keep=True)
     tape, layers = st["tape"], st["layers"]
+    if tape_in is not None:
+        tape = [(i, tape_in[i][0], tape_in[i][1] if tape_in[i][1] is not None else Y) for i, _, Y in tape]
     specs = conv_specs(n_in, n_out)
     grads = [None] * len(specs)
 

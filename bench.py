@@ -14,7 +14,14 @@ with the reference's settings (F-cycle every iteration; also the V-cycle) is
 solved once beside it, and the distance of the fast solution to the plain
 FP64 solution is reported.  Inputs are resident in HBM before any timing
 starts.  The CPU baseline is the FP64 oracle on the host cores, on a bounded
-sample (Fapply repetitions and two F-cycles of the same problem).
+sample (Fapply repetitions and ten F-cycles of the same problem).
+
+  python bench.py --cpu-configs
+
+times instead the other workloads on the GPU and, on bounded samples, the FP64
+oracle on the host cores: the 65^3 inclusion (plain V-cycle to 1e-10), the
+3-level AMR workload at a 65^3 base (composite V-cycle to 1e-8) and a 257^3
+polycrystal (plain F-cycle to 1e-8), one JSON object.
 """
 from __future__ import annotations
 
@@ -51,6 +58,8 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--weights", default=os.path.join(ROOT, "alamo_amd", "weights", "unet_polycrystal129_level2.npz"))
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="GPU and CPU-oracle times of the inclusion, AMR and 257^3 workloads instead")
     return ap.parse_args()
 
 
@@ -159,8 +168,151 @@ def measure_traffic(args):
     return out
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_configs(args):
+    """GPU times of three more workloads beside the FP64 oracle's on the host
+    cores, each oracle leg a bounded sample: a converged solve where it takes
+    seconds, else a few cycles scaled to the GPU solve's cycle count."""
+    import torch
+
+    from alamo_amd import bc as B
+    from alamo_amd import elastic as el
+    from alamo_amd import model as M
+    from alamo_amd import problems as P
+    from alamo_amd import train
+    from alamo_amd.measure import _amr_models
+    from oracle import amr_oracle as AO
+    from oracle import oracle as O
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t
+
+    res = dict(cpu=_cpu_model(), cores=O.num_threads(), kind="port")
+
+    # 65^3 inclusion, the plain FP64 V-cycle to 1e-10: the whole oracle solve
+    p = P.inclusion(64)
+    op = el.ElasticOperator(p["n_cell"], p["dx"], p["sym"])
+    op.SetBC(p["bc"])
+    C = torch.from_numpy(p["C"]).cuda()
+    op.SetModel(C)
+    b = op.bc_init(torch.zeros((3,) + op.level_shape(0), dtype=torch.float64, device="cuda"))
+    rhs = op.newton_rhs(C, torch.zeros_like(b), b, F0=torch.from_numpy(p["F0"]).cuda())
+    op.prepare()
+    gpu_t, gpu_it = [], 0
+    for _ in range(3):
+        x = torch.zeros_like(rhs)
+        s = el.Linear(op, cycle="V")
+        _, dt = timed(lambda: s.solve(x, rhs, 1e-10, 0.0))
+        gpu_t.append(dt)
+        gpu_it = s.info.iters
+    shape = (p["n_cell"] + 1,) * 3
+    rhs_o = O.newton_rhs(np.zeros((3,) + shape), O.bc_init(shape, p["bc"]), p["C"], p["sym"], p["bc"], p["dx"],
+                         F0=p["F0"])
+    t = time.perf_counter()
+    _, info = O.mlmg_solve(rhs_o, p["C"], p["sym"], p["bc"], p["dx"], tol_rel=1e-10, cycle="V")
+    cpu_s = time.perf_counter() - t
+    res["inclusion65_fp64_V_1e-10"] = dict(gpu_s=float(np.median(gpu_t)), gpu_iters=gpu_it, cpu_s=cpu_s,
+                                            cpu_iters=info["iters"], sample="the whole oracle solve")
+    log(f"inclusion 65^3: GPU {np.median(gpu_t):.4f} s ({gpu_it} V-cycles), oracle {cpu_s:.2f} s "
+        f"({info['iters']})")
+    del op, C, rhs, b
+
+    # 3-level AMR, 65^3 base, clustered patches: 3 oracle composite V-cycles
+    n0, width = 64, 2.0 / 64
+    spheres = P.random_spheres(3, seed=0)
+    base = el.ElasticOperator(n0, 1.0 / n0, M.ISOTROPIC)
+    base.SetBC(B.Constant())
+    eta_of = (lambda op_: torch.from_numpy(P.sphere_indicators(op_.origin, op_.level_shape(0), op_.dx[0], spheres,
+                                                                 width)).cuda())
+    ops = el.amr_levels(base, eta_of, levels=3, blocking_factor=8, grid_eff=0.7)
+    bs = _amr_models(ops, spheres, width)
+    gpu = {}
+    for prec in ("fp64", "fp32"):
+        ts = []
+        for _ in range(3):
+            xs = [torch.zeros_like(v) for v in bs]
+            inf_, dt = timed(lambda: el.amr_solve(ops, xs, bs, 1e-8, 0.0, max_iter=400, cycle="V", precision=prec))
+            ts.append(dt)
+        gpu[prec] = (float(np.median(ts)), inf_.iters)
+    lv = []
+    for q, o in enumerate(ops):
+        Cq, _ = P.inclusions_on_box(o.origin, o.level_shape(0), o.dx[0], spheres, width=width)
+        if q == 0:
+            lv.append(AO.Level(Cq, M.ISOTROPIC, B.Constant(), o.dx[0], dom=o.dom))
+        else:
+            L = AO.Level(Cq, M.ISOTROPIC, AO.patch_bc(B.Constant(), o.ghost_lo, o.ghost_hi), o.dx[0],
+                         origin=o.origin, clo=o.clo, chi=o.chi, gl=o.ghost_lo, gh=o.ghost_hi, dom=o.dom)
+            L.parent = next(i for i, oo in enumerate(ops) if oo is o.parent)
+            lv.append(L)
+    bh = [v.cpu().numpy() for v in bs]
+    ncyc = 3
+    t = time.perf_counter()
+    AO.amr_solve(lv, [np.zeros_like(v) for v in bh], bh, 1e-300, max_iter=ncyc, cycle="V")
+    per = (time.perf_counter() - t) / ncyc
+    res["amr3_65_base_V_1e-8"] = dict(
+        gpu_fp64_s=gpu["fp64"][0], gpu_fp32_s=gpu["fp32"][0], gpu_iters=gpu["fp64"][1],
+        nodes=[int(np.prod(o.level_shape(0))) for o in ops], cpu_s=per * gpu["fp64"][1], cpu_s_per_cycle=per,
+        sample=f"{ncyc} oracle composite V-cycles (Python over the C++ level kernels), scaled to the GPU's "
+               f"{gpu['fp64'][1]}")
+    log(f"AMR 65^3 base: GPU {gpu['fp64'][0]:.4f} s FP64 / {gpu['fp32'][0]:.4f} s FP32 ({gpu['fp64'][1]} cycles), "
+        f"oracle {per:.2f} s a cycle")
+    del ops, bs, lv, bh, base
+
+    # 257^3 polycrystal (256 grains, the first unseen field): 2 oracle F-cycles
+    n = 256
+    Cg = train.polycrystal_field(n, 256, 900000, 1.7, 1.2, 0.75)
+    bc = B.uniaxial(2, 1e-3)
+    op = el.ElasticOperator(n, 1.0 / n, M.MAJORMINOR)
+    op.SetBC(bc)
+    op.SetModel(Cg)
+    op.prepare()
+    b = op.bc_init(torch.zeros((3,) + op.level_shape(0), dtype=torch.float64, device="cuda"))
+    x = torch.zeros_like(b)
+    s = el.Linear(op, cycle="F")
+    _, gpu_plain = timed(lambda: s.solve(x, b, 1e-8, 0.0))
+    plain_it = s.info.iters
+    ts = []
+    for _ in range(3):
+        x = torch.zeros_like(b)
+        f = el.Linear(op, **el.fast_path(int(np.prod(op.level_shape(0)))))
+        _, dt = timed(lambda: f.solve(x, b, 1e-8, 0.0))
+        ts.append(dt)
+    Ch = Cg.cpu().numpy()
+    del op, Cg, b, x
+    torch.cuda.empty_cache()
+    shape = (n + 1,) * 3
+    bo = O.bc_init(shape, bc)
+    ncyc = 2
+    t = time.perf_counter()
+    O.mlmg_solve(bo, Ch, M.MAJORMINOR, bc, 1.0 / n, tol_rel=1e-300, max_iter=ncyc, cycle="F")
+    per = (time.perf_counter() - t) / ncyc
+    res["polycrystal257_F_1e-8"] = dict(gpu_plain_fp64_F_s=gpu_plain, gpu_plain_iters=plain_it,
+                                        gpu_fast_path_s=float(np.median(ts)), cpu_s=per * plain_it,
+                                        cpu_s_per_cycle=per,
+                                        sample=f"{ncyc} oracle F-cycles scaled to the GPU plain solve's {plain_it}")
+    log(f"polycrystal 257^3: GPU plain F {gpu_plain:.3f} s ({plain_it} cycles), fast path "
+        f"{np.median(ts):.3f} s; oracle {per:.1f} s a cycle")
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main():
     args = parse()
+    if args.cpu_configs:
+        return cpu_configs(args)
     if args.pmc_child:
         pmc_child(args)
         return

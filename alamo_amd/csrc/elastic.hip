@@ -1865,7 +1865,9 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         ALLOC(h->kv, (long)(m + 1) * MP);
         ALLOC(h->kz, (long)m * MP);
         ALLOC(h->kw, M0);
-        ALLOC(h->kh, 2 * (m + 1) + 2);  // two CGS passes of coefficients, |w|^2 after and before
+        // two CGS passes of coefficients (or DCGS2's), |w|^2 after and before,
+        // then DCGS2's 2 (m + 1) dot sums
+        ALLOC(h->kh, 4 * (m + 1) + 2);
         ALLOC(h->kpart, 2L * (m + 1) * kDotBlocks);
         if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
         HIPCHK(hipHostMalloc((void**)&h->red_host_vec, sizeof(double) * (2 * (m + 1) + 2)));
@@ -2010,9 +2012,11 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                     ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h->kh);
                     ael::k_multiaxpy_dot<false><<<kDotBlocks, kBlock, 0, s>>>(h->kw, h->kv, 1, M0, MP, h->kh, h->kpart);
                 } else {
+                    double* sums = h->kh + 2 * (m + 1) + 2;  // the 2n sums of pass 1
                     ael::k_multidot2_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, h->kpart);
-                    ael::k_dcgs2_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, n, h->kh);
-                    ael::k_dcgs2_update<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, h->kh, h->kpart);
+                    ael::k_multidot_final<<<2 * n, kBlock, 0, s>>>(h->kpart, kDotBlocks, sums);
+                    ael::k_dcgs2_update<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, sums, h->kh,
+                                                                      h->kpart);
                 }
                 ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, nrm_slot);
                 HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (2 * (m + 1) + 2),

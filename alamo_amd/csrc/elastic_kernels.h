@@ -1414,10 +1414,11 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w
 // orthonormal and t = V_{n-1} is the last step's vector after one projection,
 // normalized.  Its second projection shares the two passes over V with the
 // first projection of the new w:
-//   pass 1 (k_multidot2_partial): Q_i = V_i . w and R_i = V_i . t, i < n;
-//   k_dcgs2_final: P_i = R_i (i < n-1), alpha = sqrt(R_{n-1} - |P|^2),
-//     c_i = Q_i (i < n-1), c_{n-1} = (Q_{n-1} - P . Q) / alpha;
-//   pass 2 (k_dcgs2_update): V_{n-1} <- (t - sum P_i V_i) / alpha and
+//   pass 1 (k_multidot2_partial, then k_multidot_final over its 2n slots):
+//     Q_i = V_i . w and R_i = V_i . t, i < n;
+//   pass 2 (k_dcgs2_update): P_i = R_i (i < n-1), alpha = sqrt(R_{n-1} - |P|^2),
+//     c_i = Q_i (i < n-1), c_{n-1} = (Q_{n-1} - P . Q) / alpha, then
+//     V_{n-1} <- (t - sum P_i V_i) / alpha and
 //     w <- w - sum_{i<n-1} c_i V_i - c_{n-1} V_{n-1}, with |w|^2.
 // V is read twice a step where CGS2 reads it three times.  n <= kMultiDotMax.
 //
@@ -1504,56 +1505,39 @@ __global__ void __launch_bounds__(kBlock) k_multidot2_partial(const double* __re
     }
 }
 
-// Sums the 2n slots of nb partials (one wave per slot), then forms the
-// coefficients of pass 2: coef[0 .. n-2] = P, coef[n-1] = 1/alpha,
-// coef[n .. 2n-1] = c, coef[2n] = alpha (0 when t lies in the span of the
-// other vectors).  One workgroup.
-__global__ void __launch_bounds__(kBlock) k_dcgs2_final(const double* __restrict__ partial, int nb, int n,
-                                                        double* __restrict__ coef) {
-    __shared__ double d[2 * kMultiDotMax];
-    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
-    for (int s = wave; s < 2 * n; s += kBlock / 64) {
-        double a = 0.0;
-        for (int q = lane; q < nb; q += 64) a += partial[(long)s * nb + q];
+// Pass 2: V_{n-1} and w updated in place, |w|^2 partials in slot 0.  d holds
+// the 2n sums of pass 1 (Q, then R); every thread forms the coefficients from
+// them, and workgroup 0 stores them for the host: coef[0 .. n-2] = P,
+// coef[n-1] = 1/alpha, coef[n .. 2n-1] = c, coef[2n] = alpha (0 when t lies in
+// the span of the other vectors).
+__global__ void __launch_bounds__(kBlock) k_dcgs2_update(double* __restrict__ V, int n, long M, long ld,
+                                                         double* __restrict__ w, const double* __restrict__ d,
+                                                         double* __restrict__ coef, double* __restrict__ partial) {
+    __shared__ double sh[kBlock / 64];
+    double* __restrict__ t = V + (long)(n - 1) * ld;
+    const int k = n - 1;
+    double p[kMultiDotMax - 1], c[kMultiDotMax - 1];
+    double pp = 0.0, pq = 0.0;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-        if (lane == 0) d[s] = a;
+    for (int i = 0; i < kMultiDotMax - 1; ++i) {
+        p[i] = i < k ? d[n + i] : 0.0;
+        c[i] = i < k ? d[i] : 0.0;
+        pp += p[i] * p[i];
+        pq += p[i] * c[i];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int k = n - 1;
-        double pp = 0.0, pq = 0.0;
-        for (int i = 0; i < k; ++i) {
-            pp += d[n + i] * d[n + i];
-            pq += d[n + i] * d[i];
-        }
-        const double a2 = d[n + k] - pp;
-        const double alpha = a2 > 0.0 ? sqrt(a2) : 0.0;
-        const double inv = alpha > 0.0 ? 1.0 / alpha : 0.0;
+    const double a2 = d[n + k] - pp;
+    const double alpha = a2 > 0.0 ? sqrt(a2) : 0.0;
+    const double inv = alpha > 0.0 ? 1.0 / alpha : 0.0;
+    const double cl = (d[k] - pq) * inv;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int i = 0; i < k; ++i) {
             coef[i] = d[n + i];
             coef[n + i] = d[i];
         }
         coef[k] = inv;
-        coef[n + k] = (d[k] - pq) * inv;
+        coef[n + k] = cl;
         coef[2 * n] = alpha;
     }
-}
-
-// Pass 2: V_{n-1} and w updated in place, |w|^2 partials in slot 0.
-__global__ void __launch_bounds__(kBlock) k_dcgs2_update(double* __restrict__ V, int n, long M, long ld,
-                                                         double* __restrict__ w, const double* __restrict__ coef,
-                                                         double* __restrict__ partial) {
-    __shared__ double sh[kBlock / 64];
-    double* __restrict__ t = V + (long)(n - 1) * ld;
-    const int k = n - 1;
-    double p[kMultiDotMax - 1], c[kMultiDotMax - 1];
-#pragma unroll
-    for (int i = 0; i < kMultiDotMax - 1; ++i) {
-        p[i] = i < k ? coef[i] : 0.0;
-        c[i] = i < k ? coef[n + i] : 0.0;
-    }
-    const double inv = coef[k], cl = coef[n + k];
     double nrm = 0.0;
     // two grid strides per step with independent loads (four held 256 VGPRs)
     constexpr int kU = 2;

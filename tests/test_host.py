@@ -39,6 +39,24 @@ def test_library_exports_every_declared_symbol():
     assert set(learned.NN_EXPORTS) <= set(nn_names)
 
 
+def test_bench_kernel_names_are_kernels_of_the_library():
+    """bench.py finds the Fapply kernels in the profiler's output by name: each
+    name it looks for is a kernel of the built library (its host launch stub,
+    demangled by nm)."""
+    import shutil
+    import subprocess
+    import sys
+    from alamo_amd import elastic
+    path = elastic.LIB_PATH
+    if not os.path.exists(path) or shutil.which("nm") is None:
+        pytest.skip("libalamo_elastic.so not built, or nm absent")
+    syms = subprocess.run(["nm", "-C", path], capture_output=True, text=True, check=True).stdout
+    sys.path.insert(0, ROOT)
+    import bench
+    for name in (bench.FAPPLY_KERNEL, bench.FAPPLY_PALETTE_KERNEL):
+        assert name + "(" in syms, name
+
+
 def test_default_opts_are_the_reference_defaults():
     from alamo_amd import elastic
     if not os.path.exists(elastic.LIB_PATH):

@@ -73,7 +73,7 @@ struct Vecs {
 };
 
 struct Level {
-    Lvl L;
+    Lvl L{};  // L.per: periodic axes, the handle's on every MG level
     Vecs<double> d;
     Vecs<float> f;
     unsigned char* gmask = nullptr;  // per node: axes along which C varies (k_grad_mask)
@@ -97,6 +97,7 @@ template <> Vecs<float>& vecs<float>(Level& l) { return l.f; }
 struct alamo_el_s {
     int sym = 0;
     int nc = 0;
+    int per = 0;  // periodic axes (bit a), alamo_el_set_periodic
     std::vector<Level> lv;
     int* bct_dev = nullptr;
     double* bcv_dev = nullptr;
@@ -240,15 +241,15 @@ bool launch_tiled(alamo_el_t h, const Level& l, const TX* x, const T* b, const T
     const unsigned grid = nbb + ael::tile_blocks(G);
     if (l.pidx && vecs<T>(lv).pal)
         ael::k_stencil_lds<TileT, T, OP, true, 2, TX><<<grid, TileT::NT, 0, s>>>(
-            x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
+            x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
             l.nflag, out, l.pidx);
     else if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
         ael::k_stencil_lds<TileT, T, OP, true, 1, TX><<<grid, TileT::NT, 0, s>>>(
-            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
+            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
             l.nflag, out, nullptr);
     else
         ael::k_stencil_lds<TileT, T, OP, true, 0, TX><<<grid, TileT::NT, 0, s>>>(
-            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
+            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
             l.nflag, out, nullptr);
     return true;
 }
@@ -1256,6 +1257,7 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
         l.L.dx = d[0];
         l.L.dy = d[1];
         l.L.dz = d[2];
+        l.L.per = 0;
         h->lv.push_back(l);
         if (max_coarsening_level >= 0 && (int)h->lv.size() > max_coarsening_level) break;
         bool ok = true;
@@ -1333,18 +1335,61 @@ int alamo_el_level_nodes(alamo_el_t h, int mglev, int nodes[3]) {
     return 0;
 }
 
+// The axes whose faces boundary region r lies on (region_of's numbering):
+// faces 0-5, yz edges 6-9, zx edges 10-13, xy edges 14-17, corners 18-25.
+static int region_axes(int r) {
+    if (r < 6) return 1 << (r % 3);
+    if (r < 10) return 6;
+    if (r < 14) return 5;
+    if (r < 18) return 3;
+    return 7;
+}
+
+// A table entry of type periodic must lie on a periodic axis: regions on the
+// faces of a periodic axis never occur (Constant::Init skips those faces,
+// src/BC/Operator/Elastic/Constant.H:160-185), every other region is a
+// boundary row and needs a displacement, traction, Neumann or none type.
+static int periodic_table_check(const int* type, int per) {
+    for (int r = 0; r < 26; ++r) {
+        if (region_axes(r) & per) continue;
+        for (int c = 0; c < 3; ++c)
+            if (type[3 * r + c] == ALAMO_EL_BC_PERIODIC)
+                return fail(ALAMO_EL_EINVAL, "a boundary region off the periodic axes has the periodic type "
+                                             "(call alamo_el_set_periodic first)");
+    }
+    return 0;
+}
+
 int alamo_el_set_bc(alamo_el_t h, const int type_host[78], const double val_host[78]) {
     if (!h || !type_host || !val_host) return fail(ALAMO_EL_EINVAL, "null argument");
-    for (int q = 0; q < 78; ++q) {
+    for (int q = 0; q < 78; ++q)
         if (type_host[q] < 0 || type_host[q] > 4) return fail(ALAMO_EL_EINVAL, "bad boundary type");
-        if (type_host[q] == ALAMO_EL_BC_PERIODIC)
-            return fail(ALAMO_EL_EINVAL, "periodic boundaries are not supported by this library");
-    }
+    int rc;
+    if ((rc = periodic_table_check(type_host, h->per))) return rc;
     std::memcpy(h->bct_host, type_host, sizeof(h->bct_host));
     std::memcpy(h->bcv_host, val_host, sizeof(h->bcv_host));
     HIPCHK(hipMemcpy(h->bct_dev, h->bct_host, sizeof(h->bct_host), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->bcv_dev, h->bcv_host, sizeof(h->bcv_host), hipMemcpyHostToDevice));
     h->prepared = false;
+    return 0;
+}
+
+int alamo_el_set_periodic(alamo_el_t h, const int periodic[3]) {
+    if (!h || !periodic) return fail(ALAMO_EL_EINVAL, "null argument");
+    int per = 0;
+    for (int a = 0; a < 3; ++a)
+        if (periodic[a]) per |= 1 << a;
+    if (per && h->amr_parent) return fail(ALAMO_EL_EINVAL, "an AMR patch cannot be periodic");
+    if (per && h->psi_set) return fail(ALAMO_EL_EINVAL, "the psi weight is not supported on periodic axes");
+    int rc;
+    if ((rc = periodic_table_check(h->bct_host, per))) return rc;
+    if (per != h->per) {
+        // diagonals, coarse operators and the bottom depend on the wrap
+        h->prepared = false;
+        h->prepared32 = false;
+    }
+    h->per = per;
+    for (auto& l : h->lv) l.L.per = per;
     return 0;
 }
 
@@ -1398,6 +1443,7 @@ static int psi_bottom_check(alamo_el_t h, const alamo_el_opts& o) {
 
 int alamo_el_set_psi(alamo_el_t h, const double* psi, double psi_small, hipStream_t s) {
     if (!h || !psi) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (h->per) return fail(ALAMO_EL_EINVAL, "the psi weight is not supported on periodic axes");
     if (!(psi_small >= 0.0 && psi_small < 1.0)) return fail(ALAMO_EL_EINVAL, "psi_small must lie in [0, 1)");
     for (auto& l : h->lv) {
         if (l.psi) continue;
@@ -1639,6 +1685,10 @@ int alamo_el_newton_rhs_noalloc(alamo_el_t h, const double* C, int layout, const
     if (rc) return rc;
     const long cs = layout == ALAMO_EL_SOA ? L.N : 1, cn = layout == ALAMO_EL_SOA ? 1 : h->nc;
     const int rows = (int)std::min<long>(cap - 2, L.nz);
+    // a periodic z wraps the divergence of the faces' rows across the box
+    if ((L.per & 4) && rows < L.nz)
+        return fail(ALAMO_EL_EINVAL, "alamo_el_newton_rhs_noalloc: a periodic z axis needs the whole box in the "
+                                     "scratch (" + std::to_string(cap - 2) + " planes here)");
     for (int k0 = 0; k0 < L.nz; k0 += rows) {
         const int k1 = std::min(L.nz, k0 + rows);
         const int kb = std::max(0, k0 - 1), kt = std::min(L.nz - 1, k1);
@@ -2261,6 +2311,7 @@ extern "C" {
 
 int alamo_amr_patch_create(alamo_el_t parent, const int clo[3], const int chi[3], alamo_el_t* out) {
     if (!parent || !clo || !chi || !out) return fail(ALAMO_EL_EINVAL, "null argument");
+    if (parent->per) return fail(ALAMO_EL_EINVAL, "AMR patches under a periodic level are not supported");
     *out = nullptr;
     ael::Patch P{};
     const Lvl& PL = parent->lv[0].L;

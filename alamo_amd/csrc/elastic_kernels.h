@@ -35,9 +35,29 @@ template <> struct NComp<2> { static constexpr int value = 9; };
 
 struct Lvl {
     int nx, ny, nz;
+    int per;  // periodic axes: bit a set when axis a is periodic (node n-1 is node 0)
     long N;
     double dx, dy, dz;
 };
+
+// Neighbour offsets of node (i, j, k) along each axis: om[a] to the node at
+// -1, op[a] to the node at +1.  Across the faces of a periodic axis they wrap
+// (the nodes 0 and n-1 are the same point, so the node below 0 is n-2 and the
+// node above n-1 is 1: the ghost nodes AMReX fills with the periodic images).
+__device__ __forceinline__ void nb_offsets(const Lvl& L, int i, int j, int k, long om[3], long op[3]) {
+    const int pos[3] = {i, j, k}, n[3] = {L.nx, L.ny, L.nz};
+    const long st[3] = {1, (long)L.nx, (long)L.nx * L.ny};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool per = (L.per >> a) & 1;
+        const long w = (long)(n[a] - 2) * st[a];
+        om[a] = (per && pos[a] == 0) ? w : -st[a];
+        op[a] = (per && pos[a] == n[a] - 1) ? -w : st[a];
+    }
+}
+__device__ __forceinline__ bool on_face(const Lvl& L, int i, int j, int k) {
+    return i == 0 || j == 0 || k == 0 || i == L.nx - 1 || j == L.ny - 1 || k == L.nz - 1;
+}
 
 __host__ __device__ constexpr int vpair(int a, int b) {
     return a == b ? (a == 0 ? 0 : (a == 1 ? 3 : 5)) : ((a + b) == 1 ? 1 : ((a + b) == 2 ? 2 : 4));
@@ -154,7 +174,11 @@ __device__ __forceinline__ void contract3(const T* c, const T B[3][3][3], T f[3]
 // normals use the truncated constants of src/BC/Operator/Elastic/Elastic.H:79-80.
 template <typename T>
 __device__ __forceinline__ int region_of(int i, int j, int k, const Lvl& L, T nrm[3]) {
-    const bool xl = i == 0, xh = i == L.nx - 1, yl = j == 0, yh = j == L.ny - 1, zl = k == 0, zh = k == L.nz - 1;
+    // the faces of a periodic axis are no boundary (Elastic<SYM>::Fapply
+    // classifies on growPeriodicDomain(1), src/Operator/Elastic.cpp:145)
+    const bool px = L.per & 1, py = (L.per >> 1) & 1, pz = (L.per >> 2) & 1;
+    const bool xl = !px && i == 0, xh = !px && i == L.nx - 1, yl = !py && j == 0, yh = !py && j == L.ny - 1,
+               zl = !pz && k == 0, zh = !pz && k == L.nz - 1;
     const bool bx = xl || xh, by = yl || yh, bz = zl || zh;
     if (!(bx || by || bz)) return -1;
     const T S3 = T(0.57735026919), S2 = T(0.70710678118);
@@ -201,9 +225,17 @@ __device__ __forceinline__ void node_ijk(long id, const Lvl& L, int& i, int& j, 
 // the one-sided factor 1 is exact, so each case rounds as (a - b) / h or
 // (a - b) * 0.5 / h.
 // f may be narrower than T (an FP32 field in an FP64 row): widened exactly.
+// On a periodic axis the derivative is central everywhere, the neighbours
+// across the faces wrapped (nb_offsets).
 template <typename T, typename TU>
-__device__ __forceinline__ T d1(const TU* __restrict__ f, long id, long stride, int pos, int n, T h) {
+__device__ __forceinline__ T d1(const TU* __restrict__ f, long id, long stride, int pos, int n, T h,
+                                bool per = false) {
     const bool lo = pos == 0, hi = pos == n - 1;
+    if (per) {
+        const long w = (long)(n - 2) * stride;
+        const T a = (T)f[hi ? id - w : id + stride], b = (T)f[lo ? id + w : id - stride];
+        return (a - b) * T(0.5) / h;
+    }
     const T a = (T)f[hi ? id : id + stride], b = (T)f[lo ? id : id - stride];
     return (a - b) * (lo || hi ? T(1) : T(0.5)) / h;
 }
@@ -435,9 +467,9 @@ __device__ __forceinline__ void apply_boundary(const TU* __restrict__ u, const T
     for (int p = 0; p < 3; ++p) {
         const TU* f = u + p * N;
         uu[p] = (T)f[id];
-        gu[p][0] = d1(f, id, 1, i, L.nx, hx);
-        gu[p][1] = d1(f, id, sy, j, L.ny, hy);
-        gu[p][2] = d1(f, id, sz, k, L.nz, hz);
+        gu[p][0] = d1(f, id, 1, i, L.nx, hx, L.per & 1);
+        gu[p][1] = d1(f, id, sy, j, L.ny, hy, (L.per >> 1) & 1);
+        gu[p][2] = d1(f, id, sz, k, L.nz, hz, (L.per >> 2) & 1);
     }
     if constexpr (SYM == 1) {
         contract2_mm_stream<T>(C, cN, cid, gu, sg);
@@ -505,6 +537,57 @@ __device__ __forceinline__ void apply_interior(const T* __restrict__ u, const T*
     }
 }
 
+// Interior row at a node on the face of a periodic axis (and on no other
+// face): the row of apply_interior with the neighbours across the periodic
+// faces wrapped (nb_offsets), from the level's C (SoA) and every axis's grad-C
+// term.  Only such nodes take it; the others keep their unwrapped kernels.
+template <typename T, int SYM, typename TU = T>
+__device__ __forceinline__ void apply_interior_wrap(const TU* __restrict__ u, const T* __restrict__ C, int uniform,
+                                                 const Lvl& L, int i, int j, int k, long id, T out[3]) {
+    constexpr int NC = NComp<SYM>::value;
+    const long N = L.N;
+    const T hx = T(L.dx), hy = T(L.dy), hz = T(L.dz);
+    long om[3], op[3];
+    nb_offsets(L, i, j, k, om, op);
+    T c[NC];
+    for (int s = 0; s < NC; ++s) c[s] = C[s * N + id];
+    T gu[3][3], B[3][3][3];
+    for (int p = 0; p < 3; ++p) {
+        const TU* f = u + p * N;
+        const T c0 = (T)f[id];
+        const T xp = (T)f[id + op[0]], xm = (T)f[id + om[0]], yp = (T)f[id + op[1]], ym = (T)f[id + om[1]];
+        const T zp = (T)f[id + op[2]], zm = (T)f[id + om[2]];
+        gu[p][0] = (xp - xm) * T(0.5) / hx;
+        gu[p][1] = (yp - ym) * T(0.5) / hy;
+        gu[p][2] = (zp - zm) * T(0.5) / hz;
+        B[p][0][0] = (xp - T(2) * c0 + xm) / hx / hx;
+        B[p][1][1] = (yp - T(2) * c0 + ym) / hy / hy;
+        B[p][2][2] = (zp - T(2) * c0 + zm) / hz / hz;
+        B[p][0][1] = T(0.25) * ((T)f[id + op[0] + op[1]] + (T)f[id + om[0] + om[1]] - (T)f[id + op[0] + om[1]] -
+                                (T)f[id + om[0] + op[1]]) / (hx * hy);
+        B[p][0][2] = T(0.25) * ((T)f[id + op[0] + op[2]] + (T)f[id + om[0] + om[2]] - (T)f[id + op[0] + om[2]] -
+                                (T)f[id + om[0] + op[2]]) / (hx * hz);
+        B[p][1][2] = T(0.25) * ((T)f[id + op[1] + op[2]] + (T)f[id + om[1] + om[2]] - (T)f[id + op[1] + om[2]] -
+                                (T)f[id + om[1] + op[2]]) / (hy * hz);
+        B[p][1][0] = B[p][0][1];
+        B[p][2][0] = B[p][0][2];
+        B[p][2][1] = B[p][1][2];
+    }
+    contract3<T, SYM>(c, B, out);
+    if (!uniform) {
+        // f += sum_a ((dC/dx_a) : grad u).col(a)   (src/Operator/Elastic.cpp:245-254)
+        const T h[3] = {hx, hy, hz};
+        for (int a = 0; a < 3; ++a) {
+            T cg[NC], col[3];
+            for (int s = 0; s < NC; ++s) cg[s] = (C[s * N + id + op[a]] - C[s * N + id + om[a]]) * T(0.5) / h[a];
+            contract2_col<T, SYM>(cg, gu, a, col);
+            out[0] += col[0];
+            out[1] += col[1];
+            out[2] += col[2];
+        }
+    }
+}
+
 // Elastic<SYM>::Fapply at one node (src/Operator/Elastic.cpp:165-290); psi
 // unset (psi_avg = 1).
 template <typename T, int SYM>
@@ -514,6 +597,7 @@ __device__ __forceinline__ void apply_node(const T* __restrict__ u, const T* __r
     T nrm[3];
     const int reg = region_of<T>(i, j, k, L, nrm);
     if (reg >= 0) apply_boundary<T, SYM>(u, C, bct, L, i, j, k, id, reg, nrm, out);
+    else if (L.per && on_face(L, i, j, k)) apply_interior_wrap<T, SYM>(u, C, uniform, L, i, j, k, id, out);
     else apply_interior<T, SYM>(u, C, uniform, L, id, out, gmask);
 }
 
@@ -565,7 +649,9 @@ __global__ void __launch_bounds__(kBlock) k_diagonal(const T* __restrict__ C, co
                 for (int q = 0; q < 3; ++q) gu[q][a] = T(0);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const bool mn = pos[a] == 0, mx = pos[a] == nn[a] - 1;
+                // one-sided at the faces of a non-periodic axis (Gradient_Diagonal)
+                const bool per = (L.per >> a) & 1;
+                const bool mn = !per && pos[a] == 0, mx = !per && pos[a] == nn[a] - 1;
                 gu[p][a] = ((!mx ? T(0) : T(1)) - (!mn ? T(0) : T(1))) / ((mn || mx ? T(1) : T(2)) * h[a]);
             }
             contract2<T, SYM>(c, gu, sg);
@@ -598,31 +684,39 @@ __device__ __forceinline__ void restrict_node(const T* __restrict__ fine, const 
     int I, J, K;
     node_ijk(cid, Lc, I, J, K);
     const int hx = Lc.nx - 1, hy = Lc.ny - 1, hz = Lc.nz - 1;
-    const bool bi = (I == 0 || I == hx), bj = (J == 0 || J == hy), bk = (K == 0 || K == hz);
+    // faces of a periodic axis take the full weights, their fine neighbours
+    // across the face wrapped (the coarse domain is grown there,
+    // src/Operator/Operator.cpp:268)
+    const bool px = Lc.per & 1, py = (Lc.per >> 1) & 1, pz = (Lc.per >> 2) & 1;
+    const bool bi = !px && (I == 0 || I == hx), bj = !py && (J == 0 || J == hy), bk = !pz && (K == 0 || K == hz);
+    // the fine node's -1 and +1 neighbours: x1m = -1 or the wrapped offset, ...
+    long om[3], op[3];
+    nb_offsets(Lf, 2 * I, 2 * J, 2 * K, om, op);
+    const long x1m = om[0], x1p = op[0], sym = om[1], syp = op[1], szm = om[2], szp = op[2];
     const long sy = Lf.nx, sz = (long)Lf.nx * Lf.ny;
     const long f0 = (2L * K) * sz + (2L * J) * sy + 2L * I;
     for (int n = 0; n < ncomp; ++n) {
         const T* F = fine + n * Lf.N + f0;
         T v;
         if (bi && bj && bk) v = F[0];
-        else if (bj && bk) v = T(0.25) * F[-1] + T(0.5) * F[0] + T(0.25) * F[1];
-        else if (bk && bi) v = T(0.25) * F[-sy] + T(0.5) * F[0] + T(0.25) * F[sy];
-        else if (bi && bj) v = T(0.25) * F[-sz] + T(0.5) * F[0] + T(0.25) * F[sz];
+        else if (bj && bk) v = T(0.25) * F[x1m] + T(0.5) * F[0] + T(0.25) * F[x1p];
+        else if (bk && bi) v = T(0.25) * F[sym] + T(0.5) * F[0] + T(0.25) * F[syp];
+        else if (bi && bj) v = T(0.25) * F[szm] + T(0.5) * F[0] + T(0.25) * F[szp];
         else if (bi)
-            v = (+F[-sy - sz] + T(2) * F[-sz] + F[sy - sz] + T(2) * F[-sy] + T(4) * F[0] + T(2) * F[sy] + F[-sy + sz] +
-                 T(2) * F[sz] + F[sy + sz]) / T(16);
+            v = (+F[sym + szm] + T(2) * F[szm] + F[syp + szm] + T(2) * F[sym] + T(4) * F[0] + T(2) * F[syp] +
+                 F[sym + szp] + T(2) * F[szp] + F[syp + szp]) / T(16);
         else if (bj)
-            v = (+F[-1 - sz] + T(2) * F[-1] + F[-1 + sz] + T(2) * F[-sz] + T(4) * F[0] + T(2) * F[sz] + F[1 - sz] +
-                 T(2) * F[1] + F[1 + sz]) / T(16);
+            v = (+F[x1m + szm] + T(2) * F[x1m] + F[x1m + szp] + T(2) * F[szm] + T(4) * F[0] + T(2) * F[szp] +
+                 F[x1p + szm] + T(2) * F[x1p] + F[x1p + szp]) / T(16);
         else if (bk)
-            v = (+F[-1 - sy] + T(2) * F[-sy] + F[1 - sy] + T(2) * F[-1] + T(4) * F[0] + T(2) * F[1] + F[-1 + sy] +
-                 T(2) * F[sy] + F[1 + sy]) / T(16);
+            v = (+F[x1m + sym] + T(2) * F[sym] + F[x1p + sym] + T(2) * F[x1m] + T(4) * F[0] + T(2) * F[x1p] +
+                 F[x1m + syp] + T(2) * F[syp] + F[x1p + syp]) / T(16);
         else
-            v = (F[-1 - sy - sz] + F[-1 - sy + sz] + F[-1 + sy - sz] + F[-1 + sy + sz] + F[1 - sy - sz] +
-                 F[1 - sy + sz] + F[1 + sy - sz] + F[1 + sy + sz]) / T(64) +
-                (F[-sy - sz] + F[-sy + sz] + F[sy - sz] + F[sy + sz] + F[-1 - sz] + F[1 - sz] + F[-1 + sz] +
-                 F[1 + sz] + F[-1 - sy] + F[-1 + sy] + F[1 - sy] + F[1 + sy]) / T(32) +
-                (F[-1] + F[-sy] + F[-sz] + F[1] + F[sy] + F[sz]) / T(16) + F[0] / T(8);
+            v = (F[x1m + sym + szm] + F[x1m + sym + szp] + F[x1m + syp + szm] + F[x1m + syp + szp] +
+                 F[x1p + sym + szm] + F[x1p + sym + szp] + F[x1p + syp + szm] + F[x1p + syp + szp]) / T(64) +
+                (F[sym + szm] + F[sym + szp] + F[syp + szm] + F[syp + szp] + F[x1m + szm] + F[x1p + szm] +
+                 F[x1m + szp] + F[x1p + szp] + F[x1m + sym] + F[x1m + syp] + F[x1p + sym] + F[x1p + syp]) / T(32) +
+                (F[x1m] + F[sym] + F[szm] + F[x1p] + F[syp] + F[szp]) / T(16) + F[0] / T(8);
         crse[n * Lc.N + cid] = v;
     }
 }
@@ -911,9 +1005,9 @@ __device__ __forceinline__ void newton_gradu(const double* __restrict__ u, const
     const long sy = L.nx, sz = (long)L.nx * L.ny;
     for (int p = 0; p < 3; ++p) {
         const double* f = u + p * L.N;
-        gu[p][0] = d1(f, id, 1, i, L.nx, L.dx);
-        gu[p][1] = d1(f, id, sy, j, L.ny, L.dy);
-        gu[p][2] = d1(f, id, sz, k, L.nz, L.dz);
+        gu[p][0] = d1(f, id, 1, i, L.nx, L.dx, L.per & 1);
+        gu[p][1] = d1(f, id, sy, j, L.ny, L.dy, (L.per >> 1) & 1);
+        gu[p][2] = d1(f, id, sz, k, L.nz, L.dz, (L.per >> 2) & 1);
     }
 }
 
@@ -959,9 +1053,9 @@ __global__ void __launch_bounds__(kBlock) k_stress(const double* __restrict__ u,
     double gu[3][3], A[3][3], s[3][3], c[NC];
     for (int p = 0; p < 3; ++p) {
         const double* f = u + p * L.N;
-        gu[p][0] = d1(f, id, 1, i, L.nx, L.dx);
-        gu[p][1] = d1(f, id, sy, j, L.ny, L.dy);
-        gu[p][2] = d1(f, id, sz, k, L.nz, L.dz);
+        gu[p][0] = d1(f, id, 1, i, L.nx, L.dx, L.per & 1);
+        gu[p][1] = d1(f, id, sy, j, L.ny, L.dy, (L.per >> 1) & 1);
+        gu[p][2] = d1(f, id, sz, k, L.nz, L.dz, (L.per >> 2) & 1);
     }
     for (int p = 0; p < 3; ++p)
         for (int a = 0; a < 3; ++a) A[p][a] = gu[p][a] - (F0 ? F0[(3 * p + a) * L.N + id] : 0.0);
@@ -1050,13 +1144,16 @@ __global__ void __launch_bounds__(kBlock) k_newton_rhs(const double* __restrict_
         bc_set<double>(bct + 3 * reg, uu, gu, s, nrm, f);
         for (int d = 0; d < 3; ++d) rhs[d * L.N + id] = b[d * L.N + id] - f[d];
     } else {
-        // Numeric::Divergence, central (src/Numeric/Stencil.H:588-657)
-        const long st[3] = {1, (long)L.nx, P};
+        // Numeric::Divergence, central (src/Numeric/Stencil.H:588-657); on
+        // the faces of a periodic axis the neighbours wrap (a wrap along z
+        // needs the slab to hold the whole box)
+        long om[3], op[3];
+        nb_offsets(L, i, j, k, om, op);
         const double h[3] = {L.dx, L.dy, L.dz};
         double dv[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < 3; ++a)
             for (int r = 0; r < 3; ++r)
-                dv[r] += (dw9[(3 * r + a) * S + q + st[a]] - dw9[(3 * r + a) * S + q - st[a]]) / 2. / h[a];
+                dv[r] += (dw9[(3 * r + a) * S + q + op[a]] - dw9[(3 * r + a) * S + q + om[a]]) / 2. / h[a];
         for (int d = 0; d < 3; ++d) rhs[d * L.N + id] = b[d * L.N + id] - dv[d];
     }
 }

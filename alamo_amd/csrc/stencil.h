@@ -189,7 +189,8 @@ __global__ void __launch_bounds__(kBlock)
         id = ((long)k * L.ny + j) * L.nx + i;
         T nrm[3];
         const int reg = region_of<T>(i, j, k, L, nrm);
-        apply_boundary<T, SYM>(x, C, bct, L, i, j, k, id, reg, nrm, o);
+        if (reg >= 0) apply_boundary<T, SYM>(x, C, bct, L, i, j, k, id, reg, nrm, o);
+        else apply_interior_wrap<T, SYM>(x, C, uniform, L, i, j, k, id, o);  // a periodic face
     } else {
         const long ib = (long)(blockIdx.x - nbb);
         if (!slab_interior_node(L, ib, i, j, k, id)) return;

@@ -316,7 +316,8 @@ __device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PR
 template <class TS, typename T, int OP, bool DC, int CM, typename TX = T>
 __global__ void __launch_bounds__(TS::NT)
     k_stencil_lds(const TX* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
-                  const T* __restrict__ Ct, const T* __restrict__ Cb, const int* __restrict__ bct, int uniform,
+                  const T* __restrict__ Ct, const T* __restrict__ Cb, const T* __restrict__ Cf,
+                  const int* __restrict__ bct, int uniform,
                   Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
                   const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out,
                   const PalIdx* __restrict__ pidx) {
@@ -340,7 +341,8 @@ __global__ void __launch_bounds__(TS::NT)
         const long id = ((long)k * L.ny + j) * L.nx + i;
         T nrm[3], o[3];
         const int reg = region_of<T>(i, j, k, L, nrm);
-        apply_boundary<T, 1, TX>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
+        if (reg >= 0) apply_boundary<T, 1, TX>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
+        else apply_interior_wrap<T, 1, TX>(x, Cf, uniform, L, i, j, k, id, o);  // a periodic face: the level's C
         const unsigned n32 = (unsigned)id;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {

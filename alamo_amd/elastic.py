@@ -56,7 +56,7 @@ class Info(ctypes.Structure):
 
 EXPORTS = [
     "alamo_el_last_error", "alamo_el_default_opts", "alamo_el_create", "alamo_el_destroy",
-    "alamo_el_num_levels", "alamo_el_level_nodes", "alamo_el_set_bc", "alamo_el_set_uniform",
+    "alamo_el_num_levels", "alamo_el_level_nodes", "alamo_el_set_bc", "alamo_el_set_periodic", "alamo_el_set_uniform",
     "alamo_el_set_omega", "alamo_el_set_model", "alamo_el_prepare", "alamo_el_prepare_fp32", "alamo_el_fapply",
     "alamo_el_diagonal",
     "alamo_el_fsmooth", "alamo_el_residual", "alamo_el_restrict", "alamo_el_interp_add", "alamo_el_bc_init",
@@ -88,6 +88,7 @@ def lib():
         for name in ("alamo_el_destroy",):
             getattr(L, name).argtypes = [vp]
         L.alamo_el_set_bc.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+        L.alamo_el_set_periodic.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.alamo_el_set_uniform.argtypes = [vp, ctypes.c_int]
         L.alamo_el_set_omega.argtypes = [vp, ctypes.c_double]
         L.alamo_el_set_model.argtypes = [vp, vp, ctypes.c_int, vp]
@@ -194,6 +195,7 @@ class ElasticOperator:
         _check(lib().alamo_el_num_levels(h, ctypes.byref(nl)), "alamo_el_num_levels")
         self.num_levels = nl.value
         self._bc = _bc.Constant()
+        self.periodic = (0, 0, 0)
         self.parent = None
         self.origin = (0, 0, 0)       # global node index (x, y, z) of node 0
         self.dom = self.n_cell        # domain cells at this level
@@ -241,6 +243,17 @@ class ElasticOperator:
         _check(lib().alamo_el_set_bc(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                      v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "alamo_el_set_bc")
         self._bc = bc
+
+    def SetPeriodic(self, periodic):
+        """The periodic axes (px, py, pz), the geometry's is_periodic
+        (alamo_el_set_periodic): call before SetBC when the table has periodic
+        entries."""
+        p = np.ascontiguousarray([1 if v else 0 for v in periodic], dtype=np.int32)
+        if p.shape != (3,):
+            raise AlamoError("SetPeriodic: expected three flags")
+        _check(lib().alamo_el_set_periodic(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+               "alamo_el_set_periodic")
+        self.periodic = tuple(int(v) for v in p)
 
     def SetUniform(self, uniform: bool):
         _check(lib().alamo_el_set_uniform(self._h, int(bool(uniform))), "alamo_el_set_uniform")

@@ -143,6 +143,21 @@ int alamo_el_destroy(alamo_el_t h);
 int alamo_el_num_levels(alamo_el_t h, int* nlev);
 int alamo_el_level_nodes(alamo_el_t h, int mglev, int nodes[3]);
 
+/* Periodic axes: the geometry's is_periodic (amrex::Geometry; the operator
+ * classifies boundary rows on growPeriodicDomain(1),
+ * src/Operator/Elastic.cpp:145-149, restriction on the grown coarse domain,
+ * src/Operator/Operator.cpp:268, and Constant::Init skips those faces,
+ * src/BC/Operator/Elastic/Constant.H:160-185).  On a periodic axis the nodes
+ * 0 and n are one point: the nodes of its faces take interior rows with the
+ * neighbours across the face wrapped (below node 0 is node n - 1), gradients
+ * there are central, and the faces' regions of the boundary table are never
+ * used.  Call before alamo_el_set_bc when the table has periodic entries: a
+ * periodic entry in a region that lies off the periodic axes is EINVAL.  Not
+ * with alamo_el_set_psi, nor on an AMR patch or under one (EINVAL).  With no
+ * displacement row left (every axis periodic, or periodic axes and only
+ * traction and Neumann rows) the operator is singular. */
+int alamo_el_set_periodic(alamo_el_t h, const int periodic[3]);
+
 /* Operator::Elastic::SetBC with a BC::Operator::Elastic::Constant table
  * (src/Operator/Elastic.H:61-65, src/BC/Operator/Elastic/Constant.H). */
 int alamo_el_set_bc(alamo_el_t h, const int type_host[26 * 3], const double val_host[26 * 3]);

@@ -27,21 +27,39 @@ namespace {
 // ---------------------------------------------------------------------------
 // Grid helpers
 // ---------------------------------------------------------------------------
+// Periodic axes of every grid made after ora_set_periodic (the geometry's
+// is_periodic): the nodes 0 and n-1 of such an axis are one point, and an
+// index past either face wraps to its periodic image (the ghost nodes AMReX
+// fills through FillBoundary with the geometry's periodicity).
+int g_per[3] = {0, 0, 0};
+
 struct Grid {
     int n[3];
     double dx[3];
     long N;
+    int per[3];
     Grid(const int* nn, const double* d) {
-        for (int a = 0; a < 3; ++a) { n[a] = nn[a]; dx[a] = d ? d[a] : 1.0; }
+        for (int a = 0; a < 3; ++a) { n[a] = nn[a]; dx[a] = d ? d[a] : 1.0; per[a] = g_per[a]; }
         N = (long)n[0] * n[1] * n[2];
     }
-    long id(int i, int j, int k) const { return ((long)k * n[1] + j) * n[0] + i; }
+    int wrap(int c, int a) const {
+        if (!per[a]) return c;
+        if (c < 0) return c + (n[a] - 1);
+        if (c > n[a] - 1) return c - (n[a] - 1);
+        return c;
+    }
+    long id(int i, int j, int k) const {
+        return ((long)wrap(k, 2) * n[1] + wrap(j, 1)) * n[0] + wrap(i, 0);
+    }
+    // GetStencil on growPeriodicDomain: central on a periodic axis
+    int sten(int a, int c) const;
 };
 
 // Stencil type per axis: GetStencil (src/Numeric/Stencil.H:41-60).
 enum { ST_LO = 0, ST_HI = 1, ST_CENTRAL = 2 };
 
 inline int sten_of(int i, int nn) { return i == 0 ? ST_HI : (i == nn - 1 ? ST_LO : ST_CENTRAL); }
+int Grid::sten(int a, int c) const { return per[a] ? ST_CENTRAL : sten_of(c, n[a]); }
 
 // First derivative of component m along axis a (src/Numeric/Stencil.H:70-191).
 inline double d1(const double* f, const Grid& g, int i, int j, int k, int a, int st) {
@@ -60,8 +78,9 @@ inline double d1(const double* f, const Grid& g, int i, int j, int k, int a, int
 const double S3 = 0.57735026919, S2 = 0.70710678118;
 
 int region_of(const Grid& g, int i, int j, int k, double nrm[3]) {
-    const bool xl = i == 0, xh = i == g.n[0] - 1, yl = j == 0, yh = j == g.n[1] - 1,
-               zl = k == 0, zh = k == g.n[2] - 1;
+    // the faces of a periodic axis are inside growPeriodicDomain(1)
+    const bool xl = !g.per[0] && i == 0, xh = !g.per[0] && i == g.n[0] - 1, yl = !g.per[1] && j == 0,
+               yh = !g.per[1] && j == g.n[1] - 1, zl = !g.per[2] && k == 0, zh = !g.per[2] && k == g.n[2] - 1;
     const bool bx = xl || xh, by = yl || yh, bz = zl || zh;
     if (!(bx || by || bz)) return -1;
     const double sx = xl ? -1.0 : 1.0, sy = yl ? -1.0 : 1.0, sz = zl ? -1.0 : 1.0;
@@ -185,7 +204,7 @@ struct Op {
 void fapply_node(const Op& op, const double* u, int i, int j, int k, double out[3]) {
     const Grid& g = op.g;
     const long N = g.N, id = g.id(i, j, k);
-    const int st[3] = {sten_of(i, g.n[0]), sten_of(j, g.n[1]), sten_of(k, g.n[2])};
+    const int st[3] = {g.sten(0, i), g.sten(1, j), g.sten(2, k)};
     double uu[3], gu[3][3], c[21];
     for (int p = 0; p < 3; ++p) {
         uu[p] = u[p * N + id];
@@ -261,7 +280,7 @@ void diagonal(const Op& op, double* diag) {
                 double c[21], nrm[3];
                 gather_c(op.sym, op.C, g.N, id, c);
                 const int reg = region_of(g, i, j, k, nrm);
-                const int st[3] = {sten_of(i, g.n[0]), sten_of(j, g.n[1]), sten_of(k, g.n[2])};
+                const int st[3] = {g.sten(0, i), g.sten(1, j), g.sten(2, k)};
                 for (int p = 0; p < 3; ++p) {
                     double val;
                     if (reg >= 0) {
@@ -316,6 +335,7 @@ void restrict_field(const int* nf, int ncomp, const double* fine, double* crse) 
     const int nc[3] = {(nf[0] - 1) / 2 + 1, (nf[1] - 1) / 2 + 1, (nf[2] - 1) / 2 + 1};
     const long Nf = (long)nf[0] * nf[1] * nf[2], Nc = (long)nc[0] * nc[1] * nc[2];
     const int hx = nc[0] - 1, hy = nc[1] - 1, hz = nc[2] - 1;
+    const Grid gf(nf, nullptr);  // the fine grid, wrapping on the periodic axes
     for (int n = 0; n < ncomp; ++n) {
         const double* fd = fine + n * Nf;
         double* cd = crse + n * Nc;
@@ -324,8 +344,11 @@ void restrict_field(const int* nf, int ncomp, const double* fine, double* crse) 
             for (int J = 0; J <= hy; ++J)
                 for (int I = 0; I <= hx; ++I) {
                     const int i = 2 * I, j = 2 * J, k = 2 * K;
-                    auto F = [&](int a, int b, int c) { return fd[((long)c * nf[1] + b) * nf[0] + a]; };
-                    const bool bi = (I == 0 || I == hx), bj = (J == 0 || J == hy), bk = (K == 0 || K == hz);
+                    auto F = [&](int a, int b, int c) { return fd[gf.id(a, b, c)]; };
+                    // the coarse domain is grown along a periodic axis
+                    // (src/Operator/Operator.cpp:268): full weights there
+                    const bool bi = !gf.per[0] && (I == 0 || I == hx), bj = !gf.per[1] && (J == 0 || J == hy),
+                               bk = !gf.per[2] && (K == 0 || K == hz);
                     double v;
                     if (bi && bj && bk) v = F(i, j, k);
                     else if (bj && bk) v = 0.25 * F(i - 1, j, k) + 0.5 * F(i, j, k) + 0.25 * F(i + 1, j, k);
@@ -558,6 +581,13 @@ extern "C" {
 
 int ora_ncomp_of_sym(int sym) { return ncomp_of(sym); }
 
+// The periodic axes of every later call (process-wide; 0 0 0 restores the
+// default of no periodic axis).
+int ora_set_periodic(const int per[3]) {
+    for (int a = 0; a < 3; ++a) g_per[a] = per[a] ? 1 : 0;
+    return 0;
+}
+
 int ora_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();
@@ -646,7 +676,7 @@ int ora_newton_rhs(const int n[3], const double dx[3], int sym, const double* C,
         for (int j = 0; j < g.n[1]; ++j)
             for (int i = 0; i < g.n[0]; ++i) {
                 const long id = g.id(i, j, k);
-                const int st[3] = {sten_of(i, g.n[0]), sten_of(j, g.n[1]), sten_of(k, g.n[2])};
+                const int st[3] = {g.sten(0, i), g.sten(1, j), g.sten(2, k)};
                 double gu[3][3], A[3][3], s[3][3], c[21];
                 for (int p = 0; p < 3; ++p)
                     for (int a = 0; a < 3; ++a) gu[p][a] = d1(u + p * N, g, i, j, k, a, st[a]);
@@ -704,7 +734,7 @@ int ora_stress(const int n[3], const double dx[3], int sym, const double* C, con
         for (int j = 0; j < g.n[1]; ++j)
             for (int i = 0; i < g.n[0]; ++i) {
                 const long id = g.id(i, j, k);
-                const int st[3] = {sten_of(i, g.n[0]), sten_of(j, g.n[1]), sten_of(k, g.n[2])};
+                const int st[3] = {g.sten(0, i), g.sten(1, j), g.sten(2, k)};
                 double A[3][3], s[3][3], c[21];
                 for (int p = 0; p < 3; ++p)
                     for (int a = 0; a < 3; ++a)

@@ -25,6 +25,13 @@ extern "C" {
 
 int ora_ncomp_of_sym(int sym);
 
+/* The periodic axes of every later call (amrex::Geometry is_periodic; the
+ * operator then classifies on growPeriodicDomain, src/Operator/Elastic.cpp:
+ * 145-149, src/Operator/Operator.cpp:92,268): on a periodic axis the nodes 0
+ * and n-1 are one point, indices past a face wrap to the periodic image, and
+ * the faces carry interior rows.  Process-wide; 0 0 0 restores the default. */
+int ora_set_periodic(const int per[3]);
+
 /* Operator::Elastic<SYM>::Fapply  (src/Operator/Elastic.cpp:139-292) */
 int ora_fapply(const int n[3], const double dx[3], int sym, const double* C,
                const int* bc_type, int uniform, const double* u, double* f);

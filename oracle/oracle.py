@@ -71,6 +71,22 @@ def _check(rc, name):
     return rc
 
 
+class periodic:
+    """Context manager: the periodic axes (px, py, pz) of every oracle call
+    inside it (ora_set_periodic), none again on exit."""
+
+    def __init__(self, per):
+        self.per = np.ascontiguousarray([1 if p else 0 for p in per], dtype=np.int32)
+
+    def __enter__(self):
+        _check(lib().ora_set_periodic(_i(self.per)), "ora_set_periodic")
+        return self
+
+    def __exit__(self, *exc):
+        _check(lib().ora_set_periodic(_i(np.zeros(3, dtype=np.int32))), "ora_set_periodic")
+        return False
+
+
 def fapply(u, C, sym, bc, dx, uniform=False):
     u, C = _f64(u), _f64(C)
     f = np.empty_like(u)

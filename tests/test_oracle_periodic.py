@@ -1,0 +1,168 @@
+"""Pins of the oracle's periodic axes (amrex::Geometry is_periodic: the
+operator classifies on growPeriodicDomain, src/Operator/Elastic.cpp:145-149,
+the restriction on the grown coarse domain, src/Operator/Operator.cpp:268, and
+Constant::Init skips the periodic faces, src/BC/Operator/Elastic/Constant.H:
+160-185).  On a periodic axis of n nodes the nodes 0 and n-1 are one point and
+the field repeats with period n - 1.
+
+- Shift equivariance: rolling u and C by s nodes along a periodic axis rolls
+  Fapply, the diagonal and the residual by s, bit for bit, on every row (the
+  rows of the periodic faces included, and the boundary rows of the other
+  axes, whose table does not depend on that axis).  A wrapped neighbour that
+  lands one node off (the face's own duplicate instead of node n - 2) or a
+  one-sided derivative left on a periodic face breaks it.
+- The two copies of a face node compute the same row.
+- Restriction commutes with a shift of an even number of fine nodes.
+- A linear field u = (e_x x, 0, e_z z) with e_z = -lambda e_x / (lambda + 2 mu)
+  is the exact discrete solution of uniaxial tension along x with y periodic
+  (plane strain in y: the period forbids a mean e_yy), displacement x faces and
+  a traction-free top; the oracle's MLMG reproduces it to 1e-9."""
+import numpy as np
+import pytest
+
+from alamo_amd import bc as B
+from alamo_amd import model as M
+from oracle import oracle as O
+
+
+def periodize(a, per):
+    """Copy node 0 onto node n - 1 along every periodic axis (x, y, z)."""
+    a = np.array(a, copy=True)
+    for ax, p in enumerate(per):
+        if p:
+            axis = a.ndim - 1 - ax
+            last = [slice(None)] * a.ndim
+            first = [slice(None)] * a.ndim
+            last[axis], first[axis] = -1, 0
+            a[tuple(last)] = a[tuple(first)]
+    return a
+
+
+def roll_periodic(a, shift, ax):
+    """Roll the n - 1 distinct nodes along axis ax (0 x, 1 y, 2 z) and repeat
+    node 0 at the end."""
+    axis = a.ndim - 1 - ax
+    n = a.shape[axis]
+    core = np.roll(np.take(a, range(n - 1), axis=axis), shift, axis=axis)
+    return np.concatenate([core, np.take(core, [0], axis=axis)], axis=axis)
+
+
+def rand_case(sym, n_cell, per, seed):
+    rng = np.random.default_rng(seed)
+    shape = (n_cell[2] + 1, n_cell[1] + 1, n_cell[0] + 1)
+    nc = M.NCOMP[sym]
+    if sym == M.MAJORMINOR:
+        C = M.isotropic_slots(1.0, 1.0)[:, None, None, None] + 0.2 * rng.standard_normal((nc,) + shape)
+    else:
+        C = rng.uniform(0.5, 1.5, (nc,) + shape)
+    bc = B.Constant()
+    bc.type[:] = rng.choice([B.DISPLACEMENT, B.TRACTION, B.NEUMANN], size=(26, 3))
+    bc.val[:] = rng.standard_normal((26, 3))
+    u = rng.standard_normal((3,) + shape)
+    dx = (1.0 / n_cell[0], 0.7 / n_cell[1], 1.3 / n_cell[2])
+    return periodize(C, per), bc, periodize(u, per), dx
+
+
+PERS = [(0, 1, 0), (1, 0, 1), (1, 1, 1)]
+
+
+@pytest.mark.parametrize("per", PERS)
+@pytest.mark.parametrize("sym", [M.ISOTROPIC, M.MAJORMINOR, M.DIAGONAL])
+def test_periodic_operator_is_shift_equivariant(sym, per):
+    n_cell = (8, 10, 6)
+    C, bc, u, dx = rand_case(sym, n_cell, per, 7 + sym)
+    with O.periodic(per):
+        f = O.fapply(u, C, sym, bc, dx)
+        d = O.diagonal(C, sym, bc, dx, u.shape[1:])
+        b = np.random.default_rng(3).standard_normal(u.shape)
+        for ax in range(3):
+            if not per[ax]:
+                continue
+            for s in (1, 3):
+                fr = O.fapply(roll_periodic(u, s, ax), roll_periodic(C, s, ax), sym, bc, dx)
+                assert np.array_equal(fr, roll_periodic(f, s, ax)), (ax, s)
+                dr = O.diagonal(roll_periodic(C, s, ax), sym, bc, dx, u.shape[1:])
+                assert np.array_equal(dr, roll_periodic(d, s, ax)), (ax, s)
+            # the duplicate face node computes the same row as node 0
+            axis = f.ndim - 1 - ax
+            assert np.array_equal(np.take(f, [0], axis=axis), np.take(f, [-1], axis=axis))
+        r = O.residual(u, b, C, sym, bc, dx)
+        assert np.array_equal(r, b + (-1.0) * f)
+
+
+def test_periodic_face_rows_are_interior_rows():
+    """On a periodic axis the faces carry the interior row: with C uniform and
+    u = sin(2 pi y / Ly) the y-periodic Fapply of component y is
+    -(lambda + 2 mu) (2 pi / Ly)^2 u times the second-difference factor on every
+    node off the x and z faces, the face nodes of y included."""
+    n = (6, 12, 6)
+    lam, mu = 1.3, 0.7
+    shape = (n[2] + 1, n[1] + 1, n[0] + 1)
+    C = np.stack([np.full(shape, lam), np.full(shape, mu)])
+    Ly = 1.0
+    h = Ly / n[1]
+    y = np.arange(n[1] + 1) * h
+    u = np.zeros((3,) + shape)
+    u[1] = np.sin(2 * np.pi * y / Ly)[None, :, None]
+    bc = B.Constant()
+    with O.periodic((0, 1, 0)):
+        f = O.fapply(u, C, M.ISOTROPIC, bc, (1.0 / n[0], h, 1.0 / n[2]), uniform=True)
+    # the central second difference of sin: -(4 / h^2) sin^2(pi h / Ly) sin(.)
+    expect = -(lam + 2 * mu) * 4.0 / h ** 2 * np.sin(np.pi * h / Ly) ** 2 * u[1]
+    inner = (slice(1, -1), slice(None), slice(1, -1))
+    assert np.allclose(f[1][inner], expect[inner], rtol=1e-12, atol=1e-9)
+    assert np.abs(f[0][inner]).max() < 1e-9 and np.abs(f[2][inner]).max() < 1e-9
+
+
+@pytest.mark.parametrize("per", PERS)
+def test_periodic_restriction_commutes_with_even_shifts(per):
+    n_cell = (8, 12, 8)
+    rng = np.random.default_rng(5)
+    f = periodize(rng.standard_normal((3, n_cell[2] + 1, n_cell[1] + 1, n_cell[0] + 1)), per)
+    with O.periodic(per):
+        c = O.restrict(f)
+        for ax in range(3):
+            if per[ax]:
+                assert np.array_equal(O.restrict(roll_periodic(f, 2, ax)), roll_periodic(c, 1, ax))
+        # a field constant along the periodic axes restricts to the same constant there
+        g = np.ones_like(f)
+        assert np.allclose(O.restrict(g), 1.0, rtol=0, atol=1e-15)
+
+
+def periodic_tension_case(n_cell, e_x=1e-3, lam=1.2, mu=0.8):
+    """Uniaxial tension along x, y periodic: the table and the exact linear
+    solution (x faces: displacement in x and y, traction-free in z; z low:
+    traction-free in x and y, u_z = 0; z high traction-free)."""
+    L = (1.0, 0.75, 1.25)
+    dx = tuple(L[a] / n_cell[a] for a in range(3))
+    e_z = -lam * e_x / (lam + 2 * mu)
+    bc = B.Constant()
+    bc.type[:] = B.PERIODIC  # every region on a y face never occurs
+    T, D = B.TRACTION, B.DISPLACEMENT
+    xhi = e_x * L[0]
+    rows = {"xlo": ((D, 0), (D, 0), (T, 0)), "xhi": ((D, xhi), (D, 0), (T, 0)),
+            "zlo": ((T, 0), (T, 0), (D, 0)), "zhi": ((T, 0), (T, 0), (T, 0)),
+            "zloxlo": ((D, 0), (D, 0), (D, 0)), "zloxhi": ((D, xhi), (D, 0), (D, 0)),
+            "zhixlo": ((D, 0), (D, 0), (T, 0)), "zhixhi": ((D, xhi), (D, 0), (T, 0))}
+    for face, comps in rows.items():
+        for c, (t, v) in enumerate(comps):
+            bc.set(face, c, t, v)
+    shape = (n_cell[2] + 1, n_cell[1] + 1, n_cell[0] + 1)
+    C = np.stack([np.full(shape, lam), np.full(shape, mu)])
+    z, y, x = np.meshgrid(*(np.arange(shape[q]) * dx[2 - q] for q in range(3)), indexing="ij")
+    u = np.stack([e_x * x, np.zeros_like(x), e_z * z])
+    return C, bc, dx, u
+
+
+def test_periodic_uniaxial_tension_closed_form():
+    n_cell = (16, 16, 16)
+    C, bc, dx, u_exact = periodic_tension_case(n_cell)
+    shape = u_exact.shape[1:]
+    with O.periodic((0, 1, 0)):
+        # the exact field satisfies every discrete row: its residual is rounding
+        b = O.bc_init(shape, bc)
+        r = O.residual(u_exact, b, C, M.ISOTROPIC, bc, dx)
+        assert np.abs(r).max() < 1e-12
+        x, info = O.mlmg_solve(b, C, M.ISOTROPIC, bc, dx, tol_rel=1e-12, cycle="V")
+    assert info["status"] == 0
+    assert np.abs(x - u_exact).max() < 1e-9 * np.abs(u_exact).max()

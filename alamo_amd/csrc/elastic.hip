@@ -239,18 +239,24 @@ bool launch_tiled(alamo_el_t h, const Level& l, const TX* x, const T* b, const T
     // without, 12.6 us with).
     const ael::TileGrid& G = sizeof(T) == 8 ? l.tg : l.tg32;
     const unsigned grid = nbb + ael::tile_blocks(G);
-    if (l.pidx && vecs<T>(lv).pal)
-        ael::k_stencil_lds<TileT, T, OP, true, 2, TX><<<grid, TileT::NT, 0, s>>>(
-            x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
-            l.nflag, out, l.pidx);
-    else if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
-        ael::k_stencil_lds<TileT, T, OP, true, 1, TX><<<grid, TileT::NT, 0, s>>>(
-            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
-            l.nflag, out, nullptr);
-    else
-        ael::k_stencil_lds<TileT, T, OP, true, 0, TX><<<grid, TileT::NT, 0, s>>>(
-            x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase, vecs<T>(lv).D,
-            l.nflag, out, nullptr);
+    // the variant with the periodic faces' wrapped rows only on a periodic level
+    auto go = [&](auto per_tag) {
+        constexpr bool PER = decltype(per_tag)::value;
+        if (l.pidx && vecs<T>(lv).pal)
+            ael::k_stencil_lds<TileT, T, OP, true, 2, TX, PER><<<grid, TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
+                vecs<T>(lv).D, l.nflag, out, l.pidx);
+        else if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
+            ael::k_stencil_lds<TileT, T, OP, true, 1, TX, PER><<<grid, TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
+                vecs<T>(lv).D, l.nflag, out, nullptr);
+        else
+            ael::k_stencil_lds<TileT, T, OP, true, 0, TX, PER><<<grid, TileT::NT, 0, s>>>(
+                x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
+                vecs<T>(lv).D, l.nflag, out, nullptr);
+    };
+    if (L.per) go(std::true_type{});
+    else go(std::false_type{});
     return true;
 }
 
@@ -281,14 +287,29 @@ void launch_stencil_raw(alamo_el_t h, const Level& l, const T* x, const T* b, co
     const int* dwb = dc ? l.dwbase : nullptr;
     const T* Dc = dc ? vecs<T>(lv).D : nullptr;
     const int nf = dc ? l.nflag : 0;
-    switch (h->sym) {
-        case 0: ael::k_stencil_slab<T, 0, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out); break;
-        case 1:
-            if (dc) ael::k_stencil_slab<T, 1, OP, true><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out);
-            else ael::k_stencil_slab<T, 1, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out);
-            break;
-        default: ael::k_stencil_slab<T, 2, OP><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb, dwb, Dc, nf, out); break;
-    }
+    auto go = [&](auto per_tag) {
+        constexpr bool P = decltype(per_tag)::value;
+        switch (h->sym) {
+            case 0:
+                ael::k_stencil_slab<T, 0, OP, false, P><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb,
+                                                                          dwb, Dc, nf, out);
+                break;
+            case 1:
+                if (dc)
+                    ael::k_stencil_slab<T, 1, OP, true, P><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm,
+                                                                             nbb, dwb, Dc, nf, out);
+                else
+                    ael::k_stencil_slab<T, 1, OP, false, P><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm,
+                                                                              nbb, dwb, Dc, nf, out);
+                break;
+            default:
+                ael::k_stencil_slab<T, 2, OP, false, P><<<g, kBlock, 0, s>>>(x, b, diag, C, h->bct_dev, u, L, w, gm, nbb,
+                                                                          dwb, Dc, nf, out);
+                break;
+        }
+    };
+    if (L.per) go(std::true_type{});
+    else go(std::false_type{});
 }
 
 // Operator rows with the psi weight (psi_kernels.h) when SetPsi was called:

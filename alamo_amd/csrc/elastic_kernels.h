@@ -172,11 +172,13 @@ __device__ __forceinline__ void contract3(const T* c, const T B[3][3][3], T f[3]
 // Boundary region in BC::Operator::Elastic::Elastic::Face order, classified
 // as Constant::operator() does (src/BC/Operator/Elastic/Constant.H:373-400);
 // normals use the truncated constants of src/BC/Operator/Elastic/Elastic.H:79-80.
-template <typename T>
+// PER false: the caller knows that no axis of L is periodic (the stencil
+// kernels' variants without the wrap).
+template <typename T, bool PER = true>
 __device__ __forceinline__ int region_of(int i, int j, int k, const Lvl& L, T nrm[3]) {
     // the faces of a periodic axis are no boundary (Elastic<SYM>::Fapply
     // classifies on growPeriodicDomain(1), src/Operator/Elastic.cpp:145)
-    const bool px = L.per & 1, py = (L.per >> 1) & 1, pz = (L.per >> 2) & 1;
+    const bool px = PER && (L.per & 1), py = PER && ((L.per >> 1) & 1), pz = PER && ((L.per >> 2) & 1);
     const bool xl = !px && i == 0, xh = !px && i == L.nx - 1, yl = !py && j == 0, yh = !py && j == L.ny - 1,
                zl = !pz && k == 0, zh = !pz && k == L.nz - 1;
     const bool bx = xl || xh, by = yl || yh, bz = zl || zh;
@@ -448,8 +450,8 @@ __device__ __forceinline__ void contract2_mm_stream(const T* __restrict__ C, lon
 
 // Boundary row of Fapply at node (i, j, k) of region reg with normal nrm:
 // one-sided gradients, sigma = C : grad u, and the Constant BC row.  u may be
-// narrower than T (widened exactly).
-template <typename T, int SYM, typename TU = T>
+// narrower than T (widened exactly).  PER false: no axis of L is periodic.
+template <typename T, int SYM, typename TU = T, bool PER = true>
 __device__ __forceinline__ void apply_boundary(const TU* __restrict__ u, const T* __restrict__ C,
                                                const int* __restrict__ bct, const Lvl& L, int i, int j, int k,
                                                long id, int reg, const T nrm[3], T out[3], long cN = -1,
@@ -467,9 +469,9 @@ __device__ __forceinline__ void apply_boundary(const TU* __restrict__ u, const T
     for (int p = 0; p < 3; ++p) {
         const TU* f = u + p * N;
         uu[p] = (T)f[id];
-        gu[p][0] = d1(f, id, 1, i, L.nx, hx, L.per & 1);
-        gu[p][1] = d1(f, id, sy, j, L.ny, hy, (L.per >> 1) & 1);
-        gu[p][2] = d1(f, id, sz, k, L.nz, hz, (L.per >> 2) & 1);
+        gu[p][0] = d1(f, id, 1, i, L.nx, hx, PER && (L.per & 1));
+        gu[p][1] = d1(f, id, sy, j, L.ny, hy, PER && ((L.per >> 1) & 1));
+        gu[p][2] = d1(f, id, sz, k, L.nz, hz, PER && ((L.per >> 2) & 1));
     }
     if constexpr (SYM == 1) {
         contract2_mm_stream<T>(C, cN, cid, gu, sg);
@@ -678,22 +680,22 @@ __global__ void __launch_bounds__(kBlock) k_diagonal(const T* __restrict__ C, co
 // Nodal full-weighting restriction of ncomp components from level Lf to Lc
 // (src/Operator/Operator.cpp:280-353); the same weights average C between MG
 // levels (src/Operator/Elastic.cpp:786-834).
-template <typename T>
-__device__ __forceinline__ void restrict_node(const T* __restrict__ fine, const Lvl& Lf, T* __restrict__ crse,
-                                              const Lvl& Lc, int ncomp, long cid) {
+template <typename T, bool WRAP>
+__device__ __forceinline__ void restrict_node_w(const T* __restrict__ fine, const Lvl& Lf, T* __restrict__ crse,
+                                                const Lvl& Lc, int ncomp, long cid) {
     int I, J, K;
     node_ijk(cid, Lc, I, J, K);
     const int hx = Lc.nx - 1, hy = Lc.ny - 1, hz = Lc.nz - 1;
     // faces of a periodic axis take the full weights, their fine neighbours
     // across the face wrapped (the coarse domain is grown there,
     // src/Operator/Operator.cpp:268)
-    const bool px = Lc.per & 1, py = (Lc.per >> 1) & 1, pz = (Lc.per >> 2) & 1;
+    const bool px = WRAP && (Lc.per & 1), py = WRAP && ((Lc.per >> 1) & 1), pz = WRAP && ((Lc.per >> 2) & 1);
     const bool bi = !px && (I == 0 || I == hx), bj = !py && (J == 0 || J == hy), bk = !pz && (K == 0 || K == hz);
-    // the fine node's -1 and +1 neighbours: x1m = -1 or the wrapped offset, ...
-    long om[3], op[3];
-    nb_offsets(Lf, 2 * I, 2 * J, 2 * K, om, op);
-    const long x1m = om[0], x1p = op[0], sym = om[1], syp = op[1], szm = om[2], szp = op[2];
     const long sy = Lf.nx, sz = (long)Lf.nx * Lf.ny;
+    // the fine node's -1 and +1 neighbours, wrapped on a periodic axis
+    long om[3] = {-1, -sy, -sz}, op[3] = {1, sy, sz};
+    if constexpr (WRAP) nb_offsets(Lf, 2 * I, 2 * J, 2 * K, om, op);
+    const long x1m = om[0], x1p = op[0], sym = om[1], syp = op[1], szm = om[2], szp = op[2];
     const long f0 = (2L * K) * sz + (2L * J) * sy + 2L * I;
     for (int n = 0; n < ncomp; ++n) {
         const T* F = fine + n * Lf.N + f0;
@@ -719,6 +721,12 @@ __device__ __forceinline__ void restrict_node(const T* __restrict__ fine, const 
                 (F[x1m] + F[sym] + F[szm] + F[x1p] + F[syp] + F[szp]) / T(16) + F[0] / T(8);
         crse[n * Lc.N + cid] = v;
     }
+}
+template <typename T>
+__device__ __forceinline__ void restrict_node(const T* __restrict__ fine, const Lvl& Lf, T* __restrict__ crse,
+                                              const Lvl& Lc, int ncomp, long cid) {
+    if (Lc.per) restrict_node_w<T, true>(fine, Lf, crse, Lc, ncomp, cid);
+    else restrict_node_w<T, false>(fine, Lf, crse, Lc, ncomp, cid);
 }
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_restrict(const T* __restrict__ fine, Lvl Lf, T* __restrict__ crse,

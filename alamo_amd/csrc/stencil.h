@@ -173,7 +173,9 @@ __global__ void __launch_bounds__(kBlock) k_gradc_fill(const double* __restrict_
 // grad-C contribution of a clear axis is exactly zero and its 2 x NC
 // neighbour loads are skipped.  DC (MajorMinor): the grad-C term of the
 // flagged nodes is D e1 with D read from the compact array Dc.
-template <typename T, int SYM, int OP, bool DC = false>
+// PER: L has periodic axes (their face nodes take the wrapped interior row);
+// the variant without is the one every non-periodic level runs.
+template <typename T, int SYM, int OP, bool DC = false, bool PER = false>
 __global__ void __launch_bounds__(kBlock)
     k_stencil_slab(const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
                    const T* __restrict__ C, const int* __restrict__ bct, int uniform, Lvl L, T omega,
@@ -188,9 +190,13 @@ __global__ void __launch_bounds__(kBlock)
         boundary_node(L, t, i, j, k);
         id = ((long)k * L.ny + j) * L.nx + i;
         T nrm[3];
-        const int reg = region_of<T>(i, j, k, L, nrm);
-        if (reg >= 0) apply_boundary<T, SYM>(x, C, bct, L, i, j, k, id, reg, nrm, o);
-        else apply_interior_wrap<T, SYM>(x, C, uniform, L, i, j, k, id, o);  // a periodic face
+        const int reg = region_of<T, PER>(i, j, k, L, nrm);
+        if constexpr (PER) {
+            if (reg >= 0) apply_boundary<T, SYM, T, true>(x, C, bct, L, i, j, k, id, reg, nrm, o);
+            else apply_interior_wrap<T, SYM>(x, C, uniform, L, i, j, k, id, o);  // a periodic face
+        } else {
+            apply_boundary<T, SYM, T, false>(x, C, bct, L, i, j, k, id, reg, nrm, o);
+        }
     } else {
         const long ib = (long)(blockIdx.x - nbb);
         if (!slab_interior_node(L, ib, i, j, k, id)) return;

@@ -312,8 +312,10 @@ __device__ __forceinline__ void tile_store(T* __restrict__ s, const T pre[TS::PR
 // CM: how the interior nodes read C -- 0 the tile-blocked copy Ct, 1 the same
 // with the non-temporal hint, 2 the palette (Ct is then the table, pidx the
 // tile-blocked row indices).  TX: the type of x, narrower than T only for
-// Fapply (an FP32 field in FP64 rows, widened exactly on load).
-template <class TS, typename T, int OP, bool DC, int CM, typename TX = T>
+// Fapply (an FP32 field in FP64 rows, widened exactly on load).  PER: L has
+// periodic axes (their face nodes, in the boundary workgroups, take the
+// wrapped interior row from the level's C, Cf).
+template <class TS, typename T, int OP, bool DC, int CM, typename TX = T, bool PER = false>
 __global__ void __launch_bounds__(TS::NT)
     k_stencil_lds(const TX* __restrict__ x, const T* __restrict__ b, const T* __restrict__ diag,
                   const T* __restrict__ Ct, const T* __restrict__ Cb, const T* __restrict__ Cf,
@@ -340,9 +342,13 @@ __global__ void __launch_bounds__(TS::NT)
         boundary_node(L, t, i, j, k);
         const long id = ((long)k * L.ny + j) * L.nx + i;
         T nrm[3], o[3];
-        const int reg = region_of<T>(i, j, k, L, nrm);
-        if (reg >= 0) apply_boundary<T, 1, TX>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
-        else apply_interior_wrap<T, 1, TX>(x, Cf, uniform, L, i, j, k, id, o);  // a periodic face: the level's C
+        const int reg = region_of<T, PER>(i, j, k, L, nrm);
+        if constexpr (PER) {
+            if (reg >= 0) apply_boundary<T, 1, TX, true>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
+            else apply_interior_wrap<T, 1, TX>(x, Cf, uniform, L, i, j, k, id, o);  // a periodic face: the level's C
+        } else {
+            apply_boundary<T, 1, TX, false>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
+        }
         const unsigned n32 = (unsigned)id;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {

@@ -72,8 +72,8 @@ def log(msg):
 # (CM 1: the tile-blocked copy, non-temporal loads), whose HBM traffic carries
 # the algorithmic bytes; the variant that reads C through the palette of the
 # grains' distinct tensors (CM 2) moves far fewer bytes and is reported beside it.
-FAPPLY_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 1, double>"
-FAPPLY_PALETTE_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 2, double>"
+FAPPLY_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 1, double, false>"
+FAPPLY_PALETTE_KERNEL = "k_stencil_lds<ael::Tile<32, 8>, double, 0, true, 2, double, false>"
 COPY_BYTES = 1 << 28  # calibration copy: 256 MiB read and 256 MiB written
 # F-cycles of the FP64 oracle timed for the CPU baseline (about 9 s on 16
 # host threads at 129^3), scaled to the plain FP64 solve's cycle count

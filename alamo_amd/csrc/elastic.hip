@@ -82,6 +82,7 @@ struct Level {
     bool tiled = false;              // MajorMinor level served by k_stencil_lds (else k_stencil_slab)
     double* psi = nullptr;           // SetPsi weight on this level's cells (null until set)
     ael::PalIdx* pidx = nullptr;     // palette row of every interior node, tile-blocked (null: no palette)
+    ael::PalIdx* pidx_b = nullptr;   // palette row of every boundary node, in boundary_node order
     long dcap = 0, fcap = 0;         // capacities (entries) of d.D and f.D, kept across models
     long nwcap = 0;                  // capacity of dwbase
     int npal = 0;
@@ -245,15 +246,15 @@ bool launch_tiled(alamo_el_t h, const Level& l, const TX* x, const T* b, const T
         if (l.pidx && vecs<T>(lv).pal)
             ael::k_stencil_lds<TileT, T, OP, true, 2, TX, PER><<<grid, TileT::NT, 0, s>>>(
                 x, b, diag, vecs<T>(lv).pal, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out, l.pidx);
+                vecs<T>(lv).D, l.nflag, out, l.pidx, l.pidx_b);
         else if ((long)sizeof(T) * 21 * L.N >= kNtMinBytes)
             ael::k_stencil_lds<TileT, T, OP, true, 1, TX, PER><<<grid, TileT::NT, 0, s>>>(
                 x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out, nullptr);
+                vecs<T>(lv).D, l.nflag, out, nullptr, nullptr);
         else
             ael::k_stencil_lds<TileT, T, OP, true, 0, TX, PER><<<grid, TileT::NT, 0, s>>>(
                 x, b, diag, vecs<T>(lv).Ct, vecs<T>(lv).Cb, vecs<T>(lv).C, h->bct_dev, u, L, w, gm, nbb, G, l.dwbase,
-                vecs<T>(lv).D, l.nflag, out, nullptr);
+                vecs<T>(lv).D, l.nflag, out, nullptr, nullptr);
     };
     if (L.per) go(std::true_type{});
     else go(std::false_type{});
@@ -896,8 +897,10 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
         if (r) {  // an error leaves the level on its C copy, with nothing half built
             if (l.d.pal) (void)hipFree(l.d.pal);
             if (l.pidx) (void)hipFree(l.pidx);
+            if (l.pidx_b) (void)hipFree(l.pidx_b);
             l.d.pal = nullptr;
             l.pidx = nullptr;
+            l.pidx_b = nullptr;
         }
         return r;
     };
@@ -905,7 +908,8 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
         hipMalloc((void**)&rep_d, sizeof(long) * npal) != hipSuccess ||
         hipMalloc((void**)&bad_d, sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&l.d.pal, sizeof(double) * npal * ael::kPalStride) != hipSuccess ||
-        hipMalloc((void**)&l.pidx, sizeof(ael::PalIdx) * ael::TileIdx<TileT>::elems(l.L, l.tg)) != hipSuccess)
+        hipMalloc((void**)&l.pidx, sizeof(ael::PalIdx) * ael::TileIdx<TileT>::elems(l.L, l.tg)) != hipSuccess ||
+        hipMalloc((void**)&l.pidx_b, sizeof(ael::PalIdx) * std::max(1L, ael::boundary_count(l.L))) != hipSuccess)
         return done(fail(ALAMO_EL_EHIP, "palette allocation failed"));
     int bad = 0;
     if (hipMemcpyAsync(ph_d, ph.data(), sizeof(unsigned long long) * npal, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -915,6 +919,9 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
     ael::k_pal_gather<double><<<nblk((long)npal * ael::kPalStride), kBlock, 0, s>>>(l.d.C, N, rep_d, npal, l.d.pal);
     ael::k_pal_index<TileT><<<(unsigned)ael::tile_planes(l.L, l.tg), TileT::NT, 0, s>>>(
         l.d.C, l.L, l.tg, hs, ph_d, npal, l.d.pal, l.pidx, bad_d);
+    if (ael::boundary_count(l.L) > 0)
+        ael::k_pal_index_b<<<nblk(ael::boundary_count(l.L)), kBlock, 0, s>>>(l.d.C, l.L, hs, ph_d, npal, l.d.pal,
+                                                                             l.pidx_b, bad_d);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(&bad, bad_d, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
@@ -922,8 +929,10 @@ int build_palette(alamo_el_t h, Level& l, hipStream_t s) {
     if (bad) {  // a hash collision: keep the C copy
         (void)hipFree(l.d.pal);
         (void)hipFree(l.pidx);
+        (void)hipFree(l.pidx_b);
         l.d.pal = nullptr;
         l.pidx = nullptr;
+        l.pidx_b = nullptr;
         return done(0);
     }
     l.npal = npal;
@@ -946,11 +955,12 @@ int build_gradc(alamo_el_t h, Level& l, hipStream_t s) {
     // The tile copies and dwbase depend on the grid only and stay allocated
     // from one model to the next (their contents are rebuilt here and, for the
     // FP32 copies, by prepare32); the compact D arrays grow when needed.
-    for (void* p : {(void*)l.d.pal, (void*)l.f.pal, (void*)l.pidx})
+    for (void* p : {(void*)l.d.pal, (void*)l.f.pal, (void*)l.pidx, (void*)l.pidx_b})
         if (p) (void)hipFree(p);
     l.d.pal = nullptr;
     l.f.pal = nullptr;
     l.pidx = nullptr;
+    l.pidx_b = nullptr;
     l.npal = 0;
     l.nflag = 0;
     if (!l.tiled) {
@@ -1325,6 +1335,7 @@ int alamo_el_destroy(alamo_el_t h) {
                          l.f.pal})
             if (p) (void)hipFree(p);
         if (l.pidx) (void)hipFree(l.pidx);
+        if (l.pidx_b) (void)hipFree(l.pidx_b);
         if (l.gmask) (void)hipFree(l.gmask);
         if (l.dwbase) (void)hipFree(l.dwbase);
         if (l.psi) (void)hipFree(l.psi);

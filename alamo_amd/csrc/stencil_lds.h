@@ -160,6 +160,31 @@ __global__ void __launch_bounds__(TS::NT) k_pal_index(const double* __restrict__
     }
     pidx[TileIdx<TS>::block(G, k, it, jt) + threadIdx.x] = q8;
 }
+// The palette row of every boundary node, in boundary_node order (the
+// boundary workgroups' order), checked like k_pal_index.
+__global__ void __launch_bounds__(kBlock) k_pal_index_b(const double* __restrict__ C, Lvl L,
+                                                        const unsigned long long* __restrict__ hs,
+                                                        const unsigned long long* __restrict__ ph, int npal,
+                                                        const double* __restrict__ pal, PalIdx* __restrict__ pidx_b,
+                                                        int* __restrict__ bad) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= boundary_count(L)) return;
+    int i, j, k;
+    boundary_node(L, t, i, j, k);
+    const long id = ((long)k * L.ny + j) * L.nx + i;
+    const unsigned long long h = hs[id];
+    int lo = 0, hi = npal - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ph[mid] < h) lo = mid + 1;
+        else hi = mid;
+    }
+    bool ok = ph[lo] == h;
+    for (int q = 0; q < 21 && ok; ++q)
+        ok = __double_as_longlong(C[q * L.N + id]) == __double_as_longlong(pal[lo * kPalStride + q]);
+    if (!ok) bad[0] = 1;
+    pidx_b[t] = (PalIdx)lo;
+}
 // The 21 slots of palette row pr (16-byte loads).
 template <typename T>
 __device__ __forceinline__ void pal_row(const T* __restrict__ pr, T cv[21]) {
@@ -322,7 +347,7 @@ __global__ void __launch_bounds__(TS::NT)
                   const int* __restrict__ bct, int uniform,
                   Lvl L, T omega, const unsigned char* __restrict__ gmask, unsigned nbb, TileGrid G,
                   const int* __restrict__ dwbase, const T* __restrict__ Dc, int nflag, T* __restrict__ out,
-                  const PalIdx* __restrict__ pidx) {
+                  const PalIdx* __restrict__ pidx, const PalIdx* __restrict__ pidx_b) {
     static_assert(OP == 0 || std::is_same<T, TX>::value, "a narrower x is for Fapply only");
     __shared__ T su[3][TS::RING];  // [ring slot][component][hy][hx], padded to RING
     // The nbb boundary workgroups (a multiple of kXcd, so that tile ib keeps
@@ -346,6 +371,11 @@ __global__ void __launch_bounds__(TS::NT)
         if constexpr (PER) {
             if (reg >= 0) apply_boundary<T, 1, TX, true>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
             else apply_interior_wrap<T, 1, TX>(x, Cf, uniform, L, i, j, k, id, o);  // a periodic face: the level's C
+        } else if constexpr (CM == 2) {
+            // the node's tensor from the palette (Ct is the table): the values
+            // of the compact copy, so the same bits, from a row in the caches
+            apply_boundary<T, 1, TX, false>(x, Ct + (size_t)pidx_b[t] * kPalStride, bct, L, i, j, k, id, reg, nrm, o,
+                                            1, 0);
         } else {
             apply_boundary<T, 1, TX, false>(x, Cb, bct, L, i, j, k, id, reg, nrm, o, boundary_count(L), t);
         }

@@ -1197,7 +1197,7 @@ constexpr int kLearnedSmooth = 2;  // Fsmooth calls applied to a learned correct
 // kLearnedSmooth Fsmooth calls on A e = r; the step length minimizes the
 // 2-norm of r - g A e, and the step is taken only when it lowers |r|_inf (the
 // multigrid residual is the judge).  *taken tells which.
-int learned_step(alamo_el_t h, double* x, double* rnorm, int* taken, hipStream_t s) {
+int learned_step(alamo_el_t h, double* x, double* rnorm, int* taken, bool fp32, hipStream_t s) {
     Level& l0 = h->lv[0];
     const long M0 = 3 * l0.L.N;
     double* r = l0.d.res;
@@ -1206,24 +1206,40 @@ int learned_step(alamo_el_t h, double* x, double* rnorm, int* taken, hipStream_t
     int rc;
     *taken = 0;
     if ((rc = learned_apply(h, r, e, s))) return rc;
-    for (int q = 0; q < kLearnedSmooth; ++q) {
-        launch_jacobi<double>(h, l0, e, r, l0.d.diag, l0.d.C, l0.d.tmp, s);
-        launch_jacobi<double>(h, l0, l0.d.tmp, r, l0.d.diag, l0.d.C, e, s);
+    if (fp32) {
+        // the smoothing of the proposal in the FP32 cycles' precision (the
+        // FP64 residual below judges the e it leaves)
+        ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(r, l0.f.res, M0);
+        ael::k_cast<double, float><<<nblk(M0), kBlock, 0, s>>>(e, l0.f.cor, M0);
+        for (int q = 0; q < kLearnedSmooth; ++q) {
+            launch_jacobi<float>(h, l0, l0.f.cor, l0.f.res, l0.f.diag, l0.f.C, l0.f.tmp, s);
+            launch_jacobi<float>(h, l0, l0.f.tmp, l0.f.res, l0.f.diag, l0.f.C, l0.f.cor, s);
+        }
+        ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, e, M0);
+    } else {
+        for (int q = 0; q < kLearnedSmooth; ++q) {
+            launch_jacobi<double>(h, l0, e, r, l0.d.diag, l0.d.C, l0.d.tmp, s);
+            launch_jacobi<double>(h, l0, l0.d.tmp, r, l0.d.diag, l0.d.C, e, s);
+        }
     }
     launch_fapply<double>(h, l0, e, l0.d.C, w, s);
-    double rw, ww;
-    if ((rc = reduce_host<double>(h, r, w, M0, 1, &rw, s))) return rc;
-    if ((rc = reduce_host<double>(h, w, w, M0, 1, &ww, s))) return rc;
+    // r . w and w . w in one pass, one copy to the host
+    const int nb = (int)std::min<long>(kRedBlocks, std::max<long>(1, nblk(M0)));
+    ael::k_dot2_partial<<<nb, kBlock, 0, s>>>(r, w, M0, h->red_partial);
+    ael::k_multidot_final<<<2, kBlock, 0, s>>>(h->red_partial, nb, h->red_out);
+    HIPCHK(hipMemcpyAsync(h->red_host, h->red_out, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double rw = h->red_host[0], ww = h->red_host[1];
     if (!(ww > 0.0) || std::isnan(rw)) return 0;
     const double g = rw / ww;
-    // w = r - g A e, the residual after the step
-    ael::k_scale_copy<double><<<nblk(M0), kBlock, 0, s>>>(w, -g, w, M0);
-    launch_axpy<double>(w, 1.0, r, M0, s);
-    double rn;
-    if ((rc = reduce_host<double>(h, w, w, M0, 0, &rn, s))) return rc;
+    // w = r - g A e, the residual after the step, and its max-norm
+    ael::k_step_residual<<<nb, kBlock, 0, s>>>(w, g, r, M0, h->red_partial);
+    ael::k_reduce_final<<<1, kBlock, 0, s>>>(h->red_partial, nb, 0, h->red_out);
+    HIPCHK(hipMemcpyAsync(h->red_host, h->red_out, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double rn = h->red_host[0];
     if (!(rn < *rnorm)) return 0;
-    launch_axpy<double>(x, g, e, M0, s);
-    launch_copy<double>(r, w, M0, s);
+    ael::k_take_step<<<nblk(M0), kBlock, 0, s>>>(x, g, e, r, w, M0);
     HIPCHK(hipGetLastError());
     *rnorm = rn;
     *taken = 1;
@@ -1310,7 +1326,8 @@ int alamo_el_create(const int n_cell[3], const double dx[3], int sym, int max_co
     int rc;
     if ((rc = alloc((void**)&h->bct_dev, sizeof(int) * 78))) return cleanup(rc);
     if ((rc = alloc((void**)&h->bcv_dev, sizeof(double) * 78))) return cleanup(rc);
-    if ((rc = alloc((void**)&h->red_partial, sizeof(double) * kRedBlocks))) return cleanup(rc);
+    // two slots of block partials (k_dot2_partial), kRedBlocks each
+    if ((rc = alloc((void**)&h->red_partial, sizeof(double) * 2 * kRedBlocks))) return cleanup(rc);
     if ((rc = alloc((void**)&h->red_out, sizeof(double) * 2))) return cleanup(rc);
     if ((rc = alloc((void**)&h->bottom_work, sizeof(double) * 10 * Mb))) return cleanup(rc);
     if ((rc = alloc((void**)&h->bottom_status, sizeof(int) * 4))) return cleanup(rc);
@@ -1867,7 +1884,7 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
     const bool learned_pre = (o.learned & ALAMO_EL_LEARNED_PRECOND) && h->nn;
     if ((o.learned & ALAMO_EL_LEARNED_GUESS) && h->nn && resnorm0 > target) {
         int taken;
-        if ((rc = learned_step(h, x, &nrm, &taken, s))) return rc;
+        if ((rc = learned_step(h, x, &nrm, &taken, o.precision == ALAMO_EL_PREC_FP32_IR, s))) return rc;
         learned_taken += taken;
         if (o.verbose) std::fprintf(stderr, "alamo_el: learned guess %s, |r|_inf %.6e\n", taken ? "taken" : "rejected", nrm);
     }
@@ -1885,7 +1902,7 @@ int alamo_el_solve(alamo_el_t h, double* x, const double* b, double tol_rel, dou
             if (learned_pre && it > 0) {
                 // learned coarse correction between cycles, judged like the guess
                 int taken;
-                if ((rc = learned_step(h, x, &nrm, &taken, s))) return rc;
+                if ((rc = learned_step(h, x, &nrm, &taken, o.precision == ALAMO_EL_PREC_FP32_IR, s))) return rc;
                 learned_taken += taken;
                 if (nrm <= target) { iters = it; status = 0; break; }
             }
@@ -2005,7 +2022,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         if ((o.learned & ALAMO_EL_LEARNED_PRECOND) && h->nn && !first) {
             // learned coarse correction at each restart, judged like the guess
             int taken;
-            if ((rc = learned_step(h, x, &resnorm, &taken, s))) return rc;
+            if ((rc = learned_step(h, x, &resnorm, &taken, o.precision == ALAMO_EL_PREC_FP32_IR, s))) return rc;
             learned_taken += taken;
             if (resnorm <= target) return 0;
         }

@@ -888,6 +888,70 @@ __global__ void __launch_bounds__(kBlock) k_axpy(T* __restrict__ y, T a, const T
     if (id < n) y[id] += a * x[id];
 }
 
+// The judged step of a learned correction e (w = A e): partials of r . w
+// (slot 0) and w . w (slot 1), one read of r and w.
+__global__ void __launch_bounds__(kBlock) k_dot2_partial(const double* __restrict__ r, const double* __restrict__ w,
+                                                         long n, double* __restrict__ partial) {
+    __shared__ double sh[2][kBlock / 64];
+    double rw = 0.0, ww = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const double wq = w[q];
+        rw += r[q] * wq;
+        ww += wq * wq;
+    }
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        rw += __shfl_xor(rw, off, 64);
+        ww += __shfl_xor(ww, off, 64);
+    }
+    if (lane == 0) {
+        sh[0][wave] = rw;
+        sh[1][wave] = ww;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+#pragma unroll
+        for (int v = 0; v < kBlock / 64; ++v) t += sh[threadIdx.x][v];
+        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// w <- r - g w (the residual after the step x += g e) and partials of its
+// max-norm (NaN propagated, as k_reduce_partial's op 0).
+__global__ void __launch_bounds__(kBlock) k_step_residual(double* __restrict__ w, double g,
+                                                          const double* __restrict__ r, long n,
+                                                          double* __restrict__ partial) {
+    __shared__ double sh[kBlock];
+    double m = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const double v = (-g) * w[q] + 1.0 * r[q];
+        w[q] = v;
+        m = (isnan(v) || isnan(m)) ? NAN : fmax(m, fabs(v));
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const double a = sh[threadIdx.x], b = sh[threadIdx.x + st];
+            sh[threadIdx.x] = (isnan(a) || isnan(b)) ? NAN : fmax(a, b);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// x += g e and r = w, one pass (the step taken).
+__global__ void __launch_bounds__(kBlock) k_take_step(double* __restrict__ x, double g, const double* __restrict__ e,
+                                                      double* __restrict__ r, const double* __restrict__ w, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) {
+        x[id] += g * e[id];
+        r[id] = w[id];
+    }
+}
+
 // y = a * x
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_scale_copy(const T* __restrict__ x, T a, T* __restrict__ y, long n) {

@@ -680,7 +680,10 @@ __global__ void __launch_bounds__(kBlock) k_diagonal(const T* __restrict__ C, co
 // Nodal full-weighting restriction of ncomp components from level Lf to Lc
 // (src/Operator/Operator.cpp:280-353); the same weights average C between MG
 // levels (src/Operator/Elastic.cpp:786-834).
-template <typename T, bool WRAP>
+// NC > 0 fixes the component count at compile time: the loop unrolls and
+// every component's loads are issued before the first one's arithmetic (the
+// coarse levels' launches are set by that latency, not by their bytes).
+template <typename T, bool WRAP, int NC = 0>
 __device__ __forceinline__ void restrict_node_w(const T* __restrict__ fine, const Lvl& Lf, T* __restrict__ crse,
                                                 const Lvl& Lc, int ncomp, long cid) {
     int I, J, K;
@@ -697,7 +700,10 @@ __device__ __forceinline__ void restrict_node_w(const T* __restrict__ fine, cons
     if constexpr (WRAP) nb_offsets(Lf, 2 * I, 2 * J, 2 * K, om, op);
     const long x1m = om[0], x1p = op[0], sym = om[1], syp = op[1], szm = om[2], szp = op[2];
     const long f0 = (2L * K) * sz + (2L * J) * sy + 2L * I;
-    for (int n = 0; n < ncomp; ++n) {
+    const int nc = NC > 0 ? NC : ncomp;
+    T vout[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int n = 0; n < nc; ++n) {
         const T* F = fine + n * Lf.N + f0;
         T v;
         if (bi && bj && bk) v = F[0];
@@ -719,20 +725,27 @@ __device__ __forceinline__ void restrict_node_w(const T* __restrict__ fine, cons
                 (F[sym + szm] + F[sym + szp] + F[syp + szm] + F[syp + szp] + F[x1m + szm] + F[x1p + szm] +
                  F[x1m + szp] + F[x1p + szp] + F[x1m + sym] + F[x1m + syp] + F[x1p + sym] + F[x1p + syp]) / T(32) +
                 (F[x1m] + F[sym] + F[szm] + F[x1p] + F[syp] + F[szp]) / T(16) + F[0] / T(8);
-        crse[n * Lc.N + cid] = v;
+        if constexpr (NC > 0) vout[n] = v;
+        else crse[n * Lc.N + cid] = v;
+    }
+    if constexpr (NC > 0) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) crse[n * Lc.N + cid] = vout[n];
     }
 }
-template <typename T>
+template <typename T, int NC = 0>
 __device__ __forceinline__ void restrict_node(const T* __restrict__ fine, const Lvl& Lf, T* __restrict__ crse,
                                               const Lvl& Lc, int ncomp, long cid) {
-    if (Lc.per) restrict_node_w<T, true>(fine, Lf, crse, Lc, ncomp, cid);
-    else restrict_node_w<T, false>(fine, Lf, crse, Lc, ncomp, cid);
+    if (Lc.per) restrict_node_w<T, true, NC>(fine, Lf, crse, Lc, ncomp, cid);
+    else restrict_node_w<T, false, NC>(fine, Lf, crse, Lc, ncomp, cid);
 }
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_restrict(const T* __restrict__ fine, Lvl Lf, T* __restrict__ crse,
                                                      Lvl Lc, int ncomp) {
     const long cid = xcd_node();
-    if (cid < Lc.N) restrict_node<T>(fine, Lf, crse, Lc, ncomp, cid);
+    if (cid >= Lc.N) return;
+    if (ncomp == 3) restrict_node<T, 3>(fine, Lf, crse, Lc, 3, cid);
+    else restrict_node<T>(fine, Lf, crse, Lc, ncomp, cid);
 }
 
 // Restriction followed by the first Jacobi sweep from zero on the coarse
@@ -744,6 +757,15 @@ __global__ void __launch_bounds__(kBlock) k_restrict_fz(const T* __restrict__ fi
                                                         T* __restrict__ tmp) {
     const long cid = xcd_node();
     if (cid >= Lc.N) return;
+    if (ncomp == 3) {
+        restrict_node<T, 3>(fine, Lf, crse, Lc, 3, cid);
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            const long q = n * Lc.N + cid;
+            tmp[q] = omega * crse[q] / diag[q];
+        }
+        return;
+    }
     restrict_node<T>(fine, Lf, crse, Lc, ncomp, cid);
     for (int n = 0; n < ncomp; ++n) {
         const long q = n * Lc.N + cid;
@@ -793,7 +815,7 @@ __global__ void __launch_bounds__(kBlock) k_inject_all(InjectLevels<T> lv, int n
 
 // Trilinear prolongation, fine += P crse (src/Operator/Operator.cpp:399-428).
 // With zero_first the fine field is overwritten instead (cor = P cor_crse).
-template <typename T, typename F>
+template <typename T, int NC, typename F>
 __device__ __forceinline__ void interp_node_of(F crse_at, const Lvl& Lc, T* __restrict__ fine, const Lvl& Lf,
                                                int ncomp, int zero_first, long id) {
     int i, j, k;
@@ -802,7 +824,10 @@ __device__ __forceinline__ void interp_node_of(F crse_at, const Lvl& Lc, T* __re
     const bool oi = i & 1, oj = j & 1, ok = k & 1;
     const long sy = Lc.nx, sz = (long)Lc.nx * Lc.ny;
     const long c0 = (long)K * sz + (long)J * sy + I;
-    for (int n = 0; n < ncomp; ++n) {
+    const int nc = NC > 0 ? NC : ncomp;
+    T vout[NC > 0 ? NC : 1], vold[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int n = 0; n < nc; ++n) {
         const long q = n * Lc.N + c0;
         auto Cc = [&](long o) { return crse_at(q + o); };
         T v;
@@ -816,13 +841,24 @@ __device__ __forceinline__ void interp_node_of(F crse_at, const Lvl& Lc, T* __re
         else
             v = T(0.125) * (Cc(0) + Cc(1) + Cc(sy) + Cc(sz) + Cc(sy + sz) + Cc(1 + sz) + Cc(1 + sy) + Cc(1 + sy + sz));
         T* out = fine + n * Lf.N + id;
-        *out = zero_first ? v : *out + v;
+        if constexpr (NC > 0) {
+            vout[n] = v;
+            vold[n] = zero_first ? T(0) : *out;
+        } else {
+            *out = zero_first ? v : *out + v;
+        }
+    }
+    if constexpr (NC > 0) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) fine[n * Lf.N + id] = zero_first ? vout[n] : vold[n] + vout[n];
     }
 }
 template <typename T>
 __device__ __forceinline__ void interp_node(const T* __restrict__ crse, const Lvl& Lc, T* __restrict__ fine,
                                             const Lvl& Lf, int ncomp, int zero_first, long id) {
-    interp_node_of<T>([crse](long q) { return crse[q]; }, Lc, fine, Lf, ncomp, zero_first, id);
+    auto at = [crse](long q) { return crse[q]; };
+    if (ncomp == 3) interp_node_of<T, 3>(at, Lc, fine, Lf, 3, zero_first, id);
+    else interp_node_of<T, 0>(at, Lc, fine, Lf, ncomp, zero_first, id);
 }
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_interp(const T* __restrict__ crse, Lvl Lc, T* __restrict__ fine, Lvl Lf,
@@ -837,9 +873,10 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_interp_sum(const T* __restrict__ crse, const T* __restrict__ held, Lvl Lc,
                                                        T* __restrict__ fine, Lvl Lf, int ncomp, int zero_first) {
     const long id = xcd_node();
-    if (id < Lf.N)
-        interp_node_of<T>([crse, held](long q) { return crse[q] + T(1) * held[q]; }, Lc, fine, Lf, ncomp, zero_first,
-                          id);
+    if (id >= Lf.N) return;
+    auto at = [crse, held](long q) { return crse[q] + T(1) * held[q]; };
+    if (ncomp == 3) interp_node_of<T, 3>(at, Lc, fine, Lf, 3, zero_first, id);
+    else interp_node_of<T, 0>(at, Lc, fine, Lf, ncomp, zero_first, id);
 }
 
 template <typename T>

@@ -52,37 +52,19 @@ __device__ __forceinline__ bool tap_node(const G3& g, int x, int y, int z, int t
     return xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
 }
 
-// Y = act(conv3(X, W) + bias).  Workgroup: 4 waves x 16 nodes; blockIdx.y
-// selects 16*NB output channels.  Wt[n][27 * Cin] (bf16).  bias may be null.
-// Yf != null: write float32 component-major Yf[n * N + v] for n < yf_ch
-// instead of the bf16 Y (the network output).
+// The K rounds of one wave's 16 nodes x 16*NB output channels, starting at
+// k = kfirst and stepping by kstep (a multiple of 32 * KU): KU steps of 32
+// along K per round, every A and B load of the round issued before its MFMAs
+// (one wait per round instead of one per step; the small levels have few
+// waves to hide it), the MFMAs in step order.
+constexpr int kConvKU = 4;
 template <int NB>
-__global__ void __launch_bounds__(256) k_conv_fwd(const bf16_t* __restrict__ X, int Cin, const bf16_t* __restrict__ Wt,
-                                                  const float* __restrict__ bias, int Cout, G3 g, int relu,
-                                                  bf16_t* __restrict__ Y, float* __restrict__ Yf, int yf_ch) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int v0 = (blockIdx.x * 4 + wave) * 16;
-    if (v0 >= g.N) return;
-    const int nbase = blockIdx.y * 16 * NB;
-    const int row = lane & 15, grp = lane >> 4;
-    const int vox = v0 + row;
-    const bool vin = vox < g.N;
-    int x = 0, y = 0, z = 0;
-    if (vin) {
-        x = vox % g.nx;
-        const int t = vox / g.nx;
-        y = t % g.ny;
-        z = t / g.ny;
-    }
+__device__ __forceinline__ void conv_rounds(const bf16_t* __restrict__ X, int Cin, const bf16_t* __restrict__ Wt,
+                                            const G3& g, bool vin, int x, int y, int z, int row, int grp, int nbase,
+                                            int kfirst, int kstep, f32x4 acc[NB]) {
+    constexpr int KU = kConvKU;
     const int K = 27 * Cin;
-    f32x4 acc[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // KU steps of 32 along K per round: every A and B load of the round is
-    // issued before its MFMAs (one wait per round instead of one per step;
-    // the small levels have few waves to hide it), the MFMAs in step order.
-    constexpr int KU = 4;
-    for (int k0 = 0; k0 < K; k0 += 32 * KU) {
+    for (int k0 = kfirst; k0 < K; k0 += kstep) {
         uint4 a[KU], b[KU][NB];
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
@@ -110,9 +92,19 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const bf16_t* __restrict__ X, 
                                                                  0, 0);
         }
     }
+}
+
+// The epilogue of a 16-node x 16*NB-channel tile: bias, ReLU, and the bf16
+// channels-last store, or the float32 component-major one for the output.
+template <int NB>
+__device__ __forceinline__ void conv_store(const f32x4 acc[NB], const float* __restrict__ bias, int Cout,
+                                           const G3& g, int relu, int v0, int row, int grp, int nbase,
+                                           bf16_t* __restrict__ Y, float* __restrict__ Yf, int yf_ch) {
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
         const int n = nbase + nb * 16 + row;
+        // the output layer's bias holds yf_ch entries, its Cout is padded to 16
+        if (Yf && n >= yf_ch) continue;
         const float bv = bias ? bias[n] : 0.f;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -121,12 +113,82 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const bf16_t* __restrict__ X, 
             float val = acc[nb][e] + bv;
             if (relu) val = fmaxf(val, 0.f);
             if (Yf) {
-                if (n < yf_ch) Yf[(size_t)n * g.N + v] = val;
+                Yf[(size_t)n * g.N + v] = val;
             } else {
                 Y[(size_t)v * Cout + n] = f2bf(val);
             }
         }
     }
+}
+
+__device__ __forceinline__ void conv_node(const G3& g, int vox, bool vin, int& x, int& y, int& z) {
+    x = y = z = 0;
+    if (vin) {
+        x = vox % g.nx;
+        const int t = vox / g.nx;
+        y = t % g.ny;
+        z = t / g.ny;
+    }
+}
+
+// Y = act(conv3(X, W) + bias).  Workgroup: 4 waves x 16 nodes; blockIdx.y
+// selects 16*NB output channels.  Wt[n][27 * Cin] (bf16).  bias may be null.
+// Yf != null: write float32 component-major Yf[n * N + v] for n < yf_ch
+// instead of the bf16 Y (the network output).
+template <int NB>
+__global__ void __launch_bounds__(256) k_conv_fwd(const bf16_t* __restrict__ X, int Cin, const bf16_t* __restrict__ Wt,
+                                                  const float* __restrict__ bias, int Cout, G3 g, int relu,
+                                                  bf16_t* __restrict__ Y, float* __restrict__ Yf, int yf_ch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int v0 = (blockIdx.x * 4 + wave) * 16;
+    if (v0 >= g.N) return;
+    const int nbase = blockIdx.y * 16 * NB;
+    const int row = lane & 15, grp = lane >> 4;
+    const int vox = v0 + row;
+    const bool vin = vox < g.N;
+    int x, y, z;
+    conv_node(g, vox, vin, x, y, z);
+    f32x4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    conv_rounds<NB>(X, Cin, Wt, g, vin, x, y, z, row, grp, nbase, 0, 32 * kConvKU, acc);
+    conv_store<NB>(acc, bias, Cout, g, relu, v0, row, grp, nbase, Y, Yf, yf_ch);
+}
+
+// k_conv_fwd with K split over the KS waves of a workgroup, for the levels
+// with few nodes (where k_conv_fwd runs a few dozen waves, each through all
+// of K = 27 Cin one round after another).  Every wave of the workgroup takes
+// the same 16 nodes and 16*NB channels and every KS-th round of K; wave 0
+// adds the others' float32 tiles in wave order through LDS and stores.
+template <int NB, int KS>
+__global__ void __launch_bounds__(64 * KS) k_conv_fwd_ks(const bf16_t* __restrict__ X, int Cin,
+                                                         const bf16_t* __restrict__ Wt, const float* __restrict__ bias,
+                                                         int Cout, G3 g, int relu, bf16_t* __restrict__ Y,
+                                                         float* __restrict__ Yf, int yf_ch) {
+    __shared__ f32x4 part[KS - 1][NB][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int v0 = blockIdx.x * 16;
+    const int nbase = blockIdx.y * 16 * NB;
+    const int row = lane & 15, grp = lane >> 4;
+    const int vox = v0 + row;
+    const bool vin = vox < g.N;
+    int x, y, z;
+    conv_node(g, vox, vin, x, y, z);
+    f32x4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    conv_rounds<NB>(X, Cin, Wt, g, vin, x, y, z, row, grp, nbase, wave * 32 * kConvKU, 32 * kConvKU * KS, acc);
+    if (wave > 0) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) part[wave - 1][nb][lane] = acc[nb];
+    }
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int w = 0; w < KS - 1; ++w)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[nb] += part[w][nb][lane];
+    conv_store<NB>(acc, bias, Cout, g, relu, v0, row, grp, nbase, Y, Yf, yf_ch);
 }
 
 // Partial weight gradient over one chunk of nodes:

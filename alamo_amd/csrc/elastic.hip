@@ -55,6 +55,18 @@ inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 // grid of the XCD-remapped stencil kernels: a multiple of 8 blocks
 inline unsigned nblk8(long n) { return (unsigned)((nblk(n) + ael::kXcd - 1) / ael::kXcd * ael::kXcd); }
 
+// Trilinear prolongation of the three displacement components, fine = P crse
+// (zero_first) or fine += P crse; with held, of crse + held.  One thread per
+// coarse node.
+template <typename T>
+void launch_interp3(const T* crse, const T* held, const ael::Lvl& Lc, T* fine, const ael::Lvl& Lf, int zero_first,
+                    hipStream_t s) {
+    if (held)
+        ael::k_interp_cell<T, true><<<nblk8(Lc.N), kBlock, 0, s>>>(crse, held, Lc, fine, Lf, zero_first);
+    else
+        ael::k_interp_cell<T, false><<<nblk8(Lc.N), kBlock, 0, s>>>(crse, nullptr, Lc, fine, Lf, zero_first);
+}
+
 int ncomp_of(int sym) { return sym == 0 ? 2 : (sym == 1 ? 21 : (sym == 2 ? 9 : -1)); }
 
 template <typename T>
@@ -464,7 +476,7 @@ struct Cycle {
                                                                       (T)h->omega, c.tmp);
     }
     void interp(int lf, T* fine, const T* crse, int zero_first) {
-        ael::k_interp<T><<<nblk8(L(lf).L.N), kBlock, 0, s>>>(crse, L(lf + 1).L, fine, L(lf).L, 3, zero_first);
+        launch_interp3<T>(crse, nullptr, L(lf + 1).L, fine, L(lf).L, zero_first, s);
     }
     // Bottom level of the cycles: the direct level when its inverse is used,
     // else the coarsest level.
@@ -567,8 +579,7 @@ struct Cycle {
             // the level below ran a V-cycle on top of its held correction:
             // the interpolation reads their sum (the sum is not needed after)
             if (l + 1 < bot)
-                ael::k_interp_sum<T><<<nblk8(L(l).L.N), kBlock, 0, s>>>(V(l + 1).cor, V(l + 1).cor_hold, L(l + 1).L,
-                                                                        v.cor, L(l).L, 3, 1);
+                launch_interp3<T>(V(l + 1).cor, V(l + 1).cor_hold, L(l + 1).L, v.cor, L(l).L, 1, s);
             else
                 interp(l, v.cor, V(l + 1).cor, 1);
             corr_res(l, v.rescor, v.cor, v.res);
@@ -1183,8 +1194,7 @@ int learned_apply(alamo_el_t h, const double* b, double* x, hipStream_t s) {
     ael::k_scale_output<<<nblk(3 * NL), kBlock, 0, s>>>(h->nn_out, 3 * NL, sc, L == 0 ? x : lL.d.cor);
     for (int l = L - 1; l >= 0; --l) {
         double* dst = l == 0 ? x : h->lv[l].d.cor;
-        ael::k_interp<double><<<nblk8(h->lv[l].L.N), kBlock, 0, s>>>(h->lv[l + 1].d.cor, h->lv[l + 1].L, dst,
-                                                                      h->lv[l].L, 3, 1);
+        launch_interp3<double>(h->lv[l + 1].d.cor, nullptr, h->lv[l + 1].L, dst, h->lv[l].L, 1, s);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -1601,8 +1611,7 @@ int alamo_el_restrict(alamo_el_t h, int cmglev, double* crse, const double* fine
 int alamo_el_interp_add(alamo_el_t h, int fmglev, double* fine, const double* crse, hipStream_t s) {
     if (bad_level(h, fmglev) || fmglev + 1 >= (int)h->lv.size() || !crse || !fine)
         return fail(ALAMO_EL_EINVAL, "bad argument");
-    ael::k_interp<double><<<nblk8(h->lv[fmglev].L.N), kBlock, 0, s>>>(crse, h->lv[fmglev + 1].L, fine,
-                                                                      h->lv[fmglev].L, 3, 0);
+    launch_interp3<double>(crse, nullptr, h->lv[fmglev + 1].L, fine, h->lv[fmglev].L, 0, s);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1625,7 +1634,7 @@ int alamo_el_level_op32(alamo_el_t h, int mglev, int op, const float* x, const f
             ael::k_restrict<float><<<nblk8(h->lv[mglev + 1].L.N), kBlock, 0, s>>>(x, l.L, out, h->lv[mglev + 1].L, 3);
             break;
         default:
-            ael::k_interp<float><<<nblk8(l.L.N), kBlock, 0, s>>>(x, h->lv[mglev + 1].L, out, l.L, 3, 0);
+            launch_interp3<float>(x, nullptr, h->lv[mglev + 1].L, out, l.L, 0, s);
             break;
     }
     HIPCHK(hipGetLastError());

@@ -813,70 +813,63 @@ __global__ void __launch_bounds__(kBlock) k_inject_all(InjectLevels<T> lv, int n
     for (int n = 0; n < ncomp; ++n) lv.f[l][n * Lc.N + cid] = lv.f[0][n * L0.N + f0];
 }
 
-// Trilinear prolongation, fine += P crse (src/Operator/Operator.cpp:399-428).
-// With zero_first the fine field is overwritten instead (cor = P cor_crse).
-template <typename T, int NC, typename F>
-__device__ __forceinline__ void interp_node_of(F crse_at, const Lvl& Lc, T* __restrict__ fine, const Lvl& Lf,
-                                               int ncomp, int zero_first, long id) {
-    int i, j, k;
-    node_ijk(id, Lf, i, j, k);
-    const int I = i >> 1, J = j >> 1, K = k >> 1;
-    const bool oi = i & 1, oj = j & 1, ok = k & 1;
-    const long sy = Lc.nx, sz = (long)Lc.nx * Lc.ny;
-    const long c0 = (long)K * sz + (long)J * sy + I;
-    const int nc = NC > 0 ? NC : ncomp;
-    T vout[NC > 0 ? NC : 1], vold[NC > 0 ? NC : 1];
+// Trilinear prolongation of the three components, fine += P crse, or
+// fine = P crse with zero_first (src/Operator/Operator.cpp:399-428), by
+// coarse node: thread (I, J, K) loads the 8 coarse values of its cell once
+// and writes the fine nodes (2I + a, 2J + b, 2K + c), a, b, c in {0, 1}, that
+// exist.  A fine node at even (i, j, k) takes its coarse node, one with one
+// odd index the mean of 2 coarse nodes, two odd the mean of 4, three odd the
+// mean of 8.  No branch on the fine node's parity, and one load round trip
+// for 8 fine nodes (one thread per fine node ran both parities' branches in
+// every wave, each with its own loads).  With held the coarse value is
+// crse + held: the interpolation of a V-cycle's correction on top of the
+// held one, without a launch forming their sum.
+template <typename T, bool SUM>
+__global__ void __launch_bounds__(kBlock) k_interp_cell(const T* __restrict__ crse, const T* __restrict__ held, Lvl Lc,
+                                                        T* __restrict__ fine, Lvl Lf, int zero_first) {
+    const long cid = xcd_node();
+    if (cid >= Lc.N) return;
+    int I, J, K;
+    node_ijk(cid, Lc, I, J, K);
+    const bool hx = I + 1 < Lc.nx, hy = J + 1 < Lc.ny, hz = K + 1 < Lc.nz;
+    // offsets of the cell's corners; 0 where the corner does not exist (its
+    // value is then read but never used)
+    const long ox = hx ? 1 : 0, oy = hy ? (long)Lc.nx : 0, oz = hz ? (long)Lc.nx * Lc.ny : 0;
+    const long fsy = Lf.nx, fsz = (long)Lf.nx * Lf.ny;
+    const long f0 = (2L * K) * fsz + (2L * J) * fsy + 2L * I;
+    T v[3][8], old[3][8];
 #pragma unroll
-    for (int n = 0; n < nc; ++n) {
-        const long q = n * Lc.N + c0;
-        auto Cc = [&](long o) { return crse_at(q + o); };
-        T v;
-        if (!oi && !oj && !ok) v = Cc(0);
-        else if (!oj && !ok) v = T(0.5) * (Cc(0) + Cc(1));
-        else if (!ok && !oi) v = T(0.5) * (Cc(0) + Cc(sy));
-        else if (!oi && !oj) v = T(0.5) * (Cc(0) + Cc(sz));
-        else if (!oi) v = T(0.25) * (Cc(0) + Cc(sy) + Cc(sz) + Cc(sy + sz));
-        else if (!oj) v = T(0.25) * (Cc(0) + Cc(sz) + Cc(1) + Cc(1 + sz));
-        else if (!ok) v = T(0.25) * (Cc(0) + Cc(1) + Cc(sy) + Cc(1 + sy));
-        else
-            v = T(0.125) * (Cc(0) + Cc(1) + Cc(sy) + Cc(sz) + Cc(sy + sz) + Cc(1 + sz) + Cc(1 + sy) + Cc(1 + sy + sz));
-        T* out = fine + n * Lf.N + id;
-        if constexpr (NC > 0) {
-            vout[n] = v;
-            vold[n] = zero_first ? T(0) : *out;
-        } else {
-            *out = zero_first ? v : *out + v;
+    for (int n = 0; n < 3; ++n) {
+        const long q = n * Lc.N + cid;
+        auto at = [&](long o) { return SUM ? crse[q + o] + T(1) * held[q + o] : crse[q + o]; };
+        const T c000 = at(0), c100 = at(ox), c010 = at(oy), c001 = at(oz);
+        const T c011 = at(oy + oz), c101 = at(ox + oz), c110 = at(ox + oy), c111 = at(ox + oy + oz);
+        v[n][0] = c000;
+        v[n][1] = T(0.5) * (c000 + c100);
+        v[n][2] = T(0.5) * (c000 + c010);
+        v[n][3] = T(0.25) * (c000 + c100 + c010 + c110);
+        v[n][4] = T(0.5) * (c000 + c001);
+        v[n][5] = T(0.25) * (c000 + c001 + c100 + c101);
+        v[n][6] = T(0.25) * (c000 + c010 + c001 + c011);
+        v[n][7] = T(0.125) * (c000 + c100 + c010 + c001 + c011 + c101 + c110 + c111);
+        if (!zero_first) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool a = e & 1, b = e & 2, c = e & 4;
+                const bool ex = (!a || hx) && (!b || hy) && (!c || hz);
+                old[n][e] = ex ? fine[n * Lf.N + f0 + (a ? 1 : 0) + (b ? fsy : 0) + (c ? fsz : 0)] : T(0);
+            }
         }
     }
-    if constexpr (NC > 0) {
 #pragma unroll
-        for (int n = 0; n < NC; ++n) fine[n * Lf.N + id] = zero_first ? vout[n] : vold[n] + vout[n];
-    }
-}
-template <typename T>
-__device__ __forceinline__ void interp_node(const T* __restrict__ crse, const Lvl& Lc, T* __restrict__ fine,
-                                            const Lvl& Lf, int ncomp, int zero_first, long id) {
-    auto at = [crse](long q) { return crse[q]; };
-    if (ncomp == 3) interp_node_of<T, 3>(at, Lc, fine, Lf, 3, zero_first, id);
-    else interp_node_of<T, 0>(at, Lc, fine, Lf, ncomp, zero_first, id);
-}
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_interp(const T* __restrict__ crse, Lvl Lc, T* __restrict__ fine, Lvl Lf,
-                                                   int ncomp, int zero_first) {
-    const long id = xcd_node();
-    if (id < Lf.N) interp_node<T>(crse, Lc, fine, Lf, ncomp, zero_first, id);
-}
-
-// k_interp of the coarse sum crse + held, without forming it: the value
-// k_axpy(crse, 1, held) would store, rounded the same way, at every read.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_interp_sum(const T* __restrict__ crse, const T* __restrict__ held, Lvl Lc,
-                                                       T* __restrict__ fine, Lvl Lf, int ncomp, int zero_first) {
-    const long id = xcd_node();
-    if (id >= Lf.N) return;
-    auto at = [crse, held](long q) { return crse[q] + T(1) * held[q]; };
-    if (ncomp == 3) interp_node_of<T, 3>(at, Lc, fine, Lf, 3, zero_first, id);
-    else interp_node_of<T, 0>(at, Lc, fine, Lf, ncomp, zero_first, id);
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const bool a = e & 1, b = e & 2, c = e & 4;
+            if ((a && !hx) || (b && !hy) || (c && !hz)) continue;
+            T* out = fine + n * Lf.N + f0 + (a ? 1 : 0) + (b ? fsy : 0) + (c ? fsz : 0);
+            *out = zero_first ? v[n][e] : old[n][e] + v[n][e];
+        }
 }
 
 template <typename T>

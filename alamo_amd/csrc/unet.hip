@@ -97,11 +97,9 @@ void launch_conv(const bf16_t* X, int Cin, const bf16_t* W, const float* bias, i
     const dim3 grid((unsigned)((g.N + 63) / 64), 1);
     // Levels whose one-wave-per-16-nodes grid leaves most of the 256 CUs
     // idle split K over 4 waves a workgroup, 16 channels a workgroup
-    // (ALAMO_NN_KSPLIT=0 never, 2 on every level).
-    static const int ksplit = [] {
-        const char* e = std::getenv("ALAMO_NN_KSPLIT");
-        return e ? std::atoi(e) : 1;
-    }();
+    // (ALAMO_NN_KSPLIT=0 never, 2 on every level; read at every launch).
+    const char* ks_env = std::getenv("ALAMO_NN_KSPLIT");
+    const int ksplit = ks_env ? std::atoi(ks_env) : 1;
     const long plain_wg = (long)grid.x * (Cout % 64 == 0 ? Cout / 64 : Cout % 32 == 0 ? Cout / 32 : Cout / 16);
     if (ksplit == 2 || (ksplit == 1 && plain_wg < 256)) {
         aunet::k_conv_fwd_ks<1, 4><<<dim3((unsigned)((g.N + 15) / 16), Cout / 16), 256, 0, s>>>(X, Cin, W, bias, Cout, g,

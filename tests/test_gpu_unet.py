@@ -42,19 +42,31 @@ def from_dev(t):
     return np.moveaxis(t.cpu().numpy().astype(np.float64), 0, -1)
 
 
+# ALAMO_NN_KSPLIT: "1" (the default) splits K over the waves of a workgroup
+# on the levels with few nodes, "0" never (every level runs the plain
+# kernel's 16-, 32- and 64-channel forms), "2" on every level.
+KSPLIT = ["0", "1", "2"]
+
+
+@pytest.mark.parametrize("ksplit", KSPLIT)
 @pytest.mark.parametrize("nodes,n_in", [((9, 9, 9), 3), ((17, 9, 9), 27), ((9, 17, 25), 8)])
-def test_forward_matches_oracle(nodes, n_in):
+def test_forward_matches_oracle(nodes, n_in, ksplit, monkeypatch):
+    monkeypatch.setenv("ALAMO_NN_KSPLIT", ksplit)
     net, p, x = make(nodes, n_in, 3, seed=3)
     y = from_dev(net.forward(to_dev(x)))
     torch.cuda.synchronize()
     yo = U.forward(p, U.round_bf16(x), 3, bf16=True)
     e = rel(y, yo)
-    print(f"forward {nodes} n_in={n_in}: rel L2 {e:.2e}")
+    print(f"forward {nodes} n_in={n_in} ksplit={ksplit}: rel L2 {e:.2e}")
     assert e < 1e-2
 
 
+@pytest.mark.parametrize("ksplit", KSPLIT)
 @pytest.mark.parametrize("nodes,n_in", [((9, 9, 9), 3), ((17, 9, 9), 27)])
-def test_backward_matches_oracle(nodes, n_in):
+def test_backward_matches_oracle(nodes, n_in, ksplit, monkeypatch):
+    """The input gradients of the convolutions run the forward kernels on
+    the transposed weights, so the K split reaches them too."""
+    monkeypatch.setenv("ALAMO_NN_KSPLIT", ksplit)
     net, p, x = make(nodes, n_in, 3, seed=4)
     rng = np.random.default_rng(5)
     dy = rng.standard_normal((nodes[2], nodes[1], nodes[0], 3))

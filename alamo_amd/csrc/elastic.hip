@@ -124,7 +124,8 @@ struct alamo_el_s {
     double* red_host = nullptr;  // pinned
     bool psi_set = false;        // SetPsi called: every operator row carries the psi weight
     double psi_small = 1e-8;
-    double* red_host_vec = nullptr;  // pinned, FGMRES coefficients
+    double* red_host_vec = nullptr;  // pinned, FGMRES coefficients (two steps' copies)
+    hipEvent_t kev[2] = {nullptr, nullptr};  // the copies' completion, by step parity
     double* bottom_work = nullptr;
     int* bottom_status = nullptr;
     double* gu9 = nullptr;  // Newton scratch, allocated on demand
@@ -1376,6 +1377,8 @@ int alamo_el_destroy(alamo_el_t h) {
         if (p) (void)hipFree(p);
     if (h->red_host) (void)hipHostFree(h->red_host);
     if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
+    for (hipEvent_t e : h->kev)
+        if (e) (void)hipEventDestroy(e);
     delete h;
     return 0;
 }
@@ -1978,9 +1981,11 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         ALLOC(h->kh, 4 * (m + 1) + 2);
         ALLOC(h->kpart, 2L * (m + 1) * kDotBlocks);
         if (h->red_host_vec) (void)hipHostFree(h->red_host_vec);
-        HIPCHK(hipHostMalloc((void**)&h->red_host_vec, sizeof(double) * (2 * (m + 1) + 2)));
+        HIPCHK(hipHostMalloc((void**)&h->red_host_vec, sizeof(double) * 2 * (2 * (m + 1) + 2)));
         h->krylov_m = m;
     }
+    for (hipEvent_t& e : h->kev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const bool fp32 = o.precision == ALAMO_EL_PREC_FP32_IR;
     Cycle<double> c64{h, s, o.nu1, o.nu2, o.final_smooth, o.bottom_max_iter, o.bottom_tol_rel, o.bottom_solver,
                       o.smoother};
@@ -2045,21 +2050,28 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         g[0] = beta;
         int j = 0;
         double hn_prev = 0.0;
-        for (; j < m && iters < limit; ++j) {
-            double* Vj = h->kv + (long)j * MP;
-            double* Zj = h->kz + (long)j * MP;
+        double* nrm_slot = h->kh + 2 * (m + 1);
+        // The device part of step jj (iteration it): V_jj and its copy in the
+        // cycle's precision, Z_jj = cycle(V_jj), w = A Z_jj.
+        auto launch_step = [&](int jj, int it) {
+            double* Vj = h->kv + (long)jj * MP;
+            double* Zj = h->kz + (long)jj * MP;
             // V_j = r / beta (j = 0) or w / |w| of the last step, written with
             // its copy in the cycle's precision as the cycle's right-hand side
-            const double* src = j == 0 ? l0.d.res : h->kw;
-            const double scale = j == 0 ? 1.0 / beta : (hn_prev > 0 ? 1.0 / hn_prev : 0.0);
+            const double* src = jj == 0 ? l0.d.res : h->kw;
+            const double scale = jj == 0 ? 1.0 / beta : (hn_prev > 0 ? 1.0 / hn_prev : 0.0);
+            // with the delayed passes the scale is taken from |w|^2 on the
+            // device, so that the step can be launched ahead of the host
+            const double* nrm2 = lag && jj > 0 ? nrm_slot : nullptr;
             // Z_j = cycle(V_j); an F-cycle's last correction sum is formed by
             // the copy into Z_j
-            const int cyc = cycle_of(o, iters);
+            const int cyc = cycle_of(o, it);
             const bool held = cyc == ALAMO_EL_CYCLE_F;
             // a V-cycle's first sweep from zero on level 0 comes with the copy
             const bool fz = cyc == ALAMO_EL_CYCLE_V && c32.fz_ready() && c32.bot() > 0;
             if (fp32 && !z32) {  // Z_j widened to FP64 (ALAMO_EL_Z=fp64)
-                ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0);
+                ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0, nullptr, 0.f,
+                                                                             nullptr, nrm2);
                 c32.run(cyc);
                 if (held) ael::k_cast_sum<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, l0.f.cor_hold, Zj, M0);
                 else ael::k_cast<float, double><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, Zj, M0);
@@ -2068,12 +2080,13 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                 // Z_j is kept in FP32: the FP32 cycle's correction, whose FP64
                 // widening is exact, so every product with it is the one with
                 // its FP64 copy
-                float* Zf = reinterpret_cast<float*>(h->kz) + (long)j * MPf;
+                float* Zf = reinterpret_cast<float*>(h->kz) + (long)jj * MPf;
                 if (fz)
                     ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0, l0.f.diag,
-                                                                             (float)h->omega, l0.f.tmp);
+                                                                             (float)h->omega, l0.f.tmp, nrm2);
                 else
-                    ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0);
+                    ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0, nullptr, 0.f,
+                                                                             nullptr, nrm2);
                 if (held) {
                     c32.run(cyc, fz);
                     ael::k_cast_sum<float, float><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, l0.f.cor_hold, Zf, M0);
@@ -2094,13 +2107,55 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             } else {
                 if (fz)
                     ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0,
-                                                                              l0.d.diag, h->omega, l0.d.tmp);
+                                                                              l0.d.diag, h->omega, l0.d.tmp, nrm2);
                 else
-                    ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0);
+                    ael::k_scale_copy_cast<double><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.d.res, M0, nullptr, 0.0,
+                                                                              nullptr, nrm2);
                 c64.run(cyc, fz);
                 if (held) ael::k_cast_sum<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, l0.d.cor_hold, Zj, M0);
                 else ael::k_cast<double, double><<<nblk(M0), kBlock, 0, s>>>(l0.d.cor, Zj, M0);
                 launch_fapply<double>(h, l0, Zj, l0.d.C, h->kw, s);  // w = A Z_j
+            }
+        };
+        // DCGS2's two passes over V for step jj, then |w|^2, and the copy of
+        // the coefficients into the host buffer of jj's parity; the event of
+        // that parity marks the copy done.
+        auto launch_dcgs2 = [&](int jj) -> int {
+            const int n = jj + 1;
+            if (jj == 0) {
+                ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, 1, M0, MP, h->kw, h->kpart);
+                ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h->kh);
+                ael::k_multiaxpy_dot<false><<<kDotBlocks, kBlock, 0, s>>>(h->kw, h->kv, 1, M0, MP, h->kh, h->kpart);
+            } else {
+                double* sums = h->kh + 2 * (m + 1) + 2;  // the 2n sums of pass 1
+                ael::k_multidot2_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, h->kpart);
+                ael::k_multidot_final<<<2 * n, kBlock, 0, s>>>(h->kpart, kDotBlocks, sums);
+                ael::k_dcgs2_update<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, sums, h->kh, h->kpart);
+            }
+            ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, nrm_slot);
+            const long nv = 2 * (m + 1) + 2;
+            HIPCHK(hipMemcpyAsync(h->red_host_vec + (jj & 1) * nv, h->kh, sizeof(double) * nv, hipMemcpyDeviceToHost,
+                                  s));
+            HIPCHK(hipEventRecord(h->kev[jj & 1], s));
+            return 0;
+        };
+        // With the delayed passes the next step is launched before the host
+        // reads this one's coefficients (it needs none of them), so the GPU
+        // does not wait for the host between steps; it is not when this step
+        // may end the cycle (the estimate within kAheadMargin of the target,
+        // the cycle's last column or the iteration limit): a step launched
+        // ahead and then not needed costs a whole step.  ALAMO_EL_AHEAD=0
+        // turns it off.
+        const char* ahead_env = std::getenv("ALAMO_EL_AHEAD");
+        const bool ahead = lag && !(ahead_env && std::strcmp(ahead_env, "0") == 0);
+        constexpr double kAheadMargin = 32.0;
+        double est_prev = beta;
+        int launched = -1;  // the last step launched
+        for (; j < m && iters < limit; ++j) {
+            if (launched < j) {
+                launch_step(j, iters);
+                if (lag && (rc = launch_dcgs2(j))) return rc;
+                launched = j;
             }
             // classical Gram-Schmidt, twice ("twice is enough"): h = V^T w;
             // w -= V h; h' = V^T w; w -= V h'.  Up to kMultiDotMax basis
@@ -2110,27 +2165,18 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
             // come back in one copy.  Beyond that: the DGKS criterion decides
             // the second pass, with separate dot, projection and norm passes.
             const int nb = (int)std::min<long>(kRedBlocks, std::max<long>(1, nblk(M0)));
-            double* nrm_slot = h->kh + 2 * (m + 1);
             if (lag) {
                 // DCGS2: w's first projection and V_j's second in two passes
-                // over V; V_0 = r / beta needs no second projection
+                // over V (launch_dcgs2); V_0 = r / beta needs no second one
                 const int n = j + 1;
-                if (j == 0) {
-                    ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, 1, M0, MP, h->kw, h->kpart);
-                    ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h->kh);
-                    ael::k_multiaxpy_dot<false><<<kDotBlocks, kBlock, 0, s>>>(h->kw, h->kv, 1, M0, MP, h->kh, h->kpart);
-                } else {
-                    double* sums = h->kh + 2 * (m + 1) + 2;  // the 2n sums of pass 1
-                    ael::k_multidot2_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, h->kpart);
-                    ael::k_multidot_final<<<2 * n, kBlock, 0, s>>>(h->kpart, kDotBlocks, sums);
-                    ael::k_dcgs2_update<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, sums, h->kh,
-                                                                      h->kpart);
+                if (ahead && j + 1 < m && iters + 1 < limit &&
+                    est_prev / beta * inf_at_restart > kAheadMargin * target) {
+                    launch_step(j + 1, iters + 1);
+                    if ((rc = launch_dcgs2(j + 1))) return rc;
+                    launched = j + 1;
                 }
-                ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, nrm_slot);
-                HIPCHK(hipMemcpyAsync(h->red_host_vec, h->kh, sizeof(double) * (2 * (m + 1) + 2),
-                                      hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                const double* hv = h->red_host_vec;
+                HIPCHK(hipEventSynchronize(h->kev[j & 1]));
+                const double* hv = h->red_host_vec + (j & 1) * (2 * (m + 1) + 2);
                 for (int i = 0; i <= m; ++i) H[i * m + j] = 0.0;
                 if (j == 0) {
                     H[0] = hv[0];
@@ -2149,6 +2195,7 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                 hn_prev = hn;  // V_{j+1} = w / hn is written by the next step
                 ++iters;
                 const double est = estimate(j);
+                est_prev = est;
                 if (o.verbose) std::fprintf(stderr, "alamo_el: fgmres %d |r|_2 est = %.6e\n", iters, est);
                 if (est / beta * inf_at_restart <= target || hn == 0.0 || j + 1 == m || iters >= limit) {
                     finalize(j);  // as it stands: its second projection is not needed

@@ -998,11 +998,19 @@ __global__ void __launch_bounds__(kBlock) k_cast(const Tin* __restrict__ a, Tout
 // y = a x (FP64) and yc = y in the cycle's precision, one pass (k_scale_copy
 // then k_cast); yc may be x itself.  With fz, also the first Jacobi sweep
 // from zero on yc, fz = w yc / D (k_jacobi_from_zero's arithmetic).
+// With nrm2, a is 1 / sqrt(*nrm2) (0 when *nrm2 is 0), read on the device:
+// the scale of the next Krylov vector, w / |w|, can then be launched before
+// the host has read |w|^2.
 template <typename Tc>
 __global__ void __launch_bounds__(kBlock) k_scale_copy_cast(const double* x, double a, double* __restrict__ y, Tc* yc,
                                                             long n, const Tc* __restrict__ diag = nullptr,
-                                                            Tc omega = Tc(0), Tc* __restrict__ fz = nullptr) {
+                                                            Tc omega = Tc(0), Tc* __restrict__ fz = nullptr,
+                                                            const double* __restrict__ nrm2 = nullptr) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nrm2) {
+        const double hn = sqrt(*nrm2);
+        a = hn > 0.0 ? 1.0 / hn : 0.0;
+    }
     if (id < n) {
         const double v = a * x[id];
         y[id] = v;

@@ -372,3 +372,32 @@ def test_fp32_z_basis_gives_the_bits_of_the_fp64_copy(n_cell, cycle, monkeypatch
         its.append(s.info.iters)
     assert its[0] == its[1]
     assert torch.equal(xs[0], xs[1])
+
+
+@pytest.mark.parametrize("n_cell,cycle,restart", [(32, "F", 10), (40, "F", 10), (40, "V", 4), (40, "F", 3)])
+def test_steps_launched_ahead_give_the_bits_of_the_synchronous_loop(n_cell, cycle, restart, monkeypatch):
+    """With the delayed passes the next FGMRES step is launched before the
+    host has read the last one's coefficients (its scale w / |w| is read on
+    the device).  A step launched ahead is kept, or discarded when the cycle
+    ends (restarts of 3 and 4 end cycles often): the solve is the
+    synchronous loop's (ALAMO_EL_AHEAD=0), bit for bit, in FP64 and with the
+    FP32 cycles."""
+    from alamo_amd import elastic as el
+    p = P.polycrystal(n_cell, ngrains=16, seed=7)
+    _, op, rhs = gpu_setup(p)
+    for prec in ("fp64", "fp32"):
+        kw = dict(el.FAST_PATH, cycle=cycle, max_fmg_iter=2, restart=restart)
+        if prec == "fp64":
+            kw["precision"] = "fp64"
+        xs, its = [], []
+        for ahead in ("0", "1"):
+            monkeypatch.setenv("ALAMO_EL_AHEAD", ahead)
+            x = torch.zeros_like(rhs)
+            s = el.Linear(op, max_iter=400, **kw)
+            s.solve(x, rhs, 1e-8, 0.0)
+            assert s.info.status == 0
+            xs.append(x)
+            its.append(s.info.iters)
+        print(f"{n_cell}^3 {cycle} restart {restart} {prec}: {its[0]} steps")
+        assert its[0] == its[1]
+        assert torch.equal(xs[0], xs[1])

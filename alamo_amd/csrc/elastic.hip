@@ -2027,6 +2027,14 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
     // ALAMO_EL_Z=fp64 keeps the FP32 path's Z basis in FP64 (for comparison)
     const char* z_env = std::getenv("ALAMO_EL_Z");
     const bool z32 = !(z_env && std::strcmp(z_env, "fp64") == 0);
+    // With the delayed passes and FP32 cycles the basis V is kept in FP32 as
+    // well, MPf apart: both passes read it every step, and the sums stay FP64
+    // (ALAMO_EL_V=fp64 keeps it in FP64, for comparison).  The Arnoldi
+    // relation then holds to FP32 rounding relative to the residual at the
+    // restart, far below what one cycle of 10 steps reduces it by; the true
+    // residual is formed in FP64 at every restart and at the end.
+    const char* v_env = std::getenv("ALAMO_EL_V");
+    const bool v32 = lag && fp32 && z32 && !(v_env && std::strcmp(v_env, "fp64") == 0);
     int rc;
     iters = 0;
     // r = b - A x is in l0.d.res on entry
@@ -2081,12 +2089,17 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                 // widening is exact, so every product with it is the one with
                 // its FP64 copy
                 float* Zf = reinterpret_cast<float*>(h->kz) + (long)jj * MPf;
-                if (fz)
-                    ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0, l0.f.diag,
-                                                                             (float)h->omega, l0.f.tmp, nrm2);
-                else
-                    ael::k_scale_copy_cast<float><<<nblk(M0), kBlock, 0, s>>>(src, scale, Vj, l0.f.res, M0, nullptr, 0.f,
-                                                                             nullptr, nrm2);
+                auto copy_cast = [&](auto* V) {
+                    using TY = std::remove_pointer_t<decltype(V)>;
+                    if (fz)
+                        ael::k_scale_copy_cast<float, TY><<<nblk(M0), kBlock, 0, s>>>(
+                            src, scale, V, l0.f.res, M0, l0.f.diag, (float)h->omega, l0.f.tmp, nrm2);
+                    else
+                        ael::k_scale_copy_cast<float, TY><<<nblk(M0), kBlock, 0, s>>>(src, scale, V, l0.f.res, M0,
+                                                                                      nullptr, 0.f, nullptr, nrm2);
+                };
+                if (v32) copy_cast(reinterpret_cast<float*>(h->kv) + (long)jj * MPf);
+                else copy_cast(Vj);
                 if (held) {
                     c32.run(cyc, fz);
                     ael::k_cast_sum<float, float><<<nblk(M0), kBlock, 0, s>>>(l0.f.cor, l0.f.cor_hold, Zf, M0);
@@ -2120,18 +2133,23 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
         // DCGS2's two passes over V for step jj, then |w|^2, and the copy of
         // the coefficients into the host buffer of jj's parity; the event of
         // that parity marks the copy done.
-        auto launch_dcgs2 = [&](int jj) -> int {
+        auto dcgs2_passes = [&](auto* V, long ld, int jj) {
+            using TV = std::remove_pointer_t<decltype(V)>;
             const int n = jj + 1;
             if (jj == 0) {
-                ael::k_multidot_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, 1, M0, MP, h->kw, h->kpart);
+                ael::k_multidot_partial<TV><<<kDotBlocks, kBlock, 0, s>>>(V, 1, M0, ld, h->kw, h->kpart);
                 ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, h->kh);
-                ael::k_multiaxpy_dot<false><<<kDotBlocks, kBlock, 0, s>>>(h->kw, h->kv, 1, M0, MP, h->kh, h->kpart);
+                ael::k_multiaxpy_dot<false, TV><<<kDotBlocks, kBlock, 0, s>>>(h->kw, V, 1, M0, ld, h->kh, h->kpart);
             } else {
                 double* sums = h->kh + 2 * (m + 1) + 2;  // the 2n sums of pass 1
-                ael::k_multidot2_partial<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, h->kpart);
+                ael::k_multidot2_partial<TV><<<kDotBlocks, kBlock, 0, s>>>(V, n, M0, ld, h->kw, h->kpart);
                 ael::k_multidot_final<<<2 * n, kBlock, 0, s>>>(h->kpart, kDotBlocks, sums);
-                ael::k_dcgs2_update<<<kDotBlocks, kBlock, 0, s>>>(h->kv, n, M0, MP, h->kw, sums, h->kh, h->kpart);
+                ael::k_dcgs2_update<TV><<<kDotBlocks, kBlock, 0, s>>>(V, n, M0, ld, h->kw, sums, h->kh, h->kpart);
             }
+        };
+        auto launch_dcgs2 = [&](int jj) -> int {
+            if (v32) dcgs2_passes(reinterpret_cast<float*>(h->kv), MPf, jj);
+            else dcgs2_passes(h->kv, MP, jj);
             ael::k_multidot_final<<<1, kBlock, 0, s>>>(h->kpart, kDotBlocks, nrm_slot);
             const long nv = 2 * (m + 1) + 2;
             HIPCHK(hipMemcpyAsync(h->red_host_vec + (jj & 1) * nv, h->kh, sizeof(double) * nv, hipMemcpyDeviceToHost,

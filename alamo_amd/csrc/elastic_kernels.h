@@ -1001,8 +1001,8 @@ __global__ void __launch_bounds__(kBlock) k_cast(const Tin* __restrict__ a, Tout
 // With nrm2, a is 1 / sqrt(*nrm2) (0 when *nrm2 is 0), read on the device:
 // the scale of the next Krylov vector, w / |w|, can then be launched before
 // the host has read |w|^2.
-template <typename Tc>
-__global__ void __launch_bounds__(kBlock) k_scale_copy_cast(const double* x, double a, double* __restrict__ y, Tc* yc,
+template <typename Tc, typename TY = double>
+__global__ void __launch_bounds__(kBlock) k_scale_copy_cast(const double* x, double a, TY* __restrict__ y, Tc* yc,
                                                             long n, const Tc* __restrict__ diag = nullptr,
                                                             Tc omega = Tc(0), Tc* __restrict__ fz = nullptr,
                                                             const double* __restrict__ nrm2 = nullptr) {
@@ -1013,7 +1013,7 @@ __global__ void __launch_bounds__(kBlock) k_scale_copy_cast(const double* x, dou
     }
     if (id < n) {
         const double v = a * x[id];
-        y[id] = v;
+        y[id] = (TY)v;
         const Tc c = (Tc)v;
         yc[id] = c;
         if (fz) fz[id] = omega * c / diag[id];
@@ -1486,7 +1486,8 @@ __global__ void __launch_bounds__(kBlock)
 // registers, so one pass streams V and w once; the block then reduces each
 // sum through the waves (shuffles) and LDS.
 constexpr int kMultiDotMax = 12;
-__global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __restrict__ V, int k, long M, long ld,
+template <typename TV = double>
+__global__ void __launch_bounds__(kBlock) k_multidot_partial(const TV* __restrict__ V, int k, long M, long ld,
                                                              const double* __restrict__ w,
                                                              double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64][kMultiDotMax];
@@ -1504,7 +1505,7 @@ __global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __res
 #pragma unroll
         for (int i = 0; i < kMultiDotMax; ++i)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+            for (int u = 0; u < 4; ++u) vq[i][u] = i < k ? (double)V[(long)i * ld + q + u * st] : 0.0;
 #pragma unroll
         for (int i = 0; i < kMultiDotMax; ++i)
 #pragma unroll
@@ -1515,7 +1516,7 @@ __global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __res
         const double wq = w[q];
 #pragma unroll
         for (int i = 0; i < kMultiDotMax; ++i)
-            if (i < k) acc[i] += V[(long)i * ld + q] * wq;
+            if (i < k) acc[i] += (double)V[(long)i * ld + q] * wq;
     }
     const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
 #pragma unroll
@@ -1542,8 +1543,8 @@ __global__ void __launch_bounds__(kBlock) k_multidot_partial(const double* __res
 // each slot holding gridDim.x block partials: the projection and the dots
 // or the norm of classical Gram-Schmidt's next step read V once instead of
 // twice.  Grid-stride, four strides of independent loads per step.
-template <bool DOTS>
-__global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w, const double* __restrict__ V, int k,
+template <bool DOTS, typename TV = double>
+__global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w, const TV* __restrict__ V, int k,
                                                           long M, long ld, const double* __restrict__ h,
                                                           double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64][kMultiDotMax + 1];
@@ -1562,7 +1563,7 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w
 #pragma unroll
         for (int i = 0; i < kMultiDotMax; ++i)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+            for (int u = 0; u < 4; ++u) vq[i][u] = i < k ? (double)V[(long)i * ld + q + u * st] : 0.0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             double a = wq[u];
@@ -1582,7 +1583,7 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w
         double a = w[q];
         double vq[kMultiDotMax];
 #pragma unroll
-        for (int i = 0; i < kMultiDotMax; ++i) vq[i] = i < k ? V[(long)i * ld + q] : 0.0;
+        for (int i = 0; i < kMultiDotMax; ++i) vq[i] = i < k ? (double)V[(long)i * ld + q] : 0.0;
 #pragma unroll
         for (int i = 0; i < kMultiDotMax; ++i)
             if (i < k) a += -hc[i] * vq[i];
@@ -1629,17 +1630,22 @@ __global__ void __launch_bounds__(kBlock) k_multiaxpy_dot(double* __restrict__ w
 //     w <- w - sum_{i<n-1} c_i V_i - c_{n-1} V_{n-1}, with |w|^2.
 // V is read twice a step where CGS2 reads it three times.  n <= kMultiDotMax.
 //
+// V may hold FP32 vectors (TV = float): they are widened exactly, and every
+// sum is formed in FP64.
+//
 // Pass 1: slot i < n of the partials holds Q_i, slot n + i holds R_i.
-__global__ void __launch_bounds__(kBlock) k_multidot2_partial(const double* __restrict__ V, int n, long M, long ld,
+template <typename TV = double>
+__global__ void __launch_bounds__(kBlock) k_multidot2_partial(const TV* __restrict__ V, int n, long M, long ld,
                                                               const double* __restrict__ w,
                                                               double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64][2 * kMultiDotMax];
-    const double* __restrict__ t = V + (long)(n - 1) * ld;
+    const TV* __restrict__ t = V + (long)(n - 1) * ld;
     const int k = n - 1;  // vectors before t
     double aw[kMultiDotMax - 1], at[kMultiDotMax - 1], awl = 0.0, atl = 0.0;
 #pragma unroll
     for (int i = 0; i < kMultiDotMax - 1; ++i) aw[i] = at[i] = 0.0;
-    // two grid strides per step with independent loads (four held 256 VGPRs)
+    // two grid strides per step with independent loads (four held 256 VGPRs
+    // with an FP64 basis, and ran at half speed with an FP32 one)
     constexpr int kU = 2;
     const long st = (long)gridDim.x * blockDim.x;
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1648,12 +1654,12 @@ __global__ void __launch_bounds__(kBlock) k_multidot2_partial(const double* __re
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             wq[u] = w[q + u * st];
-            tq[u] = t[q + u * st];
+            tq[u] = (double)t[q + u * st];
         }
 #pragma unroll
         for (int i = 0; i < kMultiDotMax - 1; ++i)
 #pragma unroll
-            for (int u = 0; u < kU; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+            for (int u = 0; u < kU; ++u) vq[i][u] = i < k ? (double)V[(long)i * ld + q + u * st] : 0.0;
 #pragma unroll
         for (int i = 0; i < kMultiDotMax - 1; ++i)
 #pragma unroll
@@ -1669,11 +1675,11 @@ __global__ void __launch_bounds__(kBlock) k_multidot2_partial(const double* __re
         }
     }
     for (; q < M; q += st) {
-        const double wq = w[q], tq = t[q];
+        const double wq = w[q], tq = (double)t[q];
 #pragma unroll
         for (int i = 0; i < kMultiDotMax - 1; ++i)
             if (i < k) {
-                const double v = V[(long)i * ld + q];
+                const double v = (double)V[(long)i * ld + q];
                 aw[i] += v * wq;
                 at[i] += v * tq;
             }
@@ -1716,12 +1722,15 @@ __global__ void __launch_bounds__(kBlock) k_multidot2_partial(const double* __re
 // the 2n sums of pass 1 (Q, then R); every thread forms the coefficients from
 // them, and workgroup 0 stores them for the host: coef[0 .. n-2] = P,
 // coef[n-1] = 1/alpha, coef[n .. 2n-1] = c, coef[2n] = alpha (0 when t lies in
-// the span of the other vectors).
-__global__ void __launch_bounds__(kBlock) k_dcgs2_update(double* __restrict__ V, int n, long M, long ld,
+// the span of the other vectors).  With FP32 vectors the rewritten V_{n-1}
+// is rounded to FP32 and w is projected on the rounded vector, the one the
+// later passes read.
+template <typename TV = double>
+__global__ void __launch_bounds__(kBlock) k_dcgs2_update(TV* __restrict__ V, int n, long M, long ld,
                                                          double* __restrict__ w, const double* __restrict__ d,
                                                          double* __restrict__ coef, double* __restrict__ partial) {
     __shared__ double sh[kBlock / 64];
-    double* __restrict__ t = V + (long)(n - 1) * ld;
+    TV* __restrict__ t = V + (long)(n - 1) * ld;
     const int k = n - 1;
     double p[kMultiDotMax - 1], c[kMultiDotMax - 1];
     double pp = 0.0, pq = 0.0;
@@ -1755,12 +1764,12 @@ __global__ void __launch_bounds__(kBlock) k_dcgs2_update(double* __restrict__ V,
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             wq[u] = w[q + u * st];
-            tq[u] = t[q + u * st];
+            tq[u] = (double)t[q + u * st];
         }
 #pragma unroll
         for (int i = 0; i < kMultiDotMax - 1; ++i)
 #pragma unroll
-            for (int u = 0; u < kU; ++u) vq[i][u] = i < k ? V[(long)i * ld + q + u * st] : 0.0;
+            for (int u = 0; u < kU; ++u) vq[i][u] = i < k ? (double)V[(long)i * ld + q + u * st] : 0.0;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             double v = tq[u], a = wq[u];
@@ -1770,25 +1779,25 @@ __global__ void __launch_bounds__(kBlock) k_dcgs2_update(double* __restrict__ V,
                     v += -p[i] * vq[i][u];
                     a += -c[i] * vq[i][u];
                 }
-            v *= inv;
-            a += -cl * v;
-            t[q + u * st] = v;
+            const TV vs = (TV)(v * inv);
+            a += -cl * (double)vs;
+            t[q + u * st] = vs;
             w[q + u * st] = a;
             nrm += a * a;
         }
     }
     for (; q < M; q += st) {
-        double v = t[q], a = w[q];
+        double v = (double)t[q], a = w[q];
 #pragma unroll
         for (int i = 0; i < kMultiDotMax - 1; ++i)
             if (i < k) {
-                const double vi = V[(long)i * ld + q];
+                const double vi = (double)V[(long)i * ld + q];
                 v += -p[i] * vi;
                 a += -c[i] * vi;
             }
-        v *= inv;
-        a += -cl * v;
-        t[q] = v;
+        const TV vs = (TV)(v * inv);
+        a += -cl * (double)vs;
+        t[q] = vs;
         w[q] = a;
         nrm += a * a;
     }

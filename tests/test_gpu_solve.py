@@ -323,7 +323,9 @@ def test_delayed_reorthogonalization_matches_cgs2(precision, restart, n_cell, mo
     most one, and both solutions lie within 1e-5 relative L2 of an FP64 solve
     to 1e-11 (the solves stop at 1e-8 relative residual).  Restart 3 runs
     several restart cycles (the last column of each is kept as projected once);
-    12 fills the kernels' vector limit."""
+    12 fills the kernels' vector limit.  The basis is FP64 in both
+    (ALAMO_EL_V=fp64; the FP32 basis has a test of its own)."""
+    monkeypatch.setenv("ALAMO_EL_V", "fp64")
     p = P.polycrystal(n_cell, ngrains=16, seed=3)
     el, op, rhs = gpu_setup(p)
     x64 = torch.zeros_like(rhs)
@@ -353,8 +355,10 @@ def test_fp32_z_basis_gives_the_bits_of_the_fp64_copy(n_cell, cycle, monkeypatch
     them directly (tiled levels, 40^3 here) or through an FP64 copy (slab
     levels, 32^3), a V-cycle writes its correction straight into Z, and the
     solution update widens them.  The solve is the one with Z stored in FP64
-    (ALAMO_EL_Z=fp64), bit for bit: same iterations, same solution."""
+    (ALAMO_EL_Z=fp64), bit for bit: same iterations, same solution.  (The
+    basis V stays FP64 in both, ALAMO_EL_V=fp64: an FP64 Z keeps it FP64.)"""
     from alamo_amd import elastic as el
+    monkeypatch.setenv("ALAMO_EL_V", "fp64")
     p = P.polycrystal(n_cell, ngrains=16, seed=5)
     _, op, rhs = gpu_setup(p)
     kw = dict(el.FAST_PATH, cycle=cycle, max_fmg_iter=2)
@@ -401,3 +405,38 @@ def test_steps_launched_ahead_give_the_bits_of_the_synchronous_loop(n_cell, cycl
         print(f"{n_cell}^3 {cycle} restart {restart} {prec}: {its[0]} steps")
         assert its[0] == its[1]
         assert torch.equal(xs[0], xs[1])
+
+
+@pytest.mark.parametrize("n_cell,cycle,restart,steps", [(32, "F", 10, 1), (40, "F", 10, 1), (40, "V", 10, 1),
+                                                         (40, "F", 3, 1), (40, "F", 12, 2)])
+def test_fp32_krylov_basis_reaches_the_fp64_target(n_cell, cycle, restart, steps, monkeypatch):
+    """With the FP32 cycles and the delayed passes the Krylov basis V is kept
+    in FP32 (sums in FP64).  Against the FP64 basis (ALAMO_EL_V=fp64) on the
+    same problem: both meet the FP64 residual target, both solutions lie
+    within 1e-6 relative L2 of an FP64 solve to 1e-11, and the step counts
+    differ by at most one (two at restart 12, whose longer cycles lose more
+    to the FP32 rounding of the basis).  Restart 3 runs many restart cycles,
+    each starting from the true FP64 residual."""
+    from alamo_amd import elastic as el
+    p = P.polycrystal(n_cell, ngrains=16, seed=9)
+    _, op, rhs = gpu_setup(p)
+    x64 = torch.zeros_like(rhs)
+    el.Linear(op, outer="fgmres").solve(x64, rhs, 1e-11, 0.0)
+    kw = dict(el.FAST_PATH, cycle=cycle, max_fmg_iter=2, restart=restart)
+    got = {}
+    for v in ("fp64", "fp32"):
+        if v == "fp64":
+            monkeypatch.setenv("ALAMO_EL_V", "fp64")
+        else:
+            monkeypatch.delenv("ALAMO_EL_V", raising=False)
+        x = torch.zeros_like(rhs)
+        s = el.Linear(op, max_iter=400, **kw)
+        s.solve(x, rhs, 1e-8, 0.0)
+        target = 1e-8 * max(s.info.rhsnorm, s.info.res0)
+        assert s.info.status == 0
+        assert op.norm_inf(op.residual(x, rhs)) <= target * (1 + 1e-9)
+        e = rel_l2(host(x), host(x64))
+        print(f"{n_cell}^3 {cycle} restart {restart} V {v}: {s.info.iters} steps, rel L2 to FP64 {e:.2e}")
+        assert e < 1e-6
+        got[v] = s.info.iters
+    assert abs(got["fp64"] - got["fp32"]) <= steps, got

@@ -202,14 +202,15 @@ enum { kVariantSlab = 0, kVariantSlabNoMask = 1, kVariantNode = 2, kVariantSlabO
 // no faster).
 using TileT = ael::Tile<32, 8>;
 constexpr long kNtMinBytes = 64l << 20;
-// Chunks of k planes sized so that the tile workgroups fill about two
-// rounds of the resident ones: 2 workgroups per CU in FP64 (200-220 VGPRs),
-// 4 in FP32 (110-120 VGPRs).  At 129^3: FP64 chunks of 8 planes (1024
-// workgroups; Fapply 104.4 us vs 108.4 with 3 planes, 107.0 with 4, 108.8
-// with 16), FP32 chunks of 4.  ALAMO_EL_TILE_TARGET (read when a level's C
-// is set) replaces both targets: the tests use it to reach every chunk length
+// Chunks of k planes sized for the resident tile workgroups: two rounds of
+// 2 per CU in FP64 (200-220 VGPRs), one round of 4 per CU in FP32 (110-120
+// VGPRs).  At 129^3: FP64 chunks of 8 planes (1024 workgroups; Fapply
+// 104.4 us vs 108.4 with 3 planes, 107.0 with 4, 108.8 with 16), FP32 chunks
+// of 8 (sweep 37.2 us vs 39.2 with chunks of 4, 41.4 with 16; the FP32
+// solve 0.15 ms faster).  ALAMO_EL_TILE_TARGET (read when a level's C is
+// set) replaces both targets: the tests use it to reach every chunk length
 // on small boxes.
-constexpr long kTileTarget64 = 2 * 2 * 256, kTileTarget32 = 2 * 4 * 256;
+constexpr long kTileTarget64 = 2 * 2 * 256, kTileTarget32 = 4 * 256;
 ael::TileGrid tile_grid_of(const Lvl& L, long target) {
     const char* e = std::getenv("ALAMO_EL_TILE_TARGET");
     const long t = e ? std::atol(e) : 0;

@@ -175,7 +175,10 @@ struct alamo_el_s {
 namespace {
 
 constexpr int kRedBlocks = 1024;
-constexpr int kDotBlocks = 2048;  // multi-dot partials: 8 waves per SIMD over the 256 CUs
+// multi-dot partials: 4 waves per SIMD over the 256 CUs, one resident round
+// (FP32 solve at 129^3: 23.9 ms against 24.1 with 2048, 24.2 with 4096,
+// 24.4 with 768 and 25.6 with 512)
+constexpr int kDotBlocks = 1024;
 constexpr int kDenseMax = 1536;  // unknowns; LDS of the dense BiCGStab = 9 M doubles
 constexpr int kDirectMax = 4096; // unknowns of the direct bottom level (an M x M FP64 inverse)
 // Weights of the red-black Gauss-Seidel smoother.  Interior rows: 0.9; the

@@ -2093,16 +2093,22 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                 // widening is exact, so every product with it is the one with
                 // its FP64 copy
                 float* Zf = reinterpret_cast<float*>(h->kz) + (long)jj * MPf;
+                // A V-cycle only reads its level-0 right-hand side, so with an
+                // FP32 basis it reads V_j itself (the same FP32 values) and no
+                // copy is written; an F-cycle swaps its level-0 residual
+                // buffers and takes the copy.
+                float* Vf = v32 ? reinterpret_cast<float*>(h->kv) + (long)jj * MPf : nullptr;
+                float* rhs32 = (v32 && !held) ? nullptr : l0.f.res;
                 auto copy_cast = [&](auto* V) {
                     using TY = std::remove_pointer_t<decltype(V)>;
                     if (fz)
                         ael::k_scale_copy_cast<float, TY><<<nblk(M0), kBlock, 0, s>>>(
-                            src, scale, V, l0.f.res, M0, l0.f.diag, (float)h->omega, l0.f.tmp, nrm2);
+                            src, scale, V, rhs32, M0, l0.f.diag, (float)h->omega, l0.f.tmp, nrm2);
                     else
-                        ael::k_scale_copy_cast<float, TY><<<nblk(M0), kBlock, 0, s>>>(src, scale, V, l0.f.res, M0,
+                        ael::k_scale_copy_cast<float, TY><<<nblk(M0), kBlock, 0, s>>>(src, scale, V, rhs32, M0,
                                                                                       nullptr, 0.f, nullptr, nrm2);
                 };
-                if (v32) copy_cast(reinterpret_cast<float*>(h->kv) + (long)jj * MPf);
+                if (v32) copy_cast(Vf);
                 else copy_cast(Vj);
                 if (held) {
                     c32.run(cyc, fz);
@@ -2110,9 +2116,12 @@ int fgmres_solve(alamo_el_t h, double* x, const double* b, double target, const 
                 } else {
                     // a V-cycle writes its level-0 correction straight into Z_j
                     float* cor = l0.f.cor;
+                    float* res = l0.f.res;
                     l0.f.cor = Zf;
+                    if (!rhs32) l0.f.res = Vf;
                     c32.run(cyc, fz);
                     l0.f.cor = cor;
+                    l0.f.res = res;
                 }
                 // w = A Z_j: the FP64 tiled Fapply reads the FP32 field;
                 // elsewhere an FP64 copy in l0.d.rescor (free during a step)

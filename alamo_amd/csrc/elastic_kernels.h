@@ -998,7 +998,8 @@ __global__ void __launch_bounds__(kBlock) k_cast(const Tin* __restrict__ a, Tout
 // y = a x (FP64) and yc = y in the cycle's precision, one pass (k_scale_copy
 // then k_cast); yc may be x itself.  With fz, also the first Jacobi sweep
 // from zero on yc, fz = w yc / D (k_jacobi_from_zero's arithmetic).
-// With nrm2, a is 1 / sqrt(*nrm2) (0 when *nrm2 is 0), read on the device:
+// yc may be null (no copy in the cycle's precision).  With nrm2, a is
+// 1 / sqrt(*nrm2) (0 when *nrm2 is 0), read on the device:
 // the scale of the next Krylov vector, w / |w|, can then be launched before
 // the host has read |w|^2.
 template <typename Tc, typename TY = double>
@@ -1015,7 +1016,7 @@ __global__ void __launch_bounds__(kBlock) k_scale_copy_cast(const double* x, dou
         const double v = a * x[id];
         y[id] = (TY)v;
         const Tc c = (Tc)v;
-        yc[id] = c;
+        if (yc) yc[id] = c;
         if (fz) fz[id] = omega * c / diag[id];
     }
 }
